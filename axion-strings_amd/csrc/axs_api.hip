@@ -1,0 +1,665 @@
+// axs_api.hip -- the extern "C" entry points of include/axs_gpu.h: context
+// lifecycle, state transfer, the step loop and the sampling calls.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <thread>
+
+#include "axs_internal.hpp"
+
+namespace axs {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+// coefficient table of build_coefficient_matrix (s_integrate.cpp:9-14), in
+// float as there, widened to double as the reference's data_t initialiser
+int stencil_coefficients(int stencil, int ndims, double *c)
+{
+    float nd = (float)ndims;
+    switch (stencil) {
+    case 0:
+        c[0] = (double)(-2.0f * nd);
+        c[1] = (double)1.0f;
+        return 2;
+    case 1:
+        c[0] = (double)(-5.0f / 2.0f * nd);
+        c[1] = (double)(4.0f / 3.0f);
+        c[2] = (double)(-1.0f / 12.0f);
+        return 3;
+    case 2:
+        c[0] = (double)(-49.0f / 18.0f * nd);
+        c[1] = (double)(3.0f / 2.0f);
+        c[2] = (double)(-3.0f / 20.0f);
+        c[3] = (double)(1.0f / 90.0f);
+        return 4;
+    case 3:
+        c[0] = (double)(-205.0f / 72.0f * nd);
+        c[1] = (double)(8.0f / 5.0f);
+        c[2] = (double)(-1.0f / 5.0f);
+        c[3] = (double)(8.0f / 315.0f);
+        c[4] = (double)(-1.0f / 560.0f);
+        return 5;
+    default:
+        return 0;
+    }
+}
+
+// s_integrate.cpp:308-334 with tau already advanced (:317)
+StepScalars make_scalars(const axs_params &p, double tau)
+{
+    StepScalars s{};
+    double dx = (double)p.space_step;
+    s.dt = (double)p.time_step;
+    s.alpha = (double)1.0f / (dx * dx);
+    s.cl = ((double)(-1.0f) / (tau * tau)) * (double)p.lambda;
+    s.tau2 = tau * tau;
+    double T0 = (double)p.T_initial;
+    s.T03 = (T0 * T0) / (double)3.0f;
+    s.ncoef = stencil_coefficients(p.stencil_setting, p.ndims, s.c);
+    return s;
+}
+
+StepScalarsF to_float(const StepScalars &s)
+{
+    StepScalarsF f{};
+    f.dt = (float)s.dt;
+    f.half_dt = 0.5f * f.dt;
+    f.alpha = (float)s.alpha;
+    f.cl = (float)s.cl;
+    f.tau2 = (float)s.tau2;
+    f.T03 = (float)s.T03;
+    for (int i = 0; i < 5; ++i) f.c[i] = (float)s.c[i];
+    f.ncoef = s.ncoef;
+    return f;
+}
+
+int ensure_device(Context *c, void **buf, size_t *have, size_t bytes)
+{
+    if (bytes <= *have) return AXS_OK;
+    if (*buf) {
+        AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+        AXS_HIP_TRY(hipFree(*buf));
+        *buf = nullptr;
+        *have = 0;
+    }
+    size_t want = std::max(bytes, (size_t)1 << 20);
+    AXS_HIP_TRY(hipMalloc(buf, want));
+    *have = want;
+    return AXS_OK;
+}
+
+int ensure_scratch(Context *c, size_t bytes)
+{
+    return ensure_device(c, &c->scratch, &c->scratch_bytes, bytes);
+}
+
+int ensure_pinned(Context *c, size_t bytes)
+{
+    if (bytes <= c->pinned_bytes) return AXS_OK;
+    if (c->pinned) {
+        AXS_HIP_TRY(hipHostFree(c->pinned));
+        c->pinned = nullptr;
+        c->pinned_bytes = 0;
+    }
+    AXS_HIP_TRY(hipHostMalloc(&c->pinned, bytes, hipHostMallocDefault));
+    c->pinned_bytes = bytes;
+    return AXS_OK;
+}
+
+void parallel_for(size_t n, void (*fn)(size_t, size_t, void *), void *arg)
+{
+    unsigned hw = std::thread::hardware_concurrency();
+    unsigned nt = hw ? std::min(hw, 16u) : 4u;
+    if (n < ((size_t)1 << 20) || nt <= 1) {
+        fn(0, n, arg);
+        return;
+    }
+    std::vector<std::thread> ts;
+    size_t chunk = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        size_t b = t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        ts.emplace_back(fn, b, e, arg);
+    }
+    for (auto &t : ts) t.join();
+}
+
+struct ConvIn {
+    float *dst;
+    const double *a;
+    const double *k;
+    double half_dt;
+};
+
+static void conv_in_fn(size_t b, size_t e, void *arg)
+{
+    ConvIn *c = (ConvIn *)arg;
+    for (size_t i = b; i < e; ++i) {
+        double v = c->a ? c->a[i] : 0.0;
+        if (c->k) v = v + c->half_dt * c->k[i];
+        c->dst[i] = (float)v;
+    }
+}
+
+struct ConvOut {
+    double *dst;
+    const float *src;
+};
+
+static void conv_out_fn(size_t b, size_t e, void *arg)
+{
+    ConvOut *c = (ConvOut *)arg;
+    for (size_t i = b; i < e; ++i) c->dst[i] = (double)c->src[i];
+}
+
+void destroy_plans(Context *c);
+
+Context::~Context()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    destroy_plans(this);
+    for (Member &m : members) {
+        for (int h = 0; h < 2; ++h)
+            for (int a = 0; a < 6; ++a)
+                if (m.state[h][a]) (void)hipFree(m.state[h][a]);
+        for (int a = 0; a < 2; ++a) {
+            if (m.aux_ker[a]) (void)hipFree(m.aux_ker[a]);
+            if (m.aux_vel[a]) (void)hipFree(m.aux_vel[a]);
+        }
+    }
+    if (scratch) (void)hipFree(scratch);
+    if (listbuf) (void)hipFree(listbuf);
+    if (pinned) (void)hipHostFree(pinned);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int current_fields(Context *c, Member &m, FieldView *v)
+{
+    if (!m.has_state) {
+        set_error("member has no state: call axs_set_member_state first");
+        return AXS_ERR_STATE;
+    }
+    v->phi1 = m.state[c->cur][PHI1];
+    v->phi2 = m.state[c->cur][PHI2];
+    if (c->prec == AXS_FP64) {
+        v->vel1 = m.state[c->cur][VEL1];
+        v->vel2 = m.state[c->cur][VEL2];
+        return AXS_OK;
+    }
+    StepScalars s = make_scalars(c->p, c->tau);
+    AXS_TRY(launch_fp32_canonical(c, m, s, !m.ker_valid));
+    m.ker_valid = true;
+    v->vel1 = m.aux_vel[0];
+    v->vel2 = m.aux_vel[1];
+    return AXS_OK;
+}
+
+static int stencil_radius(int stencil) { return (stencil >= 0 && stencil <= 3) ? stencil + 1 : 0; }
+
+static int validate_params(const axs_params *p, int N)
+{
+    if (p->ndims != 2 && p->ndims != 3) {
+        set_error("ndims must be 2 or 3 (got %d)", p->ndims);
+        return AXS_ERR_INVALID_ARG;
+    }
+    if (N < 4 || (N % 2) != 0) {     // s_run.cpp:32 asserts N % 2 == 0
+        set_error("N must be even and >= 4 (got %d)", N);
+        return AXS_ERR_INVALID_ARG;
+    }
+    int r = stencil_radius(p->stencil_setting);
+    if (r == 0) {
+        set_error("stencil_setting must be 0..3 (got %d)", p->stencil_setting);
+        return AXS_ERR_UNSUPPORTED;
+    }
+    if (N < r) {
+        set_error("N=%d smaller than the stencil radius %d", N, r);
+        return AXS_ERR_INVALID_ARG;
+    }
+    double L = (p->ndims == 3) ? (double)N * N * N : (double)N * N;
+    if (L >= 2147483648.0) {
+        set_error("lattice of %.0f sites exceeds the 2^31 site limit", L);
+        return AXS_ERR_UNSUPPORTED;
+    }
+    if (p->precision != AXS_FP64 && p->precision != AXS_FP32) {
+        set_error("precision must be AXS_FP64 or AXS_FP32 (got %d)", p->precision);
+        return AXS_ERR_INVALID_ARG;
+    }
+    if (!(p->time_step > 0.0f) || !(p->space_step > 0.0f)) {
+        set_error("time_step and space_step must be positive");
+        return AXS_ERR_INVALID_ARG;
+    }
+    return AXS_OK;
+}
+
+static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ctx **out)
+{
+    if (!p || !out || n_members < 1 || !Ns) {
+        set_error("axs_create: null argument or empty batch");
+        return AXS_ERR_INVALID_ARG;
+    }
+    *out = nullptr;
+    for (int r = 0; r < n_members; ++r) AXS_TRY(validate_params(p, Ns[r]));
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_error("no HIP device available");
+        return AXS_ERR_NO_DEVICE;
+    }
+    if (p->device < 0 || p->device >= ndev) {
+        set_error("device %d out of range (have %d)", p->device, ndev);
+        return AXS_ERR_INVALID_ARG;
+    }
+    AXS_HIP_TRY(hipSetDevice(p->device));
+    hipDeviceProp_t prop;
+    AXS_HIP_TRY(hipGetDeviceProperties(&prop, p->device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        set_error("device %d is %s; this library is built for gfx950 only", p->device,
+                  prop.gcnArchName);
+        return AXS_ERR_NO_DEVICE;
+    }
+
+    axs_ctx *h = new axs_ctx();
+    Context *c = &h->impl;
+    c->p = *p;
+    c->nd = p->ndims;
+    c->prec = p->precision;
+    c->elem = (c->prec == AXS_FP64) ? 8 : 4;
+    c->device = p->device;
+    c->num_cu = prop.multiProcessorCount;
+    c->tau = 1.0;                                              // physics.cpp:32
+    int rc = AXS_OK;
+    auto fail = [&](int code) {
+        delete h;
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        set_error("stream/event creation failed");
+        return fail(AXS_ERR_HIP);
+    }
+    c->members.resize(n_members);
+    for (int r = 0; r < n_members; ++r) {
+        Member &m = c->members[r];
+        m.N = Ns[r];
+        m.L = (c->nd == 3) ? (int64_t)m.N * m.N * m.N : (int64_t)m.N * m.N;
+        size_t bytes = (size_t)m.L * c->elem;
+        int narr = (c->prec == AXS_FP64) ? 6 : 4;
+        for (int hh = 0; hh < 2; ++hh)
+            for (int a = 0; a < narr; ++a) {
+                if (hipMalloc(&m.state[hh][a], bytes) != hipSuccess) {
+                    set_error("device allocation of %zu bytes failed (member %d)", bytes, r);
+                    return fail(AXS_ERR_OUT_OF_MEMORY);
+                }
+                if (hipMemsetAsync(m.state[hh][a], 0, bytes, c->stream) != hipSuccess) {
+                    set_error("hipMemsetAsync failed");
+                    return fail(AXS_ERR_HIP);
+                }
+            }
+        if (c->prec == AXS_FP32) {
+            for (int a = 0; a < 2; ++a) {
+                if (hipMalloc(&m.aux_ker[a], bytes) != hipSuccess ||
+                    hipMalloc(&m.aux_vel[a], bytes) != hipSuccess) {
+                    set_error("device allocation of %zu bytes failed (member %d)", bytes, r);
+                    return fail(AXS_ERR_OUT_OF_MEMORY);
+                }
+                (void)hipMemsetAsync(m.aux_ker[a], 0, bytes, c->stream);
+                (void)hipMemsetAsync(m.aux_vel[a], 0, bytes, c->stream);
+            }
+        }
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        set_error("stream synchronize failed after allocation");
+        return fail(AXS_ERR_HIP);
+    }
+    (void)rc;
+    *out = h;
+    return AXS_OK;
+}
+
+static int check_member(axs_ctx *ctx, int member)
+{
+    if (!ctx) {
+        set_error("null context");
+        return AXS_ERR_INVALID_ARG;
+    }
+    if (member < 0 || member >= (int)ctx->impl.members.size()) {
+        set_error("member %d out of range (batch of %d)", member, (int)ctx->impl.members.size());
+        return AXS_ERR_INVALID_ARG;
+    }
+    AXS_HIP_TRY(hipSetDevice(ctx->impl.device));
+    return AXS_OK;
+}
+
+}  // namespace axs
+
+using namespace axs;
+
+extern "C" {
+
+int axs_create(const axs_params *p, axs_ctx **out)
+{
+    if (!p) {
+        set_error("axs_create: null params");
+        return AXS_ERR_INVALID_ARG;
+    }
+    int N = p->N;
+    return create_impl(p, 1, &N, out);
+}
+
+int axs_create_batch(const axs_params *p, int n_members, const int *N_members, axs_ctx **out)
+{
+    return create_impl(p, n_members, N_members, out);
+}
+
+int axs_destroy(axs_ctx *ctx)
+{
+    if (!ctx) return AXS_OK;
+    (void)hipSetDevice(ctx->impl.device);
+    delete ctx;
+    return AXS_OK;
+}
+
+int axs_num_members(const axs_ctx *ctx, int *n)
+{
+    if (!ctx || !n) {
+        set_error("null argument");
+        return AXS_ERR_INVALID_ARG;
+    }
+    *n = (int)ctx->impl.members.size();
+    return AXS_OK;
+}
+
+int axs_set_member_state(axs_ctx *ctx, int member, const double *phi1, const double *phi2,
+                         const double *phidot1, const double *phidot2, const double *ker1,
+                         const double *ker2)
+{
+    AXS_TRY(check_member(ctx, member));
+    if (!phi1 || !phi2 || !phidot1 || !phidot2) {
+        set_error("phi1, phi2, phidot1 and phidot2 are required");
+        return AXS_ERR_INVALID_ARG;
+    }
+    Context *c = &ctx->impl;
+    Member &m = c->members[member];
+    size_t L = (size_t)m.L;
+    int h = c->cur;
+    if (c->prec == AXS_FP64) {
+        size_t b = L * sizeof(double);
+        AXS_HIP_TRY(hipMemcpy(m.state[h][PHI1], phi1, b, hipMemcpyHostToDevice));
+        AXS_HIP_TRY(hipMemcpy(m.state[h][PHI2], phi2, b, hipMemcpyHostToDevice));
+        AXS_HIP_TRY(hipMemcpy(m.state[h][VEL1], phidot1, b, hipMemcpyHostToDevice));
+        AXS_HIP_TRY(hipMemcpy(m.state[h][VEL2], phidot2, b, hipMemcpyHostToDevice));
+        if (ker1)
+            AXS_HIP_TRY(hipMemcpy(m.state[h][KER1], ker1, b, hipMemcpyHostToDevice));
+        else
+            AXS_HIP_TRY(hipMemset(m.state[h][KER1], 0, b));
+        if (ker2)
+            AXS_HIP_TRY(hipMemcpy(m.state[h][KER2], ker2, b, hipMemcpyHostToDevice));
+        else
+            AXS_HIP_TRY(hipMemset(m.state[h][KER2], 0, b));
+    } else {
+        // FP32 form: phi -> float; u = phidot + dt/2 ker (in double, then
+        // rounded); ker kept for the state read-back before the first step
+        AXS_TRY(ensure_pinned(c, L * sizeof(float)));
+        float *st = (float *)c->pinned;
+        const double half_dt = 0.5 * (double)c->p.time_step;
+        struct Job {
+            void *dst;
+            const double *a;
+            const double *k;
+        } jobs[6] = {{m.state[h][PHI1], phi1, nullptr}, {m.state[h][PHI2], phi2, nullptr},
+                     {m.state[h][VEL1], phidot1, ker1},  {m.state[h][VEL2], phidot2, ker2},
+                     {m.aux_ker[0], ker1, nullptr},      {m.aux_ker[1], ker2, nullptr}};
+        for (int j = 0; j < 6; ++j) {
+            ConvIn ci{st, jobs[j].a, jobs[j].k, half_dt};
+            parallel_for(L, conv_in_fn, &ci);
+            AXS_HIP_TRY(hipMemcpy(jobs[j].dst, st, L * sizeof(float), hipMemcpyHostToDevice));
+        }
+        m.ker_valid = true;
+    }
+    m.has_state = true;
+    return AXS_OK;
+}
+
+int axs_get_member_state(axs_ctx *ctx, int member, double *phi1, double *phi2, double *phidot1,
+                         double *phidot2, double *ker1, double *ker2)
+{
+    AXS_TRY(check_member(ctx, member));
+    Context *c = &ctx->impl;
+    Member &m = c->members[member];
+    if (!m.has_state) {
+        set_error("member %d has no state", member);
+        return AXS_ERR_STATE;
+    }
+    size_t L = (size_t)m.L;
+    int h = c->cur;
+    if (c->prec == AXS_FP64) {
+        size_t b = L * sizeof(double);
+        double *dst[6] = {phi1, phi2, phidot1, phidot2, ker1, ker2};
+        for (int a = 0; a < 6; ++a)
+            if (dst[a])
+                AXS_HIP_TRY(hipMemcpyAsync(dst[a], m.state[h][a], b, hipMemcpyDeviceToHost,
+                                           c->stream));
+        AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+        return AXS_OK;
+    }
+    FieldView fv;
+    AXS_TRY(current_fields(c, m, &fv));      // computes ker and phidot
+    AXS_TRY(ensure_pinned(c, L * sizeof(float)));
+    float *st = (float *)c->pinned;
+    const void *src[6] = {m.state[h][PHI1], m.state[h][PHI2], m.aux_vel[0],
+                          m.aux_vel[1],     m.aux_ker[0],     m.aux_ker[1]};
+    double *dst[6] = {phi1, phi2, phidot1, phidot2, ker1, ker2};
+    for (int a = 0; a < 6; ++a) {
+        if (!dst[a]) continue;
+        AXS_HIP_TRY(hipMemcpyAsync(st, src[a], L * sizeof(float), hipMemcpyDeviceToHost,
+                                   c->stream));
+        AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+        ConvOut co{dst[a], st};
+        parallel_for(L, conv_out_fn, &co);
+    }
+    return AXS_OK;
+}
+
+int axs_init_member(axs_ctx *ctx, int member, int mode, uint64_t seed)
+{
+    AXS_TRY(check_member(ctx, member));
+    if (mode != AXS_INIT_KINV && mode != AXS_INIT_THERMAL) {
+        set_error("unknown init mode %d", mode);
+        return AXS_ERR_INVALID_ARG;
+    }
+    return run_init(&ctx->impl, ctx->impl.members[member], mode, seed);
+}
+
+int axs_set_tau(axs_ctx *ctx, double tau)
+{
+    if (!ctx) {
+        set_error("null context");
+        return AXS_ERR_INVALID_ARG;
+    }
+    ctx->impl.tau = tau;
+    // FP32: a stored ker no longer matches the new tau unless no step ran
+    return AXS_OK;
+}
+
+int axs_get_tau(const axs_ctx *ctx, double *tau)
+{
+    if (!ctx || !tau) {
+        set_error("null argument");
+        return AXS_ERR_INVALID_ARG;
+    }
+    *tau = ctx->impl.tau;
+    return AXS_OK;
+}
+
+int axs_set_state(axs_ctx *ctx, const double *phi1, const double *phi2, const double *phidot1,
+                  const double *phidot2, const double *ker1, const double *ker2, double tau)
+{
+    AXS_TRY(axs_set_member_state(ctx, 0, phi1, phi2, phidot1, phidot2, ker1, ker2));
+    return axs_set_tau(ctx, tau);
+}
+
+int axs_get_state(axs_ctx *ctx, double *phi1, double *phi2, double *phidot1, double *phidot2,
+                  double *ker1, double *ker2, double *tau)
+{
+    AXS_TRY(axs_get_member_state(ctx, 0, phi1, phi2, phidot1, phidot2, ker1, ker2));
+    if (tau) *tau = ctx->impl.tau;
+    return AXS_OK;
+}
+
+int axs_step(axs_ctx *ctx, int64_t nsteps)
+{
+    AXS_TRY(check_member(ctx, 0));
+    Context *c = &ctx->impl;
+    if (nsteps < 0) {
+        set_error("nsteps must be >= 0");
+        return AXS_ERR_INVALID_ARG;
+    }
+    for (Member &m : c->members)
+        if (!m.has_state) {
+            set_error("every member needs a state before axs_step");
+            return AXS_ERR_STATE;
+        }
+    if (nsteps == 0) return AXS_OK;
+    const double dt = (double)c->p.time_step;
+    AXS_HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    for (int64_t n = 0; n < nsteps; ++n) {
+        double tau_next = c->tau + dt;                    // s_integrate.cpp:317
+        StepScalars s = make_scalars(c->p, tau_next);
+        AXS_TRY(launch_step(c, s));
+        c->tau = tau_next;
+    }
+    AXS_HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    AXS_HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    AXS_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_step_ms = (double)ms / (double)nsteps;
+    for (Member &m : c->members) m.ker_valid = false;
+    return AXS_OK;
+}
+
+int axs_member_strings(axs_ctx *ctx, int member, int32_t *coords, int64_t cap, int64_t *count)
+{
+    AXS_TRY(check_member(ctx, member));
+    if (!count || cap < 0) {
+        set_error("count pointer required, cap >= 0");
+        return AXS_ERR_INVALID_ARG;
+    }
+    return run_strings(&ctx->impl, ctx->impl.members[member], coords, cap, count, nullptr);
+}
+
+int axs_strings(axs_ctx *ctx, int32_t *coords, int64_t cap, int64_t *count)
+{
+    return axs_member_strings(ctx, 0, coords, cap, count);
+}
+
+int axs_string_counts(axs_ctx *ctx, int64_t *n_plaquettes, int64_t *num_cores)
+{
+    AXS_TRY(check_member(ctx, 0));
+    Context *c = &ctx->impl;
+    for (size_t r = 0; r < c->members.size(); ++r) {
+        int64_t n = 0, nc = 0;
+        AXS_TRY(run_strings(c, c->members[r], nullptr, 0, &n, &nc));
+        if (n_plaquettes) n_plaquettes[r] = n;
+        if (num_cores) num_cores[r] = nc;
+    }
+    return AXS_OK;
+}
+
+int axs_member_background(axs_ctx *ctx, int member, double sums[6])
+{
+    AXS_TRY(check_member(ctx, member));
+    if (!sums) {
+        set_error("null output");
+        return AXS_ERR_INVALID_ARG;
+    }
+    return run_background(&ctx->impl, ctx->impl.members[member], sums);
+}
+
+int axs_background(axs_ctx *ctx, double sums[6]) { return axs_member_background(ctx, 0, sums); }
+
+int axs_member_energy(axs_ctx *ctx, int member, double out[3])
+{
+    AXS_TRY(check_member(ctx, member));
+    if (!out) {
+        set_error("null output");
+        return AXS_ERR_INVALID_ARG;
+    }
+    return run_energy(&ctx->impl, ctx->impl.members[member], out);
+}
+
+int axs_energy(axs_ctx *ctx, double out[3]) { return axs_member_energy(ctx, 0, out); }
+
+int axs_member_powerspec(axs_ctx *ctx, int member, int binning, int nbins, double *ks, double *pk,
+                         int32_t *count)
+{
+    AXS_TRY(check_member(ctx, member));
+    if (!ks || !pk || !count) {
+        set_error("null output");
+        return AXS_ERR_INVALID_ARG;
+    }
+    return run_powerspec(&ctx->impl, ctx->impl.members[member], binning, nbins, ks, pk, count);
+}
+
+int axs_powerspec(axs_ctx *ctx, int binning, int nbins, double *ks, double *pk, int32_t *count)
+{
+    return axs_member_powerspec(ctx, 0, binning, nbins, ks, pk, count);
+}
+
+int axs_last_step_ms(const axs_ctx *ctx, double *ms)
+{
+    if (!ctx || !ms) {
+        set_error("null argument");
+        return AXS_ERR_INVALID_ARG;
+    }
+    *ms = ctx->impl.last_step_ms;
+    return AXS_OK;
+}
+
+int axs_bytes_per_site(const axs_ctx *ctx, double *scheme_bytes, double *alg_bytes)
+{
+    if (!ctx) {
+        set_error("null context");
+        return AXS_ERR_INVALID_ARG;
+    }
+    const Context *c = &ctx->impl;
+    double w = (double)c->elem;
+    if (alg_bytes) *alg_bytes = 2.0 * 4.0 * w;     // read + write phi1, phi2, vel1, vel2
+    if (scheme_bytes)
+        *scheme_bytes = (c->prec == AXS_FP64) ? 2.0 * 6.0 * w : 2.0 * 4.0 * w;
+    return AXS_OK;
+}
+
+const char *axs_strerror(int status)
+{
+    switch (status) {
+    case AXS_OK: return "ok";
+    case AXS_ERR_INVALID_ARG: return "invalid argument";
+    case AXS_ERR_UNSUPPORTED: return "unsupported parameter";
+    case AXS_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case AXS_ERR_HIP: return "HIP runtime error";
+    case AXS_ERR_CAPACITY: return "output buffer too small";
+    case AXS_ERR_NO_DEVICE: return "no usable gfx950 device";
+    case AXS_ERR_FFT: return "rocFFT error";
+    case AXS_ERR_STATE: return "call out of order";
+    default: return "unknown status";
+    }
+}
+
+const char *axs_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

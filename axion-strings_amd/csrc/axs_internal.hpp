@@ -1,0 +1,154 @@
+// axs_internal.hpp -- shared declarations of the gfx950 implementation of
+// the Axion-Strings uniform-lattice path (context, device buffers, kernel
+// launchers).  Host code and kernels are written for CDNA4 (64-wide waves).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "axs_gpu.h"
+
+namespace axs {
+
+// ---------------------------------------------------------------------------
+// errors
+
+void set_error(const char *fmt, ...);
+
+struct Status {
+    int code;
+};
+
+#define AXS_HIP_TRY(expr)                                                     \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess) {                                               \
+            ::axs::set_error("%s failed: %s (%s:%d)", #expr,                  \
+                             hipGetErrorString(e_), __FILE__, __LINE__);      \
+            return (e_ == hipErrorOutOfMemory) ? AXS_ERR_OUT_OF_MEMORY        \
+                                               : AXS_ERR_HIP;                 \
+        }                                                                     \
+    } while (0)
+
+#define AXS_TRY(expr)                                                         \
+    do {                                                                      \
+        int s_ = (expr);                                                      \
+        if (s_ != AXS_OK) return s_;                                          \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// per-step uniform scalars.  Computed once per step on the host in double,
+// in the reference's expression order (s_integrate.cpp:308-334).
+
+struct StepScalars {
+    double dt;      // (double) parameters.time_step
+    double alpha;   // 1.0f / pow_2(dx)
+    double cl;      // (-1.0f / pow_2(tau)) * lambda, tau after the update
+    double tau2;    // pow_2(tau)
+    double T03;     // pow_2(T_initial) / 3.0f
+    double c[5];    // stencil coefficients, s_integrate.cpp:9-14
+    int ncoef;      // 2 for stencil_setting 0
+};
+
+struct StepScalarsF {
+    float dt, alpha, cl, tau2, T03, half_dt;
+    float c[5];
+    int ncoef;
+};
+
+int stencil_coefficients(int stencil, int ndims, double *c);
+StepScalars make_scalars(const axs_params &p, double tau_after);
+StepScalarsF to_float(const StepScalars &s);
+
+// ---------------------------------------------------------------------------
+// one lattice of the context
+
+enum ArrayId { PHI1 = 0, PHI2 = 1, VEL1 = 2, VEL2 = 3, KER1 = 4, KER2 = 5 };
+
+struct Member {
+    int N = 0;
+    int64_t L = 0;
+    // Ping-pong state.  FP64: phi1, phi2, phidot1, phidot2, ker1, ker2 (six
+    // arrays, the reference's all_data).  FP32: phi1, phi2, u1, u2 with
+    // u = phidot + dt/2 * ker (the half-step velocity); slots 4, 5 unused.
+    void *state[2][6] = {{nullptr}};
+    // FP32 only: ker (valid while ker_valid) and phidot scratch
+    void *aux_ker[2] = {nullptr, nullptr};
+    void *aux_vel[2] = {nullptr, nullptr};
+    bool has_state = false;
+    bool ker_valid = true;        // FP32: aux_ker holds ker of the current state
+};
+
+struct FftPlan;
+
+struct Context {
+    axs_params p{};
+    int nd = 2;
+    int prec = AXS_FP64;
+    int elem = 8;                 // bytes per field element
+    std::vector<Member> members;
+    int cur = 0;                  // ping-pong index holding the current state
+    double tau = 1.0;             // physics.cpp:32
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_step_ms = 0.0;
+    int device = 0;
+    int num_cu = 256;
+
+    // device scratch for reductions / compaction (grown on demand; growing
+    // discards the contents)
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // device buffer for string coordinate lists (separate from scratch so
+    // the compaction's counts survive its growth)
+    void *listbuf = nullptr;
+    size_t listbuf_bytes = 0;
+    // pinned host staging
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
+
+    std::vector<FftPlan *> fft_plans;
+
+    ~Context();
+};
+
+int ensure_scratch(Context *c, size_t bytes);
+int ensure_device(Context *c, void **buf, size_t *have, size_t bytes);
+int ensure_pinned(Context *c, size_t bytes);
+
+// host-side parallel loop over [0, n) in contiguous chunks (std::thread)
+void parallel_for(size_t n, void (*fn)(size_t begin, size_t end, void *arg), void *arg);
+
+// ---------------------------------------------------------------------------
+// launchers (return AXS_OK or an error code)
+
+// one step of every member; reads state[cur], writes state[cur ^ 1]
+int launch_step(Context *c, const StepScalars &s);
+
+// FP32: compute ker = F(phi, tau) and phidot = u - dt/2 ker into aux buffers
+int launch_fp32_canonical(Context *c, Member &m, const StepScalars &s, bool recompute_ker);
+
+int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *count,
+                int64_t *num_cores);
+int run_background(Context *c, Member &m, double out[6]);
+int run_init(Context *c, Member &m, int mode, uint64_t seed);
+int run_energy(Context *c, Member &m, double out[3]);
+int run_powerspec(Context *c, Member &m, int binning, int nbins, double *ks, double *pk,
+                  int32_t *count);
+
+// device pointers to the current (canonical) fields of a member as doubles
+// or floats, plus its velocity (phidot) -- for the FP32 form this triggers
+// the canonicalisation kernel.
+struct FieldView {
+    const void *phi1, *phi2, *vel1, *vel2;
+};
+int current_fields(Context *c, Member &m, FieldView *v);
+
+}  // namespace axs
+
+struct axs_ctx {
+    axs::Context impl;
+};

@@ -1,0 +1,66 @@
+// axs_lattice.hpp -- device-side lattice indexing shared by the kernels.
+//
+// Layout (reference src/common/common.hpp:28, 81-119):
+//   2D  m = i + N*j            axis 0 = i (stride 1), axis 1 = j (stride N)
+//   3D  m = (i*N + j)*N + k    axis 0 = i (stride N*N), axis 1 = j (stride N),
+//                              axis 2 = k (stride 1)
+// "Axis order" is the reference's COO insertion order of neighbours
+// (s_integrate.cpp:50-75, 105-142): for each distance l, +l then -l along
+// axis 0, then axis 1, then axis 2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace axs {
+
+__device__ __forceinline__ int wrap_coord(int x, int N)
+{
+    // valid for -N <= x < 2N, which holds for every stencil radius <= 4 on
+    // lattices with N >= 4 (checked at context creation)
+    x = (x >= N) ? x - N : x;
+    return (x < 0) ? x + N : x;
+}
+
+template <int ND>
+struct Site {
+    int c[3];        // coordinates in axis order
+    uint32_t m;      // linear index
+};
+
+template <int ND>
+__device__ __forceinline__ Site<ND> decompose(uint32_t m, int N)
+{
+    Site<ND> s;
+    s.m = m;
+    if (ND == 2) {
+        s.c[0] = (int)(m % (uint32_t)N);
+        s.c[1] = (int)(m / (uint32_t)N);
+        s.c[2] = 0;
+    } else {
+        uint32_t NN = (uint32_t)N * (uint32_t)N;
+        s.c[0] = (int)(m / NN);
+        uint32_t layer = m - (uint32_t)s.c[0] * NN;
+        s.c[1] = (int)(layer / (uint32_t)N);
+        s.c[2] = (int)(layer % (uint32_t)N);
+    }
+    return s;
+}
+
+template <int ND>
+__device__ __forceinline__ uint32_t axis_stride(int axis, int N)
+{
+    if (ND == 2) return axis == 0 ? 1u : (uint32_t)N;
+    return axis == 0 ? (uint32_t)N * (uint32_t)N : (axis == 1 ? (uint32_t)N : 1u);
+}
+
+// index of the site displaced by d along `axis` (periodic)
+template <int ND>
+__device__ __forceinline__ uint32_t shifted(const Site<ND> &s, int axis, int d, int N)
+{
+    int x = wrap_coord(s.c[axis] + d, N);
+    int delta = x - s.c[axis];
+    return (uint32_t)((int64_t)s.m + (int64_t)delta * (int64_t)axis_stride<ND>(axis, N));
+}
+
+}  // namespace axs

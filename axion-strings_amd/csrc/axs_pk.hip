@@ -1,0 +1,327 @@
+// axs_pk.hip -- power spectrum of phi2 (reference output_powerspec,
+// src/common/pk_estimation.cpp:37-214, called on phi2 at s_run.cpp:206).
+//
+// The reference multiplies the field by (-1)^(i+j+k) and takes a complex
+// forward FFT of the whole N^d array, so that array index i corresponds to
+// the signed frequency i - N/2.  The shift only relabels modes; here the
+// unshifted field goes through a rocFFT real-to-complex transform (half the
+// spectrum, half the memory) and each stored mode (fi, fj, fz >= 0) stands
+// for itself and, when 0 < fz < N/2, for its Hermitian partner -k with the
+// same |F|^2 and |k| (weight 2).
+//
+// Binning is done per bin rather than per mode: a bin is a range [lo, hi)
+// of the integer |k|^2, so for every (fi, fj) row the modes that fall in
+// the bin are one contiguous run of fz.  One workgroup owns one bin, walks
+// the rows of the bin's bounding box in a fixed order and sums its runs in
+// compensated FP64 -- deterministic, with no atomics.
+#include <rocfft/rocfft.h>
+
+#include <math.h>
+
+#include "axs_internal.hpp"
+
+namespace axs {
+
+struct FftPlan {
+    int N = 0, nd = 0;
+    rocfft_plan plan = nullptr;
+    rocfft_execution_info info = nullptr;
+    void *work = nullptr;
+    size_t work_bytes = 0;
+    double *in = nullptr;       // real input copy (rocFFT may overwrite input)
+    double *out = nullptr;      // complex half spectrum, interleaved
+    ~FftPlan()
+    {
+        if (plan) rocfft_plan_destroy(plan);
+        if (info) rocfft_execution_info_destroy(info);
+        if (work) (void)hipFree(work);
+        if (in) (void)hipFree(in);
+        if (out) (void)hipFree(out);
+    }
+};
+
+static int rocfft_ready = 0;
+
+static int get_plan(Context *c, int N, FftPlan **out)
+{
+    for (FftPlan *p : c->fft_plans)
+        if (p->N == N && p->nd == c->nd) {
+            *out = p;
+            return AXS_OK;
+        }
+    if (!rocfft_ready) {
+        if (rocfft_setup() != rocfft_status_success) {
+            set_error("rocfft_setup failed");
+            return AXS_ERR_FFT;
+        }
+        rocfft_ready = 1;
+    }
+    FftPlan *p = new FftPlan();
+    p->N = N;
+    p->nd = c->nd;
+    size_t lengths[3] = {(size_t)N, (size_t)N, (size_t)N};  // fastest first
+    size_t L = (c->nd == 3) ? (size_t)N * N * N : (size_t)N * N;
+    size_t half = L / N * (N / 2 + 1);
+    if (rocfft_plan_create(&p->plan, rocfft_placement_notinplace,
+                           rocfft_transform_type_real_forward, rocfft_precision_double,
+                           (size_t)c->nd, lengths, 1, nullptr) != rocfft_status_success) {
+        delete p;
+        set_error("rocfft_plan_create failed for N=%d ndims=%d", N, c->nd);
+        return AXS_ERR_FFT;
+    }
+    if (rocfft_plan_get_work_buffer_size(p->plan, &p->work_bytes) != rocfft_status_success ||
+        rocfft_execution_info_create(&p->info) != rocfft_status_success) {
+        delete p;
+        set_error("rocfft execution info setup failed");
+        return AXS_ERR_FFT;
+    }
+    hipError_t e = hipMalloc(&p->in, L * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&p->out, half * 2 * sizeof(double));
+    if (e == hipSuccess && p->work_bytes) e = hipMalloc(&p->work, p->work_bytes);
+    if (e != hipSuccess) {
+        delete p;
+        set_error("power-spectrum buffers: %s", hipGetErrorString(e));
+        return AXS_ERR_OUT_OF_MEMORY;
+    }
+    if (p->work_bytes &&
+        rocfft_execution_info_set_work_buffer(p->info, p->work, p->work_bytes) !=
+            rocfft_status_success) {
+        delete p;
+        set_error("rocfft_execution_info_set_work_buffer failed");
+        return AXS_ERR_FFT;
+    }
+    c->fft_plans.push_back(p);
+    *out = p;
+    return AXS_OK;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) to_double_kernel(uint32_t L, const T *__restrict__ x,
+                                                        double *__restrict__ y)
+{
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < L) y[m] = (double)x[m];
+}
+
+struct KahanP {
+    double s, c;
+};
+
+__device__ __forceinline__ void kaddp(KahanP &k, double x)
+{
+    double t = k.s + x;
+    if (fabs(k.s) >= fabs(x))
+        k.c += (k.s - t) + x;
+    else
+        k.c += (x - t) + k.s;
+    k.s = t;
+}
+
+__device__ __forceinline__ KahanP kmergep(KahanP a, KahanP b)
+{
+    double t = a.s + b.s;
+    double bp = t - a.s;
+    double e = (a.s - (t - bp)) + (b.s - bp);
+    return KahanP{t, a.c + b.c + e};
+}
+
+// smallest integer r >= 0 with r*r >= v (v >= 0)
+__device__ __forceinline__ int64_t isqrt_ceil(int64_t v)
+{
+    if (v <= 0) return 0;
+    int64_t r = (int64_t)sqrt((double)v);
+    while (r * r < v) ++r;
+    while (r > 0 && (r - 1) * (r - 1) >= v) --r;
+    return r;
+}
+
+// One workgroup per bin b: sums over modes with lo2[b] <= |k|^2 < hi2[b].
+// F: half spectrum, fast axis length H = N/2+1.  Rows enumerated over the
+// bounding box |fi|,|fj| <= R (3D) or |fj| <= R (2D), in row-major order.
+template <int ND>
+__global__ void __launch_bounds__(256)
+bin_kernel(int N, const double2 *__restrict__ F, const int64_t *__restrict__ lo2,
+           const int64_t *__restrict__ hi2, double *__restrict__ out_pk,
+           double *__restrict__ out_ks, long long *__restrict__ out_cnt)
+{
+    __shared__ KahanP sh_pk[4], sh_ks[4];
+    __shared__ long long sh_n[4];
+    const int b = blockIdx.x;
+    const int64_t lo = lo2[b], hi = hi2[b];
+    const int H = N / 2 + 1;
+    KahanP pk{0.0, 0.0}, ks{0.0, 0.0};
+    long long cnt = 0;
+    if (hi > lo) {
+        int64_t R = isqrt_ceil(hi);           // |f| < R on every axis
+        if (R > N / 2) R = N / 2;
+        int64_t side = 2 * R + 1;
+        int64_t nrows = (ND == 3) ? side * side : side;
+        for (int64_t r = threadIdx.x; r < nrows; r += blockDim.x) {
+            int64_t fi = 0, fj;
+            if (ND == 3) {
+                fi = r / side - R;
+                fj = r % side - R;
+            } else {
+                fj = r - R;
+            }
+            if (fi <= -N / 2 || fj <= -N / 2) continue;     // |k| >= N/2 there
+            int64_t r2 = fi * fi + fj * fj;
+            if (r2 >= hi) continue;
+            int64_t z0 = isqrt_ceil(lo - r2);
+            int64_t z1 = isqrt_ceil(hi - r2);                // exclusive
+            if (z1 > H) z1 = H;
+            if (z0 >= z1) continue;
+            int64_t ii = fi < 0 ? fi + N : fi, jj = fj < 0 ? fj + N : fj;
+            int64_t rowbase = (ND == 3) ? (ii * N + jj) * H : jj * H;
+            for (int64_t z = z0; z < z1; ++z) {
+                double2 f = F[rowbase + z];
+                double w = (z == 0 || z == N / 2) ? 1.0 : 2.0;
+                double p2 = f.x * f.x + f.y * f.y;
+                double km = sqrt((double)(r2 + z * z));
+                kaddp(pk, w * p2);
+                kaddp(ks, w * km);
+                cnt += (long long)w;
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int o = 32; o > 0; o >>= 1) {
+        KahanP a{__shfl_down(pk.s, o, 64), __shfl_down(pk.c, o, 64)};
+        KahanP k2{__shfl_down(ks.s, o, 64), __shfl_down(ks.c, o, 64)};
+        long long n2 = __shfl_down(cnt, o, 64);
+        pk = kmergep(pk, a);
+        ks = kmergep(ks, k2);
+        cnt += n2;
+    }
+    if (lane == 0) {
+        sh_pk[wave] = pk;
+        sh_ks[wave] = ks;
+        sh_n[wave] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        KahanP a = sh_pk[0], k = sh_ks[0];
+        long long n = sh_n[0];
+        for (int w = 1; w < 4; ++w) {
+            a = kmergep(a, sh_pk[w]);
+            k = kmergep(k, sh_ks[w]);
+            n += sh_n[w];
+        }
+        out_pk[b] = a.s + a.c;
+        out_ks[b] = k.s + k.c;
+        out_cnt[b] = n;
+    }
+}
+
+int run_powerspec(Context *c, Member &m, int binning, int nbins, double *ks, double *pk,
+                  int32_t *count)
+{
+    const int N = m.N;
+    if (binning == AXS_BIN_LINEAR)
+        nbins = N / 2;
+    else if (binning != AXS_BIN_LOG || nbins < 1 || nbins > 4096) {
+        set_error("powerspec: bad binning %d / nbins %d", binning, nbins);
+        return AXS_ERR_INVALID_ARG;
+    }
+    FftPlan *plan = nullptr;
+    AXS_TRY(get_plan(c, N, &plan));
+
+    FieldView fv;
+    AXS_TRY(current_fields(c, m, &fv));
+    uint32_t L = (uint32_t)m.L;
+    unsigned g = (L + 255) / 256;
+    if (c->prec == AXS_FP64)
+        hipLaunchKernelGGL(to_double_kernel<double>, dim3(g), dim3(256), 0, c->stream, L,
+                           (const double *)fv.phi2, plan->in);
+    else
+        hipLaunchKernelGGL(to_double_kernel<float>, dim3(g), dim3(256), 0, c->stream, L,
+                           (const float *)fv.phi2, plan->in);
+    AXS_HIP_TRY(hipGetLastError());
+    if (rocfft_execution_info_set_stream(plan->info, c->stream) != rocfft_status_success) {
+        set_error("rocfft_execution_info_set_stream failed");
+        return AXS_ERR_FFT;
+    }
+    void *ib[1] = {plan->in};
+    void *ob[1] = {plan->out};
+    if (rocfft_execute(plan->plan, ib, ob, plan->info) != rocfft_status_success) {
+        set_error("rocfft_execute failed");
+        return AXS_ERR_FFT;
+    }
+
+    // bin ranges in |k|^2 (integers)
+    std::vector<int64_t> lo(nbins), hi(nbins);
+    int64_t top = (int64_t)(N / 2) * (N / 2);      // |k| < N/2  (pk_estimation.cpp:93)
+    if (binning == AXS_BIN_LINEAR) {
+        // floor(|k|) - 1 == b  <=>  (b+1)^2 <= |k|^2 < (b+2)^2    (:95)
+        for (int b = 0; b < nbins; ++b) {
+            lo[b] = (int64_t)(b + 1) * (b + 1);
+            hi[b] = (int64_t)(b + 2) * (b + 2);
+            if (hi[b] > top) hi[b] = top;
+            if (lo[b] > hi[b]) lo[b] = hi[b];
+        }
+    } else {
+        // squared edges e2[b] = (N/2)^(2b/nbins); integer range [ceil(e2[b]), ceil(e2[b+1]))
+        std::vector<double> e2(nbins + 1);
+        for (int b = 0; b <= nbins; ++b) e2[b] = pow((double)(N / 2), 2.0 * b / nbins);
+        e2[nbins] = (double)top;
+        for (int b = 0; b < nbins; ++b) {
+            lo[b] = (int64_t)ceil(e2[b]);
+            hi[b] = (int64_t)ceil(e2[b + 1]);
+            if (lo[b] < 1) lo[b] = 1;
+            if (hi[b] > top) hi[b] = top;
+            if (lo[b] > hi[b]) lo[b] = hi[b];
+        }
+    }
+    size_t nb = (size_t)nbins;
+    size_t need = nb * (8 + 8 + 8 + 8 + 8);
+    AXS_TRY(ensure_scratch(c, need + 256));
+    char *sp = (char *)c->scratch;
+    int64_t *dlo = (int64_t *)sp;
+    int64_t *dhi = dlo + nb;
+    double *dpk = (double *)(dhi + nb);
+    double *dks = dpk + nb;
+    long long *dcnt = (long long *)(dks + nb);
+    AXS_HIP_TRY(hipMemcpyAsync(dlo, lo.data(), nb * 8, hipMemcpyHostToDevice, c->stream));
+    AXS_HIP_TRY(hipMemcpyAsync(dhi, hi.data(), nb * 8, hipMemcpyHostToDevice, c->stream));
+    auto k = (c->nd == 3) ? bin_kernel<3> : bin_kernel<2>;
+    hipLaunchKernelGGL(k, dim3(nbins), dim3(256), 0, c->stream, N, (const double2 *)plan->out,
+                       dlo, dhi, dpk, dks, dcnt);
+    AXS_HIP_TRY(hipGetLastError());
+    std::vector<double> hpk(nb), hks(nb);
+    std::vector<long long> hcnt(nb);
+    AXS_HIP_TRY(hipMemcpyAsync(hpk.data(), dpk, nb * 8, hipMemcpyDeviceToHost, c->stream));
+    AXS_HIP_TRY(hipMemcpyAsync(hks.data(), dks, nb * 8, hipMemcpyDeviceToHost, c->stream));
+    AXS_HIP_TRY(hipMemcpyAsync(hcnt.data(), dcnt, nb * 8, hipMemcpyDeviceToHost, c->stream));
+    AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+
+    float Lbox = N * c->p.space_step;                           // pk_estimation.cpp:41
+    for (int b = 0; b < nbins; ++b) {
+        count[b] = (int32_t)hcnt[b];
+        if (hcnt[b] == 0) {
+            ks[b] = 0.0;
+            pk[b] = 0.0;
+            continue;
+        }
+        double kbar = hks[b] / (double)hcnt[b];                 // :118 / :196
+        double p = hpk[b];
+        if (binning == AXS_BIN_LINEAR) {
+            if (c->nd == 2)
+                p /= 2.0f * M_PI * kbar;                        // :119
+            else
+                p /= 4.0f * M_PI * (kbar * kbar);               // :197
+        } else {
+            p /= (double)hcnt[b];
+        }
+        ks[b] = kbar * (2.0f * M_PI / Lbox);                    // :120 / :198
+        pk[b] = p;
+    }
+    return AXS_OK;
+}
+
+void destroy_plans(Context *c)
+{
+    for (FftPlan *p : c->fft_plans) delete p;
+    c->fft_plans.clear();
+}
+
+}  // namespace axs

@@ -1,0 +1,204 @@
+// axs_reduce.hip -- whole-lattice reductions in compensated FP64:
+//   background sums of s_run.cpp:135-150 (six quantities);
+//   energy components (kinetic, gradient, potential).
+//
+// Each work-item keeps Neumaier (sum, compensation) pairs over a fixed
+// strided subset of sites; waves combine pairs with cross-lane shuffles, the
+// block writes one pair per quantity, and a single-block pass folds the
+// block pairs in block order.  The partition depends only on the lattice
+// size, so results are identical from run to run.
+#include "axs_internal.hpp"
+#include "axs_lattice.hpp"
+
+namespace axs {
+
+constexpr int kRedBlock = 256;
+constexpr int kRedMaxBlocks = 2048;
+
+struct Kahan {
+    double s, c;
+};
+
+__device__ __forceinline__ void kadd(Kahan &k, double x)
+{
+    double t = k.s + x;
+    if (fabs(k.s) >= fabs(x))
+        k.c += (k.s - t) + x;
+    else
+        k.c += (x - t) + k.s;
+    k.s = t;
+}
+
+__device__ __forceinline__ Kahan kmerge(Kahan a, Kahan b)
+{
+    // two-sum of the leading parts, compensations added
+    double t = a.s + b.s;
+    double bp = t - a.s;
+    double e = (a.s - (t - bp)) + (b.s - bp);
+    return Kahan{t, a.c + b.c + e};
+}
+
+__device__ __forceinline__ Kahan shfl_down_k(Kahan k, int o)
+{
+    return Kahan{__shfl_down(k.s, o, 64), __shfl_down(k.c, o, 64)};
+}
+
+// block reduction of Q quantities; result in lane 0 of wave 0
+template <int Q>
+__device__ __forceinline__ void block_reduce(Kahan (&k)[Q], Kahan (*sh)[kRedBlock / 64])
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        for (int o = 32; o > 0; o >>= 1) k[q] = kmerge(k[q], shfl_down_k(k[q], o));
+        if (lane == 0) sh[q][wave] = k[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            Kahan a = sh[q][0];
+            for (int w = 1; w < kRedBlock / 64; ++w) a = kmerge(a, sh[q][w]);
+            k[q] = a;
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRedBlock)
+background_kernel(uint32_t L, const T *__restrict__ p1, const T *__restrict__ p2,
+                  const T *__restrict__ v1, const T *__restrict__ v2, Kahan *__restrict__ part)
+{
+    __shared__ Kahan sh[6][kRedBlock / 64];
+    Kahan k[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) k[q] = Kahan{0.0, 0.0};
+    uint32_t stride = gridDim.x * kRedBlock;
+    for (uint32_t m = blockIdx.x * kRedBlock + threadIdx.x; m < L; m += stride) {
+        double a = (double)p1[m], b = (double)p2[m];
+        kadd(k[0], a);
+        kadd(k[1], b);
+        kadd(k[2], (double)v1[m]);
+        kadd(k[3], (double)v2[m]);
+        kadd(k[4], fabs(atan2(a, b)));            // s_run.cpp:145
+        kadd(k[5], sqrt(a * a + b * b));          // s_run.cpp:146
+    }
+    block_reduce<6>(k, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 6; ++q) part[blockIdx.x * 6 + q] = k[q];
+}
+
+template <int ND, typename T>
+__global__ void __launch_bounds__(kRedBlock)
+energy_kernel(int N, uint32_t L, const T *__restrict__ p1, const T *__restrict__ p2,
+              const T *__restrict__ v1, const T *__restrict__ v2, double tau2, double T03,
+              double lam_4tau2, Kahan *__restrict__ part)
+{
+    __shared__ Kahan sh[3][kRedBlock / 64];
+    Kahan k[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) k[q] = Kahan{0.0, 0.0};
+    uint32_t stride = gridDim.x * kRedBlock;
+    for (uint32_t m = blockIdx.x * kRedBlock + threadIdx.x; m < L; m += stride) {
+        Site<ND> s = decompose<ND>(m, N);
+        double a = (double)p1[m], b = (double)p2[m];
+        double g = 0.0;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            uint32_t q = shifted<ND>(s, d, 1, N);
+            double d1 = (double)p1[q] - a, d2 = (double)p2[q] - b;
+            g += d1 * d1 + d2 * d2;
+        }
+        double va = (double)v1[m], vb = (double)v2[m];
+        double S = a * a + b * b - tau2 + T03;
+        kadd(k[0], 0.5 * (va * va + vb * vb));
+        kadd(k[1], 0.5 * g);
+        kadd(k[2], lam_4tau2 * S * S);
+    }
+    block_reduce<3>(k, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) part[blockIdx.x * 3 + q] = k[q];
+}
+
+__global__ void fold_kernel(const Kahan *__restrict__ part, int nblocks, int Q,
+                            double *__restrict__ out)
+{
+    int q = threadIdx.x;
+    if (q >= Q) return;
+    Kahan a = part[q];
+    for (int b = 1; b < nblocks; ++b) a = kmerge(a, part[b * Q + q]);
+    out[q] = a.s + a.c;
+}
+
+static int red_blocks(uint32_t L)
+{
+    uint32_t need = (L + kRedBlock - 1) / kRedBlock;
+    return (int)(need < (uint32_t)kRedMaxBlocks ? need : (uint32_t)kRedMaxBlocks);
+}
+
+int run_background(Context *c, Member &m, double out[6])
+{
+    FieldView fv;
+    AXS_TRY(current_fields(c, m, &fv));
+    uint32_t L = (uint32_t)m.L;
+    int nb = red_blocks(L);
+    size_t part_b = (size_t)nb * 6 * sizeof(Kahan);
+    AXS_TRY(ensure_scratch(c, part_b + 64));
+    Kahan *part = (Kahan *)c->scratch;
+    double *dout = (double *)((char *)c->scratch + part_b);
+    if (c->prec == AXS_FP64)
+        hipLaunchKernelGGL(background_kernel<double>, dim3(nb), dim3(kRedBlock), 0, c->stream, L,
+                           (const double *)fv.phi1, (const double *)fv.phi2,
+                           (const double *)fv.vel1, (const double *)fv.vel2, part);
+    else
+        hipLaunchKernelGGL(background_kernel<float>, dim3(nb), dim3(kRedBlock), 0, c->stream, L,
+                           (const float *)fv.phi1, (const float *)fv.phi2,
+                           (const float *)fv.vel1, (const float *)fv.vel2, part);
+    hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(64), 0, c->stream, part, nb, 6, dout);
+    AXS_HIP_TRY(hipGetLastError());
+    AXS_HIP_TRY(hipMemcpyAsync(out, dout, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+    return AXS_OK;
+}
+
+int run_energy(Context *c, Member &m, double out[3])
+{
+    FieldView fv;
+    AXS_TRY(current_fields(c, m, &fv));
+    uint32_t L = (uint32_t)m.L;
+    int nb = red_blocks(L);
+    size_t part_b = (size_t)nb * 3 * sizeof(Kahan);
+    AXS_TRY(ensure_scratch(c, part_b + 64));
+    Kahan *part = (Kahan *)c->scratch;
+    double *dout = (double *)((char *)c->scratch + part_b);
+    double lam = (double)c->p.lambda, T0 = (double)c->p.T_initial;
+    double tau2 = c->tau * c->tau;
+    double T03 = (T0 * T0) / 3.0;
+    double lam_4tau2 = lam / (4.0 * tau2);
+    if (c->prec == AXS_FP64) {
+        auto k = (c->nd == 2) ? energy_kernel<2, double> : energy_kernel<3, double>;
+        hipLaunchKernelGGL(k, dim3(nb), dim3(kRedBlock), 0, c->stream, m.N, L,
+                           (const double *)fv.phi1, (const double *)fv.phi2,
+                           (const double *)fv.vel1, (const double *)fv.vel2, tau2, T03, lam_4tau2,
+                           part);
+    } else {
+        auto k = (c->nd == 2) ? energy_kernel<2, float> : energy_kernel<3, float>;
+        hipLaunchKernelGGL(k, dim3(nb), dim3(kRedBlock), 0, c->stream, m.N, L,
+                           (const float *)fv.phi1, (const float *)fv.phi2,
+                           (const float *)fv.vel1, (const float *)fv.vel2, tau2, T03, lam_4tau2,
+                           part);
+    }
+    hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(64), 0, c->stream, part, nb, 3, dout);
+    AXS_HIP_TRY(hipGetLastError());
+    double raw[3];
+    AXS_HIP_TRY(hipMemcpyAsync(raw, dout, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+    double dx = (double)c->p.space_step;
+    double vol = (c->nd == 3) ? dx * dx * dx : dx * dx;
+    out[0] = raw[0] * vol;
+    out[1] = raw[1] / (dx * dx) * vol;
+    out[2] = raw[2] * vol;
+    return AXS_OK;
+}
+
+}  // namespace axs

@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Lattice site-updates/s of the velocity-Verlet step (the reference's
+vvsl_field_rescaled, src/standard/s_integrate.cpp:305-348) on one MI355X.
+
+Workload: 3D 1024^3 periodic lattice, the FP32 form (phi and its velocity
+in FP32, tau and every reduction in FP64) -- the headline -- beside the FP64
+form on the same k^-1 Gaussian random field (seed 1998, generated in device
+memory), lambda = 1, dx = 0.5, dt = 0.1 (float), T_initial = 4, 7-point
+Laplacian.  One "step" advances the whole lattice by one time step; W warm-up
+steps are followed by exactly K timed steps, with the stream synchronised
+before and after.
+
+The JSON line also carries
+  roofline     : the step kernel's achieved HBM rate, from the algorithmic
+                 bytes per launch (32 B/site FP32: read + write of phi1, phi2
+                 and the two velocities) over the kernel's average duration
+                 measured with HIP events on the library's stream;
+  cpu_baseline : the CPU oracle (reference-structured FP64 step: drift,
+                 two CSR sparse-matrix Laplacians, potential, kick, copy, with
+                 OpenMP) timed on this host over a bounded sample;
+  accuracy     : FP32 vs FP64 field rel-L2, string count, energy and
+                 power-spectrum differences after the same W+K steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "axion-strings_amd"))
+
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_step_fp32_3d_1024.json")
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def gpu_leg(np, axs, N, precision, warmup, steps, T0):
+    lat = axs.Lattice(3, N, precision=precision, T_initial=T0)
+    lat.init("kinv", seed=1998)
+    lat.tau = 1.0
+    log(f"[bench] {precision}: {N}^3 initialised, {warmup} warm-up steps")
+    if warmup:
+        lat.step(warmup)
+    log(f"[bench] {precision}: timing {steps} steps")
+    t0 = time.perf_counter()
+    lat.step(steps)                  # synchronous: returns after the stream drains
+    wall = time.perf_counter() - t0
+    kern_ms = lat.last_step_ms       # HIP-event average over the same steps
+    scheme_b, alg_b = lat.bytes_per_site()
+    L = N ** 3
+    res = {
+        "wall_s": wall,
+        "ms_per_step": 1e3 * wall / steps,
+        "kernel_ms": kern_ms,
+        "site_updates_per_s": L * steps / wall,
+        "alg_bytes_per_site": alg_b,
+        "scheme_bytes_per_site": scheme_b,
+    }
+    log(f"[bench] {precision}: {res['ms_per_step']:.3f} ms/step wall, {kern_ms:.3f} ms kernel")
+    return lat, res
+
+
+def accuracy(np, lat32, lat64):
+    s32 = lat32.get_state(fields=("phi1", "phi2"))
+    s64 = lat64.get_state(fields=("phi1", "phi2"))
+    num = np.sqrt(np.sum((s32["phi1"] - s64["phi1"]) ** 2) + np.sum((s32["phi2"] - s64["phi2"]) ** 2))
+    den = np.sqrt(np.sum(s64["phi1"] ** 2) + np.sum(s64["phi2"] ** 2))
+    del s32, s64
+    n32 = lat32.strings(coords=False)[0]
+    n64 = lat64.strings(coords=False)[0]
+    e32, e64 = lat32.energy(), lat64.energy()
+    _, pk32, c32 = lat32.powerspec(binning="log", nbins=64)
+    _, pk64, c64 = lat64.powerspec(binning="log", nbins=64)
+    sel = c64 > 0
+    pk_err = float(np.max(np.abs(pk32[sel] - pk64[sel]) / np.abs(pk64[sel])))
+    return {
+        "field_rel_l2": float(num / den),
+        "string_count_fp32": int(n32),
+        "string_count_fp64": int(n64),
+        "string_count_rel_diff": float(abs(n32 - n64) / max(n64, 1)),
+        "energy_rel_diff": float(abs(e32.sum() - e64.sum()) / abs(e64.sum())),
+        "pk_log64_max_rel_err": pk_err,
+    }
+
+
+def cpu_baseline(np, seconds: float):
+    """The oracle's reference-structured FP64 step on a 256^3 sub-sample of
+    the workload (same per-site work), all host threads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle as O
+    from fields import kinv_field
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    Ns = 256
+    p1, p2 = kinv_field(3, Ns, seed=1998)
+    z = np.zeros(Ns ** 3)
+    st = O.State(p1, p2, z, z)
+    p = O.params(3, Ns)
+    A = O.Matrix(3, Ns)
+    st.run(p, A, 1)                             # first touch
+    n, t = 0, 0.0
+    while t < seconds and n < 200:
+        t0 = time.perf_counter()
+        st.run(p, A, 1)
+        t += time.perf_counter() - t0
+        n += 1
+    rate = Ns ** 3 * n / t
+    return {
+        "value": rate,
+        "unit": "site-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"3D {Ns}^3 FP64, {n} reference-structured steps (drift, 2x CSR SpMV "
+                  f"Laplacian, potential, kick, copy) of the CPU oracle, {t:.1f} s, "
+                  f"OMP_NUM_THREADS={threads}",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--N", type=int, default=1024)
+    ap.add_argument("--T-initial", type=float, default=4.0)
+    ap.add_argument("--skip-fp64", action="store_true", help="time the FP32 form only")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--skip-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU slab decomposition is a design only)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import axs
+
+    N, K, W = args.N, args.steps, args.warmup
+    lat32, r32 = gpu_leg(np, axs, N, "fp32", W, K, args.T_initial)
+    out64 = None
+    acc = None
+    if not args.skip_fp64:
+        lat64, r64 = gpu_leg(np, axs, N, "fp64", W, K, args.T_initial)
+        log("[bench] comparing FP32 with FP64")
+        acc = accuracy(np, lat32, lat64)
+        lat64.close()
+        out64 = {
+            "value": r64["site_updates_per_s"],
+            "ms_per_step": r64["ms_per_step"],
+            "kernel_ms": r64["kernel_ms"],
+            "roofline_frac_alg": (r64["alg_bytes_per_site"] * N ** 3 / (r64["kernel_ms"] * 1e-3))
+            / (HBM_PEAK_GBS * 1e9),
+            "scheme_bytes_per_site": r64["scheme_bytes_per_site"],
+        }
+    lat32.close()
+
+    L = N ** 3
+    achieved = r32["alg_bytes_per_site"] * L / (r32["kernel_ms"] * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        try:
+            t = json.load(open(TRAFFIC_FILE))
+            if int(t.get("N", -1)) == N:
+                traffic = t.get("bytes_per_launch")
+        except (ValueError, OSError):
+            traffic = None
+
+    cpu = None
+    if not args.skip_cpu:
+        log("[bench] CPU baseline (oracle)")
+        cpu = cpu_baseline(np, args.cpu_seconds)
+
+    line = {
+        "metric": "lattice site-updates/s (3D 1024^3, FP32 form; FP64 form alongside)",
+        "value": r32["site_updates_per_s"],
+        "unit": "site-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: k^-1 Gaussian random field, seed 1998, generated on the device",
+        "config": {
+            "workload": f"3D {N}^3 periodic lattice velocity-Verlet step, FP32 phi/velocity, "
+                        "lambda=1, dx=0.5, dt=0.1, T_initial=%g, 7-point Laplacian" % args.T_initial,
+            "N": N, "ndims": 3, "precision": "fp32", "T_initial": args.T_initial,
+            "stencil_setting": 0, "init": "kinv seed 1998",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+        },
+        "kernel_ms_per_step": r32["kernel_ms"],
+        "fp64": out64,
+        "fp32_speedup_over_fp64": (r32["site_updates_per_s"] / out64["value"]) if out64 else None,
+        "accuracy_fp32_vs_fp64": acc,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,191 @@
+/*
+ * axs_gpu.h -- C-ABI of the MI355X (gfx950) uniform-lattice evolution path.
+ *
+ * This is the drop-in boundary: the host driver (our counterpart of the
+ * reference's src/standard/s_run.cpp) calls these functions where the
+ * reference calls its in-process routines.  Every entry point returns an int
+ * status (AXS_OK == 0); no C++ exception crosses it; every pointer argument
+ * is a plain host pointer owned by the caller.  Host field arrays are double
+ * in the reference layout (src/common/common.hpp:28, 81-119):
+ *     2D  m = i + N*j            (i fastest)
+ *     3D  m = (i*N + j)*N + k    (k fastest)
+ *
+ * A context owns its device memory and one HIP stream.  It is not thread-
+ * safe: use one context per host thread.  Calls are synchronous with respect
+ * to the host buffers they take (results are in place when the call returns).
+ */
+#ifndef AXS_GPU_H
+#define AXS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define AXS_OK                 0
+#define AXS_ERR_INVALID_ARG    1   /* bad pointer, size, index or parameter  */
+#define AXS_ERR_UNSUPPORTED    2   /* parameter outside the supported set    */
+#define AXS_ERR_OUT_OF_MEMORY  3   /* device allocation failed               */
+#define AXS_ERR_HIP            4   /* a HIP runtime call failed              */
+#define AXS_ERR_CAPACITY       5   /* output buffer too small (count is set) */
+#define AXS_ERR_NO_DEVICE      6   /* no usable gfx950 device                */
+#define AXS_ERR_FFT            7   /* rocFFT plan or execution failed        */
+#define AXS_ERR_STATE          8   /* call out of order (e.g. no state yet)  */
+
+/* ---- forms ------------------------------------------------------------- */
+#define AXS_FP64  0   /* plain form: reference velocity-Verlet, bit-exact      */
+#define AXS_FP32  1   /* phi and its velocity in FP32; tau and sums in FP64    */
+
+#define AXS_INIT_KINV    0 /* complex GRF, |F| ~ 1/|k| below pi/(4dx), <|phi|^2>=1 */
+#define AXS_INIT_THERMAL 1 /* thermal spectrum of gaussian_thermal (init.cpp)     */
+
+#define AXS_BIN_LINEAR 0   /* reference bins floor(|k|)-1, N/2 of them          */
+#define AXS_BIN_LOG    1   /* nbins log-spaced bins over 1 <= |k| < N/2          */
+
+/*
+ * Run parameters.  Mirrors the fields of the reference's `parameters` struct
+ * that the evolution path reads (src/common/common.hpp:30-69), with the same
+ * types (dt, dx, lambda and T_initial are float there and here), plus the
+ * two build forms.  `N` is per lattice; a batch gives one N per member.
+ */
+typedef struct axs_params {
+    int   ndims;            /* NDIMS: 2 or 3                        (common.hpp:38) */
+    int   N;                /* points per side, even                (common.hpp:39) */
+    float space_step;       /* dx                                   (common.hpp:40) */
+    float time_step;        /* dt                                   (common.hpp:41) */
+    float lambda;           /* quartic coupling                     (common.hpp:35) */
+    float T_initial;        /* T0 in the thermal-mass term          (common.hpp:37) */
+    int   stencil_setting;  /* 0..3, s_integrate.cpp:9-14           (common.hpp:42) */
+    int   thr;              /* string threshold, percent            (common.hpp:62) */
+    int   precision;        /* AXS_FP64 or AXS_FP32                                 */
+    int   device;           /* HIP device ordinal                                   */
+} axs_params;
+
+typedef struct axs_ctx axs_ctx;
+
+/* ---- lifecycle --------------------------------------------------------- */
+
+/* Replaces the allocation block of run_standard (s_run.cpp:35-58): the
+ * stencil matrix of build_coefficient_matrix (s_integrate.cpp:5-149) is not
+ * built -- the kernels apply the stencil directly.  One lattice.           */
+int axs_create(const axs_params *p, axs_ctx **out);
+
+/* An ensemble of n_members independent lattices of side N_members[r] that
+ * share every other parameter and advance in lockstep (one tau).  No
+ * reference counterpart: the reference ran ensembles as separate processes.*/
+int axs_create_batch(const axs_params *p, int n_members, const int *N_members,
+                     axs_ctx **out);
+
+/* Frees everything (s_run.cpp:221-231). */
+int axs_destroy(axs_ctx *ctx);
+
+int axs_num_members(const axs_ctx *ctx, int *n_members);
+
+/* ---- state ------------------------------------------------------------- */
+
+/* Uploads the state of member `member` (0 for a single lattice): phi1, phi2,
+ * phidot1, phidot2 and the current kernels ker1_curr, ker2_curr of the
+ * reference's all_data (s_internal.hpp:6-21).  ker1/ker2 may be NULL, which
+ * means zero -- the reference's calloc'd start (s_run.cpp:44-47).         */
+int axs_set_member_state(axs_ctx *ctx, int member,
+                         const double *phi1, const double *phi2,
+                         const double *phidot1, const double *phidot2,
+                         const double *ker1, const double *ker2);
+
+/* Downloads the same six arrays (any pointer may be NULL to skip it). */
+int axs_get_member_state(axs_ctx *ctx, int member,
+                         double *phi1, double *phi2,
+                         double *phidot1, double *phidot2,
+                         double *ker1, double *ker2);
+
+/* Generates initial conditions for member `member` directly in device
+ * memory: AXS_INIT_KINV (the benchmark configurations' field, phidot = 0)
+ * or AXS_INIT_THERMAL (the spectrum of gaussian_thermal, init.cpp:33-200,
+ * replacing its MKL VSL normals and FFTW inverse transform by a Philox
+ * counter-based generator and rocFFT).  ker is set to zero as after the
+ * reference's calloc (s_run.cpp:44-47).  Same seed -> same field in the
+ * FP64 and FP32 forms (rounded once).                                      */
+int axs_init_member(axs_ctx *ctx, int member, int mode, uint64_t seed);
+
+/* Program time tau (the global `tau`, physics.cpp:3, starts at 1.0). */
+int axs_set_tau(axs_ctx *ctx, double tau);
+int axs_get_tau(const axs_ctx *ctx, double *tau);
+
+/* Single-lattice shorthands for member 0. */
+int axs_set_state(axs_ctx *ctx, const double *phi1, const double *phi2,
+                  const double *phidot1, const double *phidot2,
+                  const double *ker1, const double *ker2, double tau);
+int axs_get_state(axs_ctx *ctx, double *phi1, double *phi2,
+                  double *phidot1, double *phidot2,
+                  double *ker1, double *ker2, double *tau);
+
+/* ---- the step ---------------------------------------------------------- */
+
+/* nsteps calls of vvsl_field_rescaled (s_integrate.cpp:305-348) on every
+ * member: drift, tau += dt, Laplacian, potential, kick.  In the FP64 form
+ * the result is bit-identical to the reference expression trees.          */
+int axs_step(axs_ctx *ctx, int64_t nsteps);
+
+/* ---- strings ----------------------------------------------------------- */
+
+/* cores2 / cores3 (string_finding.cpp:3-86) on axion = atan2(phi1, phi2)
+ * (s_run.cpp:92-94) of member `member`.  Writes up to `cap` (i,j) or
+ * (i,j,k) int32 tuples in ascending site order (the reference's single-
+ * thread order) to coords (may be NULL) and the number of flagged sites to
+ * *count.  Returns AXS_ERR_CAPACITY (with *count set) if cap is too small. */
+int axs_member_strings(axs_ctx *ctx, int member, int32_t *coords, int64_t cap,
+                       int64_t *count);
+int axs_strings(axs_ctx *ctx, int32_t *coords, int64_t cap, int64_t *count);
+
+/* Flagged-site count and, in 2D, num_cores = n - (list-adjacent pairs)
+ * (s_run.cpp:99-109); in 3D num_cores == n_plaquettes.  One value per
+ * member, arrays of length n_members.                                     */
+int axs_string_counts(axs_ctx *ctx, int64_t *n_plaquettes, int64_t *num_cores);
+
+/* ---- reductions -------------------------------------------------------- */
+
+/* The six background sums of s_run.cpp:139-147 (not yet divided by the
+ * length): phi1, phi2, phidot1, phidot2, |atan2(phi1,phi2)|,
+ * sqrt(phi1^2+phi2^2).  Compensated FP64.  sums has 6 entries.            */
+int axs_member_background(axs_ctx *ctx, int member, double sums[6]);
+int axs_background(axs_ctx *ctx, double sums[6]);
+
+/* Energy components (kinetic, gradient, potential) times the cell volume;
+ * no reference counterpart.  Compensated FP64.                            */
+int axs_member_energy(axs_ctx *ctx, int member, double out[3]);
+int axs_energy(axs_ctx *ctx, double out[3]);
+
+/* ---- power spectrum ---------------------------------------------------- */
+
+/* output_powerspec (pk_estimation.cpp:37-214) of phi2 (s_run.cpp:206) of
+ * member `member`: AXS_BIN_LINEAR gives the reference's N/2 unit-width bins
+ * (nbins ignored; arrays need N/2 entries); AXS_BIN_LOG gives nbins log bins
+ * with pk the mean |F|^2 per mode.  ks, pk, count: one entry per bin.  Empty
+ * bins have count 0.  Transform by rocFFT, bins in compensated FP64.       */
+int axs_member_powerspec(axs_ctx *ctx, int member, int binning, int nbins,
+                         double *ks, double *pk, int32_t *count);
+int axs_powerspec(axs_ctx *ctx, int binning, int nbins,
+                  double *ks, double *pk, int32_t *count);
+
+/* ---- diagnostics ------------------------------------------------------- */
+
+/* Average device duration (ms) of the step kernels launched by the last
+ * axs_step call, measured with HIP events on the context's stream.        */
+int axs_last_step_ms(const axs_ctx *ctx, double *ms_per_step);
+
+/* Bytes each step kernel must move per site-update in this form (the
+ * scheme bytes) and the path's algorithmic bytes (read+write of phi and its
+ * velocity).                                                              */
+int axs_bytes_per_site(const axs_ctx *ctx, double *scheme_bytes, double *alg_bytes);
+
+const char *axs_strerror(int status);
+
+/* Message of the last failure on this thread (empty if none). */
+const char *axs_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AXS_GPU_H */

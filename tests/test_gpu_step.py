@@ -1,0 +1,120 @@
+"""GPU parity of the step (vvsl_field_rescaled, s_integrate.cpp:305-348):
+the FP64 form against the oracle, bit for bit; the FP32 form against FP64
+within the stated tolerance."""
+import numpy as np
+import pytest
+
+import axs
+import oracle as O
+from fields import kinv_field, random_state, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_run(nd, N, state, nsteps, **kw):
+    p = O.params(nd, N, **kw)
+    st = O.State(*state)
+    st.run(p, O.Matrix(nd, N, kw.get("stencil_setting", 0)), nsteps)
+    return st
+
+
+def _gpu_run(nd, N, state, nsteps, precision="fp64", **kw):
+    lat = axs.Lattice(nd, N, precision=precision, **kw)
+    lat.set_state(*state, tau=1.0)
+    lat.step(nsteps)
+    return lat
+
+
+def _assert_same(lat, st):
+    g = lat.get_state()
+    for k in ("phi1", "phi2", "phidot1", "phidot2"):
+        np.testing.assert_array_equal(g[k], getattr(st, k), err_msg=k)
+    np.testing.assert_array_equal(g["ker1"], st.ker1)
+    np.testing.assert_array_equal(g["ker2"], st.ker2)
+    assert lat.tau == st.tau
+
+
+def test_fp64_2d_256_500_steps_bit_exact():
+    # 2D 256^2, k^-1 field, phidot = 0, ker = 0, lambda=1, dx=0.5, dt=0.1
+    N = 256
+    p1, p2 = kinv_field(2, N, seed=1998)
+    z = np.zeros(N * N)
+    state = (p1, p2, z, z)
+    st = _oracle_run(2, N, state, 500)
+    lat = _gpu_run(2, N, state, 500)
+    _assert_same(lat, st)
+
+
+@pytest.mark.parametrize("N,nsteps", [(64, 120), (48, 60)])
+def test_fp64_3d_bit_exact_with_velocity_and_ker(N, nsteps):
+    state = random_state(3, N, seed=N, ker=True)
+    st = _oracle_run(3, N, state, nsteps)
+    lat = _gpu_run(3, N, state, nsteps)
+    _assert_same(lat, st)
+
+
+def test_fp64_3d_256_first_steps_bit_exact():
+    # the 3D 256^3 lattice itself, 8 steps from the k^-1 field
+    N = 256
+    p1, p2 = kinv_field(3, N, seed=1998)
+    z = np.zeros(N ** 3)
+    st = _oracle_run(3, N, (p1, p2, z, z), 8)
+    lat = _gpu_run(3, N, (p1, p2, z, z), 8)
+    _assert_same(lat, st)
+
+
+@pytest.mark.parametrize("nd,N,stencil", [(2, 64, 1), (2, 64, 2), (2, 64, 3), (3, 24, 1), (3, 24, 3)])
+def test_fp64_higher_order_stencils_bit_exact(nd, N, stencil):
+    rng = np.random.default_rng(stencil)
+    L = N ** nd
+    state = (rng.standard_normal(L), rng.standard_normal(L), 0.1 * rng.standard_normal(L),
+             0.1 * rng.standard_normal(L))
+    st = _oracle_run(nd, N, state, 25, stencil_setting=stencil)
+    lat = _gpu_run(nd, N, state, 25, stencil_setting=stencil)
+    _assert_same(lat, st)
+
+
+def test_split_calls_equal_one_call():
+    # step(a); step(b) == step(a+b): tau and state carry across calls
+    N = 64
+    state = random_state(2, N, seed=3)
+    a = _gpu_run(2, N, state, 70)
+    b = _gpu_run(2, N, state, 30)
+    b.step(40)
+    ga, gb = a.get_state(), b.get_state()
+    for k in ga:
+        np.testing.assert_array_equal(ga[k], gb[k])
+    assert a.tau == b.tau
+
+
+@pytest.mark.parametrize("nd,N,nsteps", [(2, 256, 500), (3, 64, 300)])
+def test_fp32_form_against_fp64(nd, N, nsteps):
+    # FP32 phi / velocity vs the FP64 form on the same input: field rel-L2
+    # <= 1e-4 (the precision form's accuracy target)
+    p1, p2 = kinv_field(nd, N, seed=1998)
+    z = np.zeros(N ** nd)
+    f64 = _gpu_run(nd, N, (p1, p2, z, z), nsteps, precision="fp64").get_state()
+    f32 = _gpu_run(nd, N, (p1, p2, z, z), nsteps, precision="fp32").get_state()
+    err = max(rel_l2(f32["phi1"], f64["phi1"]), rel_l2(f32["phi2"], f64["phi2"]))
+    verr = max(rel_l2(f32["phidot1"], f64["phidot1"]), rel_l2(f32["phidot2"], f64["phidot2"]))
+    print(f"fp32 vs fp64 {nd}D N={N} steps={nsteps}: field rel-L2 {err:.3e}, velocity {verr:.3e}")
+    assert err <= 1e-4
+    assert verr <= 1e-3
+
+
+def test_fp32_state_round_trip():
+    N = 32
+    state = random_state(2, N, seed=4, ker=True)
+    lat = axs.Lattice(2, N, precision="fp32")
+    lat.set_state(*state, tau=1.0)
+    g = lat.get_state()
+    for k, ref in zip(("phi1", "phi2", "phidot1", "phidot2", "ker1", "ker2"), state):
+        np.testing.assert_allclose(g[k], ref, rtol=0, atol=2e-6 * max(1.0, np.abs(ref).max()))
+
+
+def test_step_timing_reported():
+    N = 128
+    lat = _gpu_run(3, N, random_state(3, N, seed=1), 5)
+    assert lat.last_step_ms > 0
+    s, a = lat.bytes_per_site()
+    assert (s, a) == (96.0, 64.0)
