@@ -46,7 +46,25 @@ EXPORTS = (
     "axs_string_counts", "axs_member_background", "axs_background",
     "axs_member_energy", "axs_energy", "axs_member_powerspec", "axs_powerspec",
     "axs_last_step_ms", "axs_bytes_per_site", "axs_strerror", "axs_last_error",
+    "axs_set_kernel_config", "axs_get_kernel_config",
 )
+
+AXS_KERNEL_AUTO = 0
+AXS_KERNEL_SIMPLE = 1
+AXS_KERNEL_MARCH = 2
+KERNELS = {"auto": AXS_KERNEL_AUTO, "simple": AXS_KERNEL_SIMPLE, "march": AXS_KERNEL_MARCH}
+
+
+class AxsKernelConfig(ctypes.Structure):
+    """struct axs_kernel_config (include/axs_gpu.h)."""
+
+    _fields_ = [("kernel", c_int), ("rows", c_int), ("chunk", c_int), ("depth", c_int),
+                ("flags", c_int)]
+
+
+AXS_MARCH_PREFETCH = 1
+AXS_MARCH_NT_STORE = 2
+AXS_MARCH_JFAST = 4
 
 
 class AxsParams(ctypes.Structure):
@@ -111,6 +129,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "axs_powerspec": ([c_void_p, c_int, c_int, _DP, _DP, POINTER(c_int32)], c_int),
         "axs_last_step_ms": ([c_void_p, _DP], c_int),
         "axs_bytes_per_site": ([c_void_p, _DP, _DP], c_int),
+        "axs_set_kernel_config": ([c_void_p, POINTER(AxsKernelConfig)], c_int),
+        "axs_get_kernel_config": ([c_void_p, POINTER(AxsKernelConfig)], c_int),
         "axs_strerror": ([c_int], ctypes.c_char_p),
         "axs_last_error": ([], ctypes.c_char_p),
     }
@@ -237,6 +257,19 @@ class Lattice:
     # -- step --------------------------------------------------------------
     def step(self, nsteps: int = 1) -> None:
         _check(self.lib.axs_step(self._ctx, int(nsteps)), "axs_step")
+
+    def set_kernel(self, kernel: str = "auto", rows: int = 0, chunk: int = 0, depth: int = 1,
+                   flags: int = AXS_MARCH_PREFETCH):
+        cfg = AxsKernelConfig(KERNELS[kernel], rows, chunk, depth, flags)
+        _check(self.lib.axs_set_kernel_config(self._ctx, ctypes.byref(cfg)),
+               "axs_set_kernel_config")
+
+    def kernel_config(self) -> dict:
+        cfg = AxsKernelConfig()
+        _check(self.lib.axs_get_kernel_config(self._ctx, ctypes.byref(cfg)),
+               "axs_get_kernel_config")
+        return {"kernel": cfg.kernel, "rows": cfg.rows, "chunk": cfg.chunk, "depth": cfg.depth,
+                "flags": cfg.flags}
 
     @property
     def last_step_ms(self) -> float:

@@ -620,6 +620,54 @@ int axs_powerspec(axs_ctx *ctx, int binning, int nbins, double *ks, double *pk, 
     return axs_member_powerspec(ctx, 0, binning, nbins, ks, pk, count);
 }
 
+int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
+{
+    if (!ctx || !cfg) {
+        set_error("null argument");
+        return AXS_ERR_INVALID_ARG;
+    }
+    Context *c = &ctx->impl;
+    bool rows_ok = cfg->rows == 0 || cfg->rows == 4 ||
+                   (cfg->rows == 8 && c->prec == AXS_FP32) ||
+                   (cfg->rows == 2 && c->prec == AXS_FP64);
+    if (cfg->kernel < AXS_KERNEL_AUTO || cfg->kernel > AXS_KERNEL_MARCH || !rows_ok ||
+        cfg->chunk < 0 || (cfg->depth != 0 && cfg->depth != 1) || (cfg->flags & ~7) != 0) {
+        set_error("unsupported kernel configuration (kernel %d rows %d chunk %d depth %d "
+                  "flags %d)", cfg->kernel, cfg->rows, cfg->chunk, cfg->depth, cfg->flags);
+        return AXS_ERR_UNSUPPORTED;
+    }
+    for (const Member &m : c->members)
+        if (cfg->chunk > 0 && (m.N % cfg->chunk) != 0) {
+            set_error("chunk %d does not divide N=%d", cfg->chunk, m.N);
+            return AXS_ERR_INVALID_ARG;
+        }
+    c->tune.kernel = cfg->kernel;
+    if (c->prec == AXS_FP32)
+        c->tune.rj32 = cfg->rows ? cfg->rows : 4;
+    else
+        c->tune.rj64 = cfg->rows ? cfg->rows : 4;
+    c->tune.ci32 = cfg->chunk;
+    c->tune.ci64 = cfg->chunk;
+    c->tune.depth = 1;
+    c->tune.flags = cfg->flags;
+    return AXS_OK;
+}
+
+int axs_get_kernel_config(const axs_ctx *ctx, axs_kernel_config *cfg)
+{
+    if (!ctx || !cfg) {
+        set_error("null argument");
+        return AXS_ERR_INVALID_ARG;
+    }
+    const Context *c = &ctx->impl;
+    cfg->kernel = c->tune.kernel;
+    cfg->rows = (c->prec == AXS_FP32) ? c->tune.rj32 : c->tune.rj64;
+    cfg->chunk = (c->prec == AXS_FP32) ? c->tune.ci32 : c->tune.ci64;
+    cfg->depth = c->tune.depth;
+    cfg->flags = c->tune.flags;
+    return AXS_OK;
+}
+
 int axs_last_step_ms(const axs_ctx *ctx, double *ms)
 {
     if (!ctx || !ms) {

@@ -84,6 +84,21 @@ struct Member {
 
 struct FftPlan;
 
+// march-kernel variant switches (Tune::flags)
+constexpr int kTunePrefetch = 1;   // keep the next plane's loads in flight
+constexpr int kTuneNT = 2;         // non-temporal stores of the new state
+constexpr int kTuneJFast = 4;      // tiles ordered j fastest instead of k
+
+struct Tune {
+    int kernel = AXS_KERNEL_AUTO;
+    int rj32 = 4;      // rows per work-item, FP32 march
+    int rj64 = 4;      // rows per work-item, FP64 march
+    int ci32 = 0;      // planes per chunk, 0 = automatic
+    int ci64 = 0;
+    int depth = 1;
+    int flags = kTunePrefetch;
+};
+
 struct Context {
     axs_params p{};
     int nd = 2;
@@ -92,6 +107,7 @@ struct Context {
     std::vector<Member> members;
     int cur = 0;                  // ping-pong index holding the current state
     double tau = 1.0;             // physics.cpp:32
+    Tune tune;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_step_ms = 0.0;
@@ -127,6 +143,10 @@ void parallel_for(size_t n, void (*fn)(size_t begin, size_t end, void *arg), voi
 
 // one step of every member; reads state[cur], writes state[cur ^ 1]
 int launch_step(Context *c, const StepScalars &s);
+
+// 3D 2.5D streaming step (stencil_setting 0, N a multiple of 64)
+bool march3d_supported(const Context *c, int N);
+int launch_march3d(Context *c, Member &m, int src, int dst, const StepScalars &s);
 
 // FP32: compute ker = F(phi, tau) and phidot = u - dt/2 ker into aux buffers
 int launch_fp32_canonical(Context *c, Member &m, const StepScalars &s, bool recompute_ker);

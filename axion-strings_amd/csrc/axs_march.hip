@@ -1,0 +1,417 @@
+// axs_march.hip -- 3D step kernel with 2.5D streaming for gfx950.
+//
+// A workgroup of four 64-wide waves owns a TJ x 64 tile of the (j, k) plane
+// (k is the contiguous axis: one wave covers one 256/512-byte row segment)
+// and marches along i over a chunk of CI planes.  Each work-item keeps the
+// drifted field of its RJ sites at planes i-1, i, i+1 in registers; plane i
+// with a one-site halo in j and k sits in LDS (double-buffered, one barrier
+// per plane), so every field value is read from HBM once per chunk, plus the
+// tile halo (mostly L2 hits) and two extra planes per chunk.
+//
+// Work-group -> tile order is XCD-aware: the hardware deals work-groups to
+// the eight XCDs round-robin, so the logical tile index is rebuilt so that
+// each XCD walks a contiguous run of tiles, k fastest: the work-groups
+// resident on an XCD then cover whole 4 KB rows, and neighbouring tiles --
+// which read each other's edge sites as halo -- share an L2.  (With j
+// fastest the one-site k-halo columns miss in L2 and each drags in a full
+// cache line: measured at twice the algorithmic read traffic.)
+//
+// The per-site arithmetic is supplied by a traits class:
+//   ExactF64  the reference's velocity-Verlet (s_integrate.cpp:305-348) in
+//             FP64 with its expression trees: drift from (phi, phidot, ker),
+//             Laplacian summed in COO row order (centre, +-i, +-j, +-k),
+//             then alpha, potential, kick.  Bit-identical to the oracle.
+//   HalfF32   FP32 phi and half-step velocity u:  x = phi + dt u,
+//             ker = F(x), u' = u + dt ker.
+#include "axs_internal.hpp"
+
+namespace axs {
+
+constexpr int kMK = 64;          // tile width along k = wave width
+constexpr int kMW = 4;           // waves per work-group
+constexpr int kLDK = kMK + 2;    // LDS row stride (halo column each side)
+
+struct ExactF64 {
+    using T = double;
+    using S = StepScalars;
+    struct In {
+        const double *__restrict__ p1, *__restrict__ p2, *__restrict__ v1, *__restrict__ v2,
+            *__restrict__ k1, *__restrict__ k2;
+    };
+    struct Out {
+        double *__restrict__ p1, *__restrict__ p2, *__restrict__ v1, *__restrict__ v2,
+            *__restrict__ k1, *__restrict__ k2;
+    };
+    struct Raw {
+        double p1, p2, v1, v2, k1, k2;
+    };
+    struct Own {
+        double v1, v2, k1, k2;
+    };
+    static __device__ __forceinline__ double c0(const S &s) { return s.c[0]; }
+    static __device__ __forceinline__ void load(const In &a, uint32_t q, Raw &r)
+    {
+        r.p1 = a.p1[q];
+        r.p2 = a.p2[q];
+        r.v1 = a.v1[q];
+        r.v2 = a.v2[q];
+        r.k1 = a.k1[q];
+        r.k2 = a.k2[q];
+    }
+    // drift, s_integrate.cpp:313-314
+    static __device__ __forceinline__ void drift(const Raw &r, const S &s, double &x1, double &x2,
+                                                 Own &o)
+    {
+        o.v1 = r.v1;
+        o.v2 = r.v2;
+        o.k1 = r.k1;
+        o.k2 = r.k2;
+        x1 = r.p1 + s.dt * (r.v1 + 0.5 * r.k1 * s.dt);
+        x2 = r.p2 + s.dt * (r.v2 + 0.5 * r.k2 * s.dt);
+    }
+    // s_integrate.cpp:322-340 for one site, sums already in COO order
+    template <bool NT>
+    static __device__ __forceinline__ void update(const Out &o, uint32_t q, const S &s, double x1,
+                                                  double x2, double sum1, double sum2,
+                                                  const Own &w)
+    {
+        double lap1 = s.alpha * sum1, lap2 = s.alpha * sum2;
+        double S_ = x1 * x1 + x2 * x2 - s.tau2 + s.T03;
+        double kn1 = lap1 + s.cl * x1 * S_;
+        double kn2 = lap2 + s.cl * x2 * S_;
+        double v1 = w.v1 + 0.5 * (w.k1 + kn1) * s.dt;
+        double v2 = w.v2 + 0.5 * (w.k2 + kn2) * s.dt;
+        st<NT>(o.v1 + q, v1);
+        st<NT>(o.v2 + q, v2);
+        st<NT>(o.p1 + q, x1);
+        st<NT>(o.p2 + q, x2);
+        st<NT>(o.k1 + q, kn1);
+        st<NT>(o.k2 + q, kn2);
+    }
+    template <bool NT>
+    static __device__ __forceinline__ void st(double *p, double v)
+    {
+        if (NT)
+            __builtin_nontemporal_store(v, p);
+        else
+            *p = v;
+    }
+};
+
+struct HalfF32 {
+    using T = float;
+    using S = StepScalarsF;
+    struct In {
+        const float *__restrict__ p1, *__restrict__ p2, *__restrict__ u1, *__restrict__ u2;
+    };
+    struct Out {
+        float *__restrict__ p1, *__restrict__ p2, *__restrict__ u1, *__restrict__ u2;
+    };
+    struct Raw {
+        float p1, p2, u1, u2;
+    };
+    struct Own {
+        float u1, u2;
+    };
+    static __device__ __forceinline__ float c0(const S &s) { return s.c[0]; }
+    static __device__ __forceinline__ void load(const In &a, uint32_t q, Raw &r)
+    {
+        r.p1 = a.p1[q];
+        r.p2 = a.p2[q];
+        r.u1 = a.u1[q];
+        r.u2 = a.u2[q];
+    }
+    static __device__ __forceinline__ void drift(const Raw &r, const S &s, float &x1, float &x2,
+                                                 Own &o)
+    {
+        o.u1 = r.u1;
+        o.u2 = r.u2;
+        x1 = r.p1 + s.dt * r.u1;
+        x2 = r.p2 + s.dt * r.u2;
+    }
+    template <bool NT>
+    static __device__ __forceinline__ void update(const Out &o, uint32_t q, const S &s, float x1,
+                                                  float x2, float sum1, float sum2, const Own &w)
+    {
+        float S_ = x1 * x1 + x2 * x2 - s.tau2 + s.T03;
+        float k1 = s.alpha * sum1 + s.cl * x1 * S_;
+        float k2 = s.alpha * sum2 + s.cl * x2 * S_;
+        st<NT>(o.u1 + q, w.u1 + s.dt * k1);
+        st<NT>(o.u2 + q, w.u2 + s.dt * k2);
+        st<NT>(o.p1 + q, x1);
+        st<NT>(o.p2 + q, x2);
+    }
+    template <bool NT>
+    static __device__ __forceinline__ void st(float *p, float v)
+    {
+        if (NT)
+            __builtin_nontemporal_store(v, p);
+        else
+            *p = v;
+    }
+};
+
+struct MarchGeom {
+    int N;        // lattice side
+    int CI;       // planes per chunk
+    int nTj, nTk; // tiles along j and k
+    int remap;    // XCD-aware work-group order
+    int kfast;    // 1: tiles ordered k fastest (k-neighbours co-resident)
+};
+
+// PF: prefetch plane i+2 while plane i is computed (two planes of raw loads
+// in flight per work-item).  NT: non-temporal stores of the new state.
+template <class Tr, int RJ, bool PF, bool NT>
+__global__ void __launch_bounds__(kMW * 64)
+march3d_kernel(MarchGeom g, typename Tr::In in, typename Tr::Out out, typename Tr::S s)
+{
+    using T = typename Tr::T;
+    using Raw = typename Tr::Raw;
+    using Own = typename Tr::Own;
+    constexpr int TJ = kMW * RJ;
+    constexpr int PL = (TJ + 2) * kLDK;          // one component of one LDS plane
+    __shared__ T lds[2][2][PL];
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int N = g.N;
+    const uint32_t NN = (uint32_t)N * (uint32_t)N;
+
+    uint32_t b = blockIdx.x;
+    if (g.remap) b = (b & 7u) * (gridDim.x >> 3) + (b >> 3);
+    int tj, tk;
+    if (g.kfast) {
+        tk = (int)(b % (uint32_t)g.nTk);
+        b /= (uint32_t)g.nTk;
+        tj = (int)(b % (uint32_t)g.nTj);
+        b /= (uint32_t)g.nTj;
+    } else {
+        tj = (int)(b % (uint32_t)g.nTj);
+        b /= (uint32_t)g.nTj;
+        tk = (int)(b % (uint32_t)g.nTk);
+        b /= (uint32_t)g.nTk;
+    }
+    const int ch = (int)b;
+    const int j0 = tj * TJ, k0 = tk * kMK, i0 = ch * g.CI;
+    const int k = k0 + lane;
+
+    // halo role of this work-item: waves 0/1 the rows j0-1 / j0+TJ, wave 2
+    // the columns k0-1 (lanes 0..TJ-1) and k0+64 (lanes 32..32+TJ-1)
+    bool has_halo = false;
+    uint32_t hrow = 0;        // j*N + k of the halo site within a plane
+    int hoff = 0;             // LDS offset of the halo site
+    if (wave == 0) {
+        int j = (j0 == 0) ? N - 1 : j0 - 1;
+        has_halo = true;
+        hrow = (uint32_t)j * N + k;
+        hoff = lane + 1;
+    } else if (wave == 1) {
+        int j = (j0 + TJ == N) ? 0 : j0 + TJ;
+        has_halo = true;
+        hrow = (uint32_t)j * N + k;
+        hoff = (TJ + 1) * kLDK + lane + 1;
+    } else if (wave == 2) {
+        if (lane < TJ) {
+            int kk = (k0 == 0) ? N - 1 : k0 - 1;
+            has_halo = true;
+            hrow = (uint32_t)(j0 + lane) * N + kk;
+            hoff = (lane + 1) * kLDK;
+        } else if (lane >= 32 && lane < 32 + TJ) {
+            int kk = (k0 + kMK == N) ? 0 : k0 + kMK;
+            has_halo = true;
+            hrow = (uint32_t)(j0 + lane - 32) * N + kk;
+            hoff = (lane - 32 + 1) * kLDK + kMK + 1;
+        }
+    }
+
+    auto plane = [&](int i) -> uint32_t {
+        int w = (i < 0) ? i + N : ((i >= N) ? i - N : i);
+        return (uint32_t)w * NN;
+    };
+    auto own_q = [&](uint32_t po, int r) -> uint32_t {
+        return po + (uint32_t)(j0 + wave + kMW * r) * N + k;
+    };
+    auto load_plane = [&](int i, Raw (&r)[RJ], Raw &h) {
+        uint32_t po = plane(i);
+#pragma unroll
+        for (int rr = 0; rr < RJ; ++rr) Tr::load(in, own_q(po, rr), r[rr]);
+        if (has_halo) Tr::load(in, po + hrow, h);
+    };
+    // drift a loaded plane into registers (own) and LDS buffer `buf`
+    auto drift_plane = [&](const Raw (&r)[RJ], const Raw &h, T (&x)[RJ][2], Own (&o)[RJ],
+                           int buf) {
+        T *W1 = lds[buf][0];
+        T *W2 = lds[buf][1];
+#pragma unroll
+        for (int rr = 0; rr < RJ; ++rr) {
+            Tr::drift(r[rr], s, x[rr][0], x[rr][1], o[rr]);
+            int off = (wave + kMW * rr + 1) * kLDK + lane + 1;
+            W1[off] = x[rr][0];
+            W2[off] = x[rr][1];
+        }
+        if (has_halo) {
+            T h1, h2;
+            Own dummy;
+            Tr::drift(h, s, h1, h2, dummy);
+            W1[hoff] = h1;
+            W2[hoff] = h2;
+        }
+    };
+
+    T xp[RJ][2], xc[RJ][2], xn[RJ][2];
+    Own oc[RJ], on[RJ];
+    Raw r1[RJ], r2[RJ], h1r, h2r;
+
+    // prologue: plane i0-1 (own sites), plane i0 (own + halo) into LDS[0],
+    // plane i0+1 requested
+    {
+        Raw rp[RJ];
+        uint32_t po = plane(i0 - 1);
+#pragma unroll
+        for (int rr = 0; rr < RJ; ++rr) Tr::load(in, own_q(po, rr), rp[rr]);
+        load_plane(i0, r1, h1r);
+        Own dummy[RJ];
+#pragma unroll
+        for (int rr = 0; rr < RJ; ++rr) Tr::drift(rp[rr], s, xp[rr][0], xp[rr][1], dummy[rr]);
+        drift_plane(r1, h1r, xc, oc, 0);
+    }
+    if (PF) load_plane(i0 + 1, r1, h1r);
+    __syncthreads();
+
+    const T c0 = Tr::c0(s);
+    for (int it = 0; it < g.CI; ++it) {
+        const int i = i0 + it;
+        const int buf = it & 1;
+        if (PF) {
+            if (it + 1 < g.CI) load_plane(i + 2, r2, h2r);     // in flight during plane i
+            drift_plane(r1, h1r, xn, on, buf ^ 1);            // plane i+1 (loaded earlier)
+        } else {
+            load_plane(i + 1, r1, h1r);
+            drift_plane(r1, h1r, xn, on, buf ^ 1);
+        }
+
+        // plane i: Laplacian in COO order, potential, kick
+        uint32_t pc = plane(i);
+        const T *L1 = lds[buf][0];
+        const T *L2 = lds[buf][1];
+#pragma unroll
+        for (int rr = 0; rr < RJ; ++rr) {
+            int o = (wave + kMW * rr + 1) * kLDK + lane + 1;
+            T s1 = c0 * xc[rr][0];
+            T s2 = c0 * xc[rr][1];
+            s1 = s1 + xn[rr][0];
+            s2 = s2 + xn[rr][1];      // (i+1, j, k)
+            s1 = s1 + xp[rr][0];
+            s2 = s2 + xp[rr][1];      // (i-1, j, k)
+            s1 = s1 + L1[o + kLDK];
+            s2 = s2 + L2[o + kLDK];   // (i, j+1, k)
+            s1 = s1 + L1[o - kLDK];
+            s2 = s2 + L2[o - kLDK];   // (i, j-1, k)
+            s1 = s1 + L1[o + 1];
+            s2 = s2 + L2[o + 1];      // (i, j, k+1)
+            s1 = s1 + L1[o - 1];
+            s2 = s2 + L2[o - 1];      // (i, j, k-1)
+            Tr::template update<NT>(out, own_q(pc, rr), s, xc[rr][0], xc[rr][1], s1, s2, oc[rr]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int rr = 0; rr < RJ; ++rr) {
+            xp[rr][0] = xc[rr][0];
+            xp[rr][1] = xc[rr][1];
+            xc[rr][0] = xn[rr][0];
+            xc[rr][1] = xn[rr][1];
+            oc[rr] = on[rr];
+            if (PF) r1[rr] = r2[rr];
+        }
+        if (PF) h1r = h2r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+
+static int pick_ci(int N, int tiles_per_plane)
+{
+    // enough work-groups to fill 256 CUs several times over, planes per
+    // chunk in [16, 128] and dividing N
+    int want_chunks = (2048 + tiles_per_plane - 1) / tiles_per_plane;
+    int ci = N / (want_chunks > 0 ? want_chunks : 1);
+    if (ci > 128) ci = 128;
+    if (ci < 16) ci = 16;
+    while (ci > 1 && N % ci != 0) --ci;
+    return ci;
+}
+
+bool march3d_supported(const Context *c, int N)
+{
+    return c->nd == 3 && c->p.stencil_setting == 0 && N % kMK == 0 && N >= kMK;
+}
+
+template <class Tr, int RJ, bool PF, bool NT>
+static int launch_march(Context *c, int N, typename Tr::In in, typename Tr::Out out,
+                        const typename Tr::S &s, int ci)
+{
+    constexpr int TJ = kMW * RJ;
+    if (N % TJ != 0) return AXS_ERR_UNSUPPORTED;
+    MarchGeom g;
+    g.N = N;
+    g.nTj = N / TJ;
+    g.nTk = N / kMK;
+    g.CI = ci > 0 ? ci : pick_ci(N, g.nTj * g.nTk);
+    if (N % g.CI != 0) return AXS_ERR_UNSUPPORTED;
+    uint32_t nblocks = (uint32_t)g.nTj * g.nTk * (N / g.CI);
+    g.remap = (nblocks % 8 == 0) ? 1 : 0;
+    g.kfast = (c->tune.flags & kTuneJFast) ? 0 : 1;
+    hipLaunchKernelGGL((march3d_kernel<Tr, RJ, PF, NT>), dim3(nblocks), dim3(kMW * 64), 0,
+                       c->stream, g, in, out, s);
+    return AXS_OK;
+}
+
+int launch_march3d(Context *c, Member &m, int src, int dst, const StepScalars &s)
+{
+    const int N = m.N;
+    if (c->prec == AXS_FP64) {
+        ExactF64::In in{(const double *)m.state[src][PHI1], (const double *)m.state[src][PHI2],
+                        (const double *)m.state[src][VEL1], (const double *)m.state[src][VEL2],
+                        (const double *)m.state[src][KER1], (const double *)m.state[src][KER2]};
+        ExactF64::Out out{(double *)m.state[dst][PHI1], (double *)m.state[dst][PHI2],
+                          (double *)m.state[dst][VEL1], (double *)m.state[dst][VEL2],
+                          (double *)m.state[dst][KER1], (double *)m.state[dst][KER2]};
+        const int f = c->tune.flags & (kTunePrefetch | kTuneNT);
+        const int rj = c->tune.rj64;
+        if (rj == 2) {
+            switch (f) {
+            case 0: return launch_march<ExactF64, 2, false, false>(c, N, in, out, s, c->tune.ci64);
+            case kTunePrefetch: return launch_march<ExactF64, 2, true, false>(c, N, in, out, s, c->tune.ci64);
+            case kTuneNT: return launch_march<ExactF64, 2, false, true>(c, N, in, out, s, c->tune.ci64);
+            default: return launch_march<ExactF64, 2, true, true>(c, N, in, out, s, c->tune.ci64);
+            }
+        }
+        switch (f) {
+        case 0: return launch_march<ExactF64, 4, false, false>(c, N, in, out, s, c->tune.ci64);
+        case kTunePrefetch: return launch_march<ExactF64, 4, true, false>(c, N, in, out, s, c->tune.ci64);
+        case kTuneNT: return launch_march<ExactF64, 4, false, true>(c, N, in, out, s, c->tune.ci64);
+        default: return launch_march<ExactF64, 4, true, true>(c, N, in, out, s, c->tune.ci64);
+        }
+    }
+    StepScalarsF f = to_float(s);
+    HalfF32::In in{(const float *)m.state[src][PHI1], (const float *)m.state[src][PHI2],
+                   (const float *)m.state[src][VEL1], (const float *)m.state[src][VEL2]};
+    HalfF32::Out out{(float *)m.state[dst][PHI1], (float *)m.state[dst][PHI2],
+                     (float *)m.state[dst][VEL1], (float *)m.state[dst][VEL2]};
+    const int fl = c->tune.flags & (kTunePrefetch | kTuneNT);
+    if (c->tune.rj32 == 8 && N % 32 == 0) {
+        switch (fl) {
+        case 0: return launch_march<HalfF32, 8, false, false>(c, N, in, out, f, c->tune.ci32);
+        case kTunePrefetch: return launch_march<HalfF32, 8, true, false>(c, N, in, out, f, c->tune.ci32);
+        case kTuneNT: return launch_march<HalfF32, 8, false, true>(c, N, in, out, f, c->tune.ci32);
+        default: return launch_march<HalfF32, 8, true, true>(c, N, in, out, f, c->tune.ci32);
+        }
+    }
+    switch (fl) {
+    case 0: return launch_march<HalfF32, 4, false, false>(c, N, in, out, f, c->tune.ci32);
+    case kTunePrefetch: return launch_march<HalfF32, 4, true, false>(c, N, in, out, f, c->tune.ci32);
+    case kTuneNT: return launch_march<HalfF32, 4, false, true>(c, N, in, out, f, c->tune.ci32);
+    default: return launch_march<HalfF32, 4, true, true>(c, N, in, out, f, c->tune.ci32);
+    }
+}
+
+}  // namespace axs

@@ -165,6 +165,10 @@ int launch_step(Context *c, const StepScalars &s)
     for (Member &m : c->members) {
         uint32_t L = (uint32_t)m.L;
         unsigned grid = grid_for(L, 256);
+        if (c->tune.kernel != AXS_KERNEL_SIMPLE && march3d_supported(c, m.N)) {
+            AXS_TRY(launch_march3d(c, m, src, dst, s));
+            continue;
+        }
         if (c->prec == AXS_FP64) {
             In64 in{(const double *)m.state[src][PHI1], (const double *)m.state[src][PHI2],
                     (const double *)m.state[src][VEL1], (const double *)m.state[src][VEL2],

@@ -169,6 +169,31 @@ int axs_member_powerspec(axs_ctx *ctx, int member, int binning, int nbins,
 int axs_powerspec(axs_ctx *ctx, int binning, int nbins,
                   double *ks, double *pk, int32_t *count);
 
+/* ---- kernel configuration ---------------------------------------------- */
+
+#define AXS_KERNEL_AUTO   0   /* march kernel where it applies, else simple   */
+#define AXS_KERNEL_SIMPLE 1   /* one work-item per site, neighbours via cache  */
+#define AXS_KERNEL_MARCH  2   /* 2.5D streaming along the slow axis (3D)       */
+
+/* Launch shape of the step kernels; no reference counterpart (the
+ * reference's loops are OpenMP `parallel for`).  Zero fields mean "choose
+ * automatically".  The same results come out of every configuration (the
+ * FP64 form stays bit-exact).                                             */
+#define AXS_MARCH_PREFETCH 1   /* next plane's loads in flight during compute */
+#define AXS_MARCH_NT_STORE 2   /* non-temporal stores of the new state        */
+#define AXS_MARCH_JFAST    4   /* tiles ordered j fastest (default k fastest) */
+
+typedef struct axs_kernel_config {
+    int kernel;   /* AXS_KERNEL_*                                           */
+    int rows;     /* rows per work-item: FP32 4 or 8, FP64 2 or 4           */
+    int chunk;    /* planes marched per work-group (divides N)              */
+    int depth;    /* time steps per pass (1)                                */
+    int flags;    /* AXS_MARCH_* bits                                       */
+} axs_kernel_config;
+
+int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg);
+int axs_get_kernel_config(const axs_ctx *ctx, axs_kernel_config *cfg);
+
 /* ---- diagnostics ------------------------------------------------------- */
 
 /* Average device duration (ms) of the step kernels launched by the last

@@ -118,3 +118,37 @@ def test_step_timing_reported():
     assert lat.last_step_ms > 0
     s, a = lat.bytes_per_site()
     assert (s, a) == (96.0, 64.0)
+
+
+@pytest.mark.parametrize("precision,rows,chunk,flags", [
+    ("fp64", 4, 0, 1), ("fp64", 4, 16, 0), ("fp64", 4, 128, 3), ("fp64", 2, 32, 1), ("fp64", 2, 0, 7),
+    ("fp32", 4, 0, 1), ("fp32", 8, 32, 0), ("fp32", 8, 128, 3), ("fp32", 4, 64, 5), ("fp32", 4, 16, 2)])
+def test_march_kernel_matches_simple(precision, rows, chunk, flags):
+    # the 2.5D march kernel and the one-site-per-work-item kernel compute
+    # the same expression trees: identical bits for every launch shape
+    N, nsteps = 128, 12
+    state = random_state(3, N, seed=5, ker=True)
+    a = axs.Lattice(3, N, precision=precision)
+    a.set_kernel("simple")
+    a.set_state(*state, tau=1.0)
+    a.step(nsteps)
+    b = axs.Lattice(3, N, precision=precision)
+    b.set_kernel("march", rows=rows, chunk=chunk, flags=flags)
+    b.set_state(*state, tau=1.0)
+    b.step(nsteps)
+    ga, gb = a.get_state(), b.get_state()
+    for k in ga:
+        np.testing.assert_array_equal(ga[k], gb[k], err_msg=k)
+
+
+def test_march_kernel_fp64_oracle_192():
+    # a side that is a multiple of 64 but not a power of two (the 192^3
+    # ensemble members), march kernel against the oracle
+    N, nsteps = 192, 4
+    state = random_state(3, N, seed=9)
+    st = _oracle_run(3, N, state, nsteps)
+    lat = axs.Lattice(3, N)
+    lat.set_kernel("march")
+    lat.set_state(*state, tau=1.0)
+    lat.step(nsteps)
+    _assert_same(lat, st)
