@@ -259,7 +259,7 @@ class Lattice:
         _check(self.lib.axs_step(self._ctx, int(nsteps)), "axs_step")
 
     def set_kernel(self, kernel: str = "auto", rows: int = 0, chunk: int = 0, depth: int = 1,
-                   flags: int = AXS_MARCH_PREFETCH):
+                   flags: int = AXS_MARCH_PREFETCH | AXS_MARCH_NT_STORE):
         cfg = AxsKernelConfig(KERNELS[kernel], rows, chunk, depth, flags)
         _check(self.lib.axs_set_kernel_config(self._ctx, ctypes.byref(cfg)),
                "axs_set_kernel_config")

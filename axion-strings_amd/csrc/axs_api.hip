@@ -137,32 +137,84 @@ void parallel_for(size_t n, void (*fn)(size_t, size_t, void *), void *arg)
     for (auto &t : ts) t.join();
 }
 
-struct ConvIn {
-    float *dst;
-    const double *a;
-    const double *k;
-    double half_dt;
+// host-side (de)interleaving between the reference's separate double
+// arrays and the device layouts
+struct Pack64 {
+    double *dst;              // pairs
+    const double *a, *b;      // either may be null (zero)
 };
 
-static void conv_in_fn(size_t b, size_t e, void *arg)
+static void pack64_fn(size_t lo, size_t hi, void *arg)
 {
-    ConvIn *c = (ConvIn *)arg;
-    for (size_t i = b; i < e; ++i) {
-        double v = c->a ? c->a[i] : 0.0;
-        if (c->k) v = v + c->half_dt * c->k[i];
-        c->dst[i] = (float)v;
+    Pack64 *c = (Pack64 *)arg;
+    for (size_t i = lo; i < hi; ++i) {
+        c->dst[2 * i] = c->a ? c->a[i] : 0.0;
+        c->dst[2 * i + 1] = c->b ? c->b[i] : 0.0;
     }
 }
 
-struct ConvOut {
-    double *dst;
-    const float *src;
+struct Unpack64 {
+    const double *src;        // pairs
+    double *a, *b;            // either may be null (skip)
 };
 
-static void conv_out_fn(size_t b, size_t e, void *arg)
+static void unpack64_fn(size_t lo, size_t hi, void *arg)
 {
-    ConvOut *c = (ConvOut *)arg;
-    for (size_t i = b; i < e; ++i) c->dst[i] = (double)c->src[i];
+    Unpack64 *c = (Unpack64 *)arg;
+    for (size_t i = lo; i < hi; ++i) {
+        if (c->a) c->a[i] = c->src[2 * i];
+        if (c->b) c->b[i] = c->src[2 * i + 1];
+    }
+}
+
+// FP32 state quadruple (phi1, phi2, u1, u2), u = phidot + dt/2 ker (in double)
+struct Pack32 {
+    float *dst;
+    const double *p1, *p2, *v1, *v2, *k1, *k2;
+    double half_dt;
+};
+
+static void pack32_fn(size_t lo, size_t hi, void *arg)
+{
+    Pack32 *c = (Pack32 *)arg;
+    for (size_t i = lo; i < hi; ++i) {
+        double u1 = c->v1[i], u2 = c->v2[i];
+        if (c->k1) u1 = u1 + c->half_dt * c->k1[i];
+        if (c->k2) u2 = u2 + c->half_dt * c->k2[i];
+        c->dst[4 * i] = (float)c->p1[i];
+        c->dst[4 * i + 1] = (float)c->p2[i];
+        c->dst[4 * i + 2] = (float)u1;
+        c->dst[4 * i + 3] = (float)u2;
+    }
+}
+
+struct PackPair32 {
+    float *dst;
+    const double *a, *b;
+};
+
+static void packpair32_fn(size_t lo, size_t hi, void *arg)
+{
+    PackPair32 *c = (PackPair32 *)arg;
+    for (size_t i = lo; i < hi; ++i) {
+        c->dst[2 * i] = c->a ? (float)c->a[i] : 0.0f;
+        c->dst[2 * i + 1] = c->b ? (float)c->b[i] : 0.0f;
+    }
+}
+
+struct Unpack32 {
+    const float *src;
+    int stride;               // 4 for the state, 2 for pairs
+    double *a, *b;
+};
+
+static void unpack32_fn(size_t lo, size_t hi, void *arg)
+{
+    Unpack32 *c = (Unpack32 *)arg;
+    for (size_t i = lo; i < hi; ++i) {
+        if (c->a) c->a[i] = (double)c->src[c->stride * i];
+        if (c->b) c->b[i] = (double)c->src[c->stride * i + 1];
+    }
 }
 
 void destroy_plans(Context *c);
@@ -173,12 +225,10 @@ Context::~Context()
     destroy_plans(this);
     for (Member &m : members) {
         for (int h = 0; h < 2; ++h)
-            for (int a = 0; a < 6; ++a)
+            for (int a = 0; a < 3; ++a)
                 if (m.state[h][a]) (void)hipFree(m.state[h][a]);
-        for (int a = 0; a < 2; ++a) {
-            if (m.aux_ker[a]) (void)hipFree(m.aux_ker[a]);
-            if (m.aux_vel[a]) (void)hipFree(m.aux_vel[a]);
-        }
+        if (m.aux_ker) (void)hipFree(m.aux_ker);
+        if (m.aux_vel) (void)hipFree(m.aux_vel);
     }
     if (scratch) (void)hipFree(scratch);
     if (listbuf) (void)hipFree(listbuf);
@@ -194,18 +244,15 @@ int current_fields(Context *c, Member &m, FieldView *v)
         set_error("member has no state: call axs_set_member_state first");
         return AXS_ERR_STATE;
     }
-    v->phi1 = m.state[c->cur][PHI1];
-    v->phi2 = m.state[c->cur][PHI2];
+    v->phi = m.state[c->cur][PHI];
     if (c->prec == AXS_FP64) {
-        v->vel1 = m.state[c->cur][VEL1];
-        v->vel2 = m.state[c->cur][VEL2];
+        v->vel = m.state[c->cur][VEL];
         return AXS_OK;
     }
     StepScalars s = make_scalars(c->p, c->tau);
     AXS_TRY(launch_fp32_canonical(c, m, s, !m.ker_valid));
     m.ker_valid = true;
-    v->vel1 = m.aux_vel[0];
-    v->vel2 = m.aux_vel[1];
+    v->vel = m.aux_vel;
     return AXS_OK;
 }
 
@@ -297,8 +344,10 @@ static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ct
         Member &m = c->members[r];
         m.N = Ns[r];
         m.L = (c->nd == 3) ? (int64_t)m.N * m.N * m.N : (int64_t)m.N * m.N;
-        size_t bytes = (size_t)m.L * c->elem;
-        int narr = (c->prec == AXS_FP64) ? 6 : 4;
+        // FP64: (phi, phidot, ker) pairs of doubles; FP32: one float
+        // quadruple per site plus float-pair read-back buffers
+        size_t bytes = (size_t)m.L * 16;
+        int narr = (c->prec == AXS_FP64) ? 3 : 1;
         for (int hh = 0; hh < 2; ++hh)
             for (int a = 0; a < narr; ++a) {
                 if (hipMalloc(&m.state[hh][a], bytes) != hipSuccess) {
@@ -311,15 +360,13 @@ static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ct
                 }
             }
         if (c->prec == AXS_FP32) {
-            for (int a = 0; a < 2; ++a) {
-                if (hipMalloc(&m.aux_ker[a], bytes) != hipSuccess ||
-                    hipMalloc(&m.aux_vel[a], bytes) != hipSuccess) {
-                    set_error("device allocation of %zu bytes failed (member %d)", bytes, r);
-                    return fail(AXS_ERR_OUT_OF_MEMORY);
-                }
-                (void)hipMemsetAsync(m.aux_ker[a], 0, bytes, c->stream);
-                (void)hipMemsetAsync(m.aux_vel[a], 0, bytes, c->stream);
+            size_t pb = (size_t)m.L * 8;
+            if (hipMalloc(&m.aux_ker, pb) != hipSuccess || hipMalloc(&m.aux_vel, pb) != hipSuccess) {
+                set_error("device allocation of %zu bytes failed (member %d)", pb, r);
+                return fail(AXS_ERR_OUT_OF_MEMORY);
             }
+            (void)hipMemsetAsync(m.aux_ker, 0, pb, c->stream);
+            (void)hipMemsetAsync(m.aux_vel, 0, pb, c->stream);
         }
     }
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
@@ -397,38 +444,25 @@ int axs_set_member_state(axs_ctx *ctx, int member, const double *phi1, const dou
     Member &m = c->members[member];
     size_t L = (size_t)m.L;
     int h = c->cur;
+    AXS_TRY(ensure_pinned(c, L * 16));
     if (c->prec == AXS_FP64) {
-        size_t b = L * sizeof(double);
-        AXS_HIP_TRY(hipMemcpy(m.state[h][PHI1], phi1, b, hipMemcpyHostToDevice));
-        AXS_HIP_TRY(hipMemcpy(m.state[h][PHI2], phi2, b, hipMemcpyHostToDevice));
-        AXS_HIP_TRY(hipMemcpy(m.state[h][VEL1], phidot1, b, hipMemcpyHostToDevice));
-        AXS_HIP_TRY(hipMemcpy(m.state[h][VEL2], phidot2, b, hipMemcpyHostToDevice));
-        if (ker1)
-            AXS_HIP_TRY(hipMemcpy(m.state[h][KER1], ker1, b, hipMemcpyHostToDevice));
-        else
-            AXS_HIP_TRY(hipMemset(m.state[h][KER1], 0, b));
-        if (ker2)
-            AXS_HIP_TRY(hipMemcpy(m.state[h][KER2], ker2, b, hipMemcpyHostToDevice));
-        else
-            AXS_HIP_TRY(hipMemset(m.state[h][KER2], 0, b));
-    } else {
-        // FP32 form: phi -> float; u = phidot + dt/2 ker (in double, then
-        // rounded); ker kept for the state read-back before the first step
-        AXS_TRY(ensure_pinned(c, L * sizeof(float)));
-        float *st = (float *)c->pinned;
-        const double half_dt = 0.5 * (double)c->p.time_step;
-        struct Job {
-            void *dst;
-            const double *a;
-            const double *k;
-        } jobs[6] = {{m.state[h][PHI1], phi1, nullptr}, {m.state[h][PHI2], phi2, nullptr},
-                     {m.state[h][VEL1], phidot1, ker1},  {m.state[h][VEL2], phidot2, ker2},
-                     {m.aux_ker[0], ker1, nullptr},      {m.aux_ker[1], ker2, nullptr}};
-        for (int j = 0; j < 6; ++j) {
-            ConvIn ci{st, jobs[j].a, jobs[j].k, half_dt};
-            parallel_for(L, conv_in_fn, &ci);
-            AXS_HIP_TRY(hipMemcpy(jobs[j].dst, st, L * sizeof(float), hipMemcpyHostToDevice));
+        double *st = (double *)c->pinned;
+        const double *srcs[3][2] = {{phi1, phi2}, {phidot1, phidot2}, {ker1, ker2}};
+        for (int a = 0; a < 3; ++a) {
+            Pack64 pk{st, srcs[a][0], srcs[a][1]};
+            parallel_for(L, pack64_fn, &pk);
+            AXS_HIP_TRY(hipMemcpy(m.state[h][a], st, L * 16, hipMemcpyHostToDevice));
         }
+    } else {
+        // FP32 form: u = phidot + dt/2 ker in double, rounded once; ker kept
+        // for the state read-back before the first step
+        float *st = (float *)c->pinned;
+        Pack32 pk{st, phi1, phi2, phidot1, phidot2, ker1, ker2, 0.5 * (double)c->p.time_step};
+        parallel_for(L, pack32_fn, &pk);
+        AXS_HIP_TRY(hipMemcpy(m.state[h][PHI], st, L * 16, hipMemcpyHostToDevice));
+        PackPair32 kp{st, ker1, ker2};
+        parallel_for(L, packpair32_fn, &kp);
+        AXS_HIP_TRY(hipMemcpy(m.aux_ker, st, L * 8, hipMemcpyHostToDevice));
         m.ker_valid = true;
     }
     m.has_state = true;
@@ -447,30 +481,37 @@ int axs_get_member_state(axs_ctx *ctx, int member, double *phi1, double *phi2, d
     }
     size_t L = (size_t)m.L;
     int h = c->cur;
+    AXS_TRY(ensure_pinned(c, L * 16));
     if (c->prec == AXS_FP64) {
-        size_t b = L * sizeof(double);
-        double *dst[6] = {phi1, phi2, phidot1, phidot2, ker1, ker2};
-        for (int a = 0; a < 6; ++a)
-            if (dst[a])
-                AXS_HIP_TRY(hipMemcpyAsync(dst[a], m.state[h][a], b, hipMemcpyDeviceToHost,
-                                           c->stream));
-        AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+        double *st = (double *)c->pinned;
+        double *dst[3][2] = {{phi1, phi2}, {phidot1, phidot2}, {ker1, ker2}};
+        for (int a = 0; a < 3; ++a) {
+            if (!dst[a][0] && !dst[a][1]) continue;
+            AXS_HIP_TRY(hipMemcpyAsync(st, m.state[h][a], L * 16, hipMemcpyDeviceToHost,
+                                       c->stream));
+            AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+            Unpack64 up{st, dst[a][0], dst[a][1]};
+            parallel_for(L, unpack64_fn, &up);
+        }
         return AXS_OK;
     }
     FieldView fv;
-    AXS_TRY(current_fields(c, m, &fv));      // computes ker and phidot
-    AXS_TRY(ensure_pinned(c, L * sizeof(float)));
+    AXS_TRY(current_fields(c, m, &fv));      // rebuilds ker and phidot
     float *st = (float *)c->pinned;
-    const void *src[6] = {m.state[h][PHI1], m.state[h][PHI2], m.aux_vel[0],
-                          m.aux_vel[1],     m.aux_ker[0],     m.aux_ker[1]};
-    double *dst[6] = {phi1, phi2, phidot1, phidot2, ker1, ker2};
-    for (int a = 0; a < 6; ++a) {
-        if (!dst[a]) continue;
-        AXS_HIP_TRY(hipMemcpyAsync(st, src[a], L * sizeof(float), hipMemcpyDeviceToHost,
-                                   c->stream));
+    struct {
+        const void *src;
+        size_t bytes;
+        int stride;
+        double *a, *b;
+    } jobs[3] = {{m.state[h][PHI], L * 16, 4, phi1, phi2},
+                 {m.aux_vel, L * 8, 2, phidot1, phidot2},
+                 {m.aux_ker, L * 8, 2, ker1, ker2}};
+    for (auto &j : jobs) {
+        if (!j.a && !j.b) continue;
+        AXS_HIP_TRY(hipMemcpyAsync(st, j.src, j.bytes, hipMemcpyDeviceToHost, c->stream));
         AXS_HIP_TRY(hipStreamSynchronize(c->stream));
-        ConvOut co{dst[a], st};
-        parallel_for(L, conv_out_fn, &co);
+        Unpack32 up{st, j.stride, j.a, j.b};
+        parallel_for(L, unpack32_fn, &up);
     }
     return AXS_OK;
 }
@@ -643,7 +684,7 @@ int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
         }
     c->tune.kernel = cfg->kernel;
     if (c->prec == AXS_FP32)
-        c->tune.rj32 = cfg->rows ? cfg->rows : 4;
+        c->tune.rj32 = cfg->rows ? cfg->rows : 8;
     else
         c->tune.rj64 = cfg->rows ? cfg->rows : 4;
     c->tune.ci32 = cfg->chunk;
@@ -687,7 +728,7 @@ int axs_bytes_per_site(const axs_ctx *ctx, double *scheme_bytes, double *alg_byt
     const Context *c = &ctx->impl;
     double w = (double)c->elem;
     if (alg_bytes) *alg_bytes = 2.0 * 4.0 * w;     // read + write phi1, phi2, vel1, vel2
-    if (scheme_bytes)
+    if (scheme_bytes)   // FP64 also moves the two kernel arrays
         *scheme_bytes = (c->prec == AXS_FP64) ? 2.0 * 6.0 * w : 2.0 * 4.0 * w;
     return AXS_OK;
 }

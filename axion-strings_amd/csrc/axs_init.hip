@@ -22,6 +22,7 @@
 #include <math.h>
 
 #include "axs_internal.hpp"
+#include "axs_lattice.hpp"
 
 namespace axs {
 
@@ -155,23 +156,55 @@ norm2_kernel(uint32_t L, const double2 *__restrict__ z, KahanI *__restrict__ par
     }
 }
 
-template <typename T>
+// scaled inverse transforms into the state layout; ker = 0 (and, for the
+// FP32 form, u = phidot since ker = 0)
 __global__ void __launch_bounds__(256)
-split_kernel(uint32_t L, const double2 *__restrict__ z, double scale, T *__restrict__ re,
-             T *__restrict__ im)
+store_init64_kernel(uint32_t L, const double2 *__restrict__ z, const double2 *__restrict__ zv,
+                    double scale, dvec2 *__restrict__ phi, dvec2 *__restrict__ vel,
+                    dvec2 *__restrict__ ker)
 {
     uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= L) return;
-    double2 v = z[m];
-    re[m] = (T)(v.x * scale);
-    im[m] = (T)(v.y * scale);
+    double2 a = z[m];
+    dvec2 p, v, k;
+    p.x = a.x * scale;
+    p.y = a.y * scale;
+    v.x = 0.0;
+    v.y = 0.0;
+    if (zv) {
+        double2 b = zv[m];
+        v.x = b.x * scale;
+        v.y = b.y * scale;
+    }
+    k.x = 0.0;
+    k.y = 0.0;
+    phi[m] = p;
+    vel[m] = v;
+    ker[m] = k;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) zero_kernel(uint32_t L, T *__restrict__ a)
+__global__ void __launch_bounds__(256)
+store_init32_kernel(uint32_t L, const double2 *__restrict__ z, const double2 *__restrict__ zv,
+                    double scale, fvec4 *__restrict__ st, fvec2 *__restrict__ ker)
 {
     uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < L) a[m] = (T)0;
+    if (m >= L) return;
+    double2 a = z[m];
+    fvec4 r;
+    r.x = (float)(a.x * scale);
+    r.y = (float)(a.y * scale);
+    r.z = 0.0f;
+    r.w = 0.0f;
+    if (zv) {
+        double2 b = zv[m];
+        r.z = (float)(b.x * scale);
+        r.w = (float)(b.y * scale);
+    }
+    fvec2 k;
+    k.x = 0.0f;
+    k.y = 0.0f;
+    st[m] = r;
+    ker[m] = k;
 }
 
 static int inverse_fft(Context *c, int N, double2 *buf)
@@ -276,34 +309,13 @@ int run_init(Context *c, Member &m, int mode, uint64_t seed)
     }
 
     int h = c->cur;
-    void *dst[4] = {m.state[h][PHI1], m.state[h][PHI2], m.state[h][VEL1], m.state[h][VEL2]};
     if (c->prec == AXS_FP64) {
-        hipLaunchKernelGGL(split_kernel<double>, dim3(g), dim3(256), 0, c->stream, L, phik, scale,
-                           (double *)dst[0], (double *)dst[1]);
-        if (velk)
-            hipLaunchKernelGGL(split_kernel<double>, dim3(g), dim3(256), 0, c->stream, L, velk,
-                               scale, (double *)dst[2], (double *)dst[3]);
-        else
-            for (int a = 2; a < 4; ++a)
-                hipLaunchKernelGGL(zero_kernel<double>, dim3(g), dim3(256), 0, c->stream, L,
-                                   (double *)dst[a]);
-        for (int a = KER1; a <= KER2; ++a)
-            hipLaunchKernelGGL(zero_kernel<double>, dim3(g), dim3(256), 0, c->stream, L,
-                               (double *)m.state[h][a]);
+        hipLaunchKernelGGL(store_init64_kernel, dim3(g), dim3(256), 0, c->stream, L, phik, velk,
+                           scale, (dvec2 *)m.state[h][PHI], (dvec2 *)m.state[h][VEL],
+                           (dvec2 *)m.state[h][KER]);
     } else {
-        // FP32 form: u = phidot + dt/2 * ker = phidot (ker = 0)
-        hipLaunchKernelGGL(split_kernel<float>, dim3(g), dim3(256), 0, c->stream, L, phik, scale,
-                           (float *)dst[0], (float *)dst[1]);
-        if (velk)
-            hipLaunchKernelGGL(split_kernel<float>, dim3(g), dim3(256), 0, c->stream, L, velk,
-                               scale, (float *)dst[2], (float *)dst[3]);
-        else
-            for (int a = 2; a < 4; ++a)
-                hipLaunchKernelGGL(zero_kernel<float>, dim3(g), dim3(256), 0, c->stream, L,
-                                   (float *)dst[a]);
-        for (int a = 0; a < 2; ++a)
-            hipLaunchKernelGGL(zero_kernel<float>, dim3(g), dim3(256), 0, c->stream, L,
-                               (float *)m.aux_ker[a]);
+        hipLaunchKernelGGL(store_init32_kernel, dim3(g), dim3(256), 0, c->stream, L, phik, velk,
+                           scale, (fvec4 *)m.state[h][PHI], (fvec2 *)m.aux_ker);
         m.ker_valid = true;
     }
     AXS_HIP_TRY(hipGetLastError());

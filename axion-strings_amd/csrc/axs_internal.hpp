@@ -66,20 +66,21 @@ StepScalarsF to_float(const StepScalars &s);
 // ---------------------------------------------------------------------------
 // one lattice of the context
 
-enum ArrayId { PHI1 = 0, PHI2 = 1, VEL1 = 2, VEL2 = 3, KER1 = 4, KER2 = 5 };
+// slots of Member::state.  FP64: three (x, y) double pairs per site, the
+// reference's all_data arrays interleaved by component.  FP32: slot PHI holds
+// (phi1, phi2, u1, u2) per site, u = phidot + dt/2 ker the half-step velocity.
+enum ArrayId { PHI = 0, VEL = 1, KER = 2 };
 
 struct Member {
     int N = 0;
     int64_t L = 0;
-    // Ping-pong state.  FP64: phi1, phi2, phidot1, phidot2, ker1, ker2 (six
-    // arrays, the reference's all_data).  FP32: phi1, phi2, u1, u2 with
-    // u = phidot + dt/2 * ker (the half-step velocity); slots 4, 5 unused.
-    void *state[2][6] = {{nullptr}};
-    // FP32 only: ker (valid while ker_valid) and phidot scratch
-    void *aux_ker[2] = {nullptr, nullptr};
-    void *aux_vel[2] = {nullptr, nullptr};
+    void *state[2][3] = {{nullptr}};   // ping-pong halves
+    // FP32 only: (ker1, ker2) of the current state while ker_valid, and the
+    // (phidot1, phidot2) read-back buffer, float pairs
+    void *aux_ker = nullptr;
+    void *aux_vel = nullptr;
     bool has_state = false;
-    bool ker_valid = true;        // FP32: aux_ker holds ker of the current state
+    bool ker_valid = true;
 };
 
 struct FftPlan;
@@ -91,12 +92,12 @@ constexpr int kTuneJFast = 4;      // tiles ordered j fastest instead of k
 
 struct Tune {
     int kernel = AXS_KERNEL_AUTO;
-    int rj32 = 4;      // rows per work-item, FP32 march
+    int rj32 = 8;      // rows per work-item, FP32 march
     int rj64 = 4;      // rows per work-item, FP64 march
     int ci32 = 0;      // planes per chunk, 0 = automatic
     int ci64 = 0;
     int depth = 1;
-    int flags = kTunePrefetch;
+    int flags = kTunePrefetch | kTuneNT;
 };
 
 struct Context {
@@ -159,11 +160,11 @@ int run_energy(Context *c, Member &m, double out[3]);
 int run_powerspec(Context *c, Member &m, int binning, int nbins, double *ks, double *pk,
                   int32_t *count);
 
-// device pointers to the current (canonical) fields of a member as doubles
-// or floats, plus its velocity (phidot) -- for the FP32 form this triggers
-// the canonicalisation kernel.
+// device pointers to the current fields of a member: phi (FP64 pairs or
+// FP32 quadruples) and phidot (FP64 pairs or FP32 pairs) -- for the FP32 form
+// this runs the kernel that rebuilds phidot (and ker) from u.
 struct FieldView {
-    const void *phi1, *phi2, *vel1, *vel2;
+    const void *phi, *vel;
 };
 int current_fields(Context *c, Member &m, FieldView *v);
 

@@ -14,6 +14,59 @@
 
 namespace axs {
 
+// Storage vector types.  Device state is interleaved per site:
+//   FP64 form  PHI = (phi1, phi2), VEL = (phidot1, phidot2), KER = (ker1, ker2),
+//              one dvec2 array each;
+//   FP32 form  one fvec4 array (phi1, phi2, u1, u2), u = phidot + dt/2 ker,
+//              with fvec2 (ker1, ker2) and (phidot1, phidot2) read-back buffers.
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+typedef float fvec2 __attribute__((ext_vector_type(2)));
+
+// read-only views of the field and its velocity, widened to double
+struct PhiF64 {
+    const dvec2 *__restrict__ p;
+    __device__ __forceinline__ void get(uint32_t m, double &a, double &b) const
+    {
+        dvec2 v = p[m];
+        a = v.x;
+        b = v.y;
+    }
+};
+struct PhiF32 {
+    const fvec4 *__restrict__ p;
+    __device__ __forceinline__ void get(uint32_t m, double &a, double &b) const
+    {
+        fvec4 v = p[m];
+        a = (double)v.x;
+        b = (double)v.y;
+    }
+};
+struct VelF64 {
+    const dvec2 *__restrict__ p;
+    __device__ __forceinline__ void get(uint32_t m, double &a, double &b) const
+    {
+        dvec2 v = p[m];
+        a = v.x;
+        b = v.y;
+    }
+};
+struct VelF32 {
+    const fvec2 *__restrict__ p;
+    __device__ __forceinline__ void get(uint32_t m, double &a, double &b) const
+    {
+        fvec2 v = p[m];
+        a = (double)v.x;
+        b = (double)v.y;
+    }
+};
+
+template <typename V>
+__device__ __forceinline__ void store_nt(V *p, V v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
 __device__ __forceinline__ int wrap_coord(int x, int N)
 {
     // valid for -N <= x < 2N, which holds for every stencil radius <= 4 on

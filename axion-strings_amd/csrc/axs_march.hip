@@ -1,7 +1,8 @@
 // axs_march.hip -- 3D step kernel with 2.5D streaming for gfx950.
 //
 // A workgroup of four 64-wide waves owns a TJ x 64 tile of the (j, k) plane
-// (k is the contiguous axis: one wave covers one 256/512-byte row segment)
+// (k is the contiguous axis: one wave covers one 1 KB row segment of the
+// interleaved per-site state)
 // and marches along i over a chunk of CI planes.  Each work-item keeps the
 // drifted field of its RJ sites at planes i-1, i, i+1 in registers; plane i
 // with a one-site halo in j and k sits in LDS (double-buffered, one barrier
@@ -24,6 +25,7 @@
 //   HalfF32   FP32 phi and half-step velocity u:  x = phi + dt u,
 //             ker = F(x), u' = u + dt ker.
 #include "axs_internal.hpp"
+#include "axs_lattice.hpp"
 
 namespace axs {
 
@@ -35,39 +37,29 @@ struct ExactF64 {
     using T = double;
     using S = StepScalars;
     struct In {
-        const double *__restrict__ p1, *__restrict__ p2, *__restrict__ v1, *__restrict__ v2,
-            *__restrict__ k1, *__restrict__ k2;
+        const dvec2 *__restrict__ phi, *__restrict__ vel, *__restrict__ ker;
     };
     struct Out {
-        double *__restrict__ p1, *__restrict__ p2, *__restrict__ v1, *__restrict__ v2,
-            *__restrict__ k1, *__restrict__ k2;
+        dvec2 *__restrict__ phi, *__restrict__ vel, *__restrict__ ker;
     };
     struct Raw {
-        double p1, p2, v1, v2, k1, k2;
+        dvec2 p, v, k;
     };
-    struct Own {
-        double v1, v2, k1, k2;
-    };
+    using Own = Raw;
     static __device__ __forceinline__ double c0(const S &s) { return s.c[0]; }
     static __device__ __forceinline__ void load(const In &a, uint32_t q, Raw &r)
     {
-        r.p1 = a.p1[q];
-        r.p2 = a.p2[q];
-        r.v1 = a.v1[q];
-        r.v2 = a.v2[q];
-        r.k1 = a.k1[q];
-        r.k2 = a.k2[q];
+        r.p = a.phi[q];
+        r.v = a.vel[q];
+        r.k = a.ker[q];
     }
     // drift, s_integrate.cpp:313-314
     static __device__ __forceinline__ void drift(const Raw &r, const S &s, double &x1, double &x2,
                                                  Own &o)
     {
-        o.v1 = r.v1;
-        o.v2 = r.v2;
-        o.k1 = r.k1;
-        o.k2 = r.k2;
-        x1 = r.p1 + s.dt * (r.v1 + 0.5 * r.k1 * s.dt);
-        x2 = r.p2 + s.dt * (r.v2 + 0.5 * r.k2 * s.dt);
+        o = r;
+        x1 = r.p.x + s.dt * (r.v.x + 0.5 * r.k.x * s.dt);
+        x2 = r.p.y + s.dt * (r.v.y + 0.5 * r.k.y * s.dt);
     }
     // s_integrate.cpp:322-340 for one site, sums already in COO order
     template <bool NT>
@@ -77,24 +69,22 @@ struct ExactF64 {
     {
         double lap1 = s.alpha * sum1, lap2 = s.alpha * sum2;
         double S_ = x1 * x1 + x2 * x2 - s.tau2 + s.T03;
-        double kn1 = lap1 + s.cl * x1 * S_;
-        double kn2 = lap2 + s.cl * x2 * S_;
-        double v1 = w.v1 + 0.5 * (w.k1 + kn1) * s.dt;
-        double v2 = w.v2 + 0.5 * (w.k2 + kn2) * s.dt;
-        st<NT>(o.v1 + q, v1);
-        st<NT>(o.v2 + q, v2);
-        st<NT>(o.p1 + q, x1);
-        st<NT>(o.p2 + q, x2);
-        st<NT>(o.k1 + q, kn1);
-        st<NT>(o.k2 + q, kn2);
-    }
-    template <bool NT>
-    static __device__ __forceinline__ void st(double *p, double v)
-    {
-        if (NT)
-            __builtin_nontemporal_store(v, p);
-        else
-            *p = v;
+        dvec2 kn, vn, xn;
+        kn.x = lap1 + s.cl * x1 * S_;
+        kn.y = lap2 + s.cl * x2 * S_;
+        vn.x = w.v.x + 0.5 * (w.k.x + kn.x) * s.dt;
+        vn.y = w.v.y + 0.5 * (w.k.y + kn.y) * s.dt;
+        xn.x = x1;
+        xn.y = x2;
+        if (NT) {
+            store_nt(o.phi + q, xn);
+            store_nt(o.vel + q, vn);
+            store_nt(o.ker + q, kn);
+        } else {
+            o.phi[q] = xn;
+            o.vel[q] = vn;
+            o.ker[q] = kn;
+        }
     }
 };
 
@@ -102,32 +92,24 @@ struct HalfF32 {
     using T = float;
     using S = StepScalarsF;
     struct In {
-        const float *__restrict__ p1, *__restrict__ p2, *__restrict__ u1, *__restrict__ u2;
+        const fvec4 *__restrict__ st;
     };
     struct Out {
-        float *__restrict__ p1, *__restrict__ p2, *__restrict__ u1, *__restrict__ u2;
+        fvec4 *__restrict__ st;
     };
-    struct Raw {
-        float p1, p2, u1, u2;
-    };
+    using Raw = fvec4;
     struct Own {
         float u1, u2;
     };
     static __device__ __forceinline__ float c0(const S &s) { return s.c[0]; }
-    static __device__ __forceinline__ void load(const In &a, uint32_t q, Raw &r)
-    {
-        r.p1 = a.p1[q];
-        r.p2 = a.p2[q];
-        r.u1 = a.u1[q];
-        r.u2 = a.u2[q];
-    }
+    static __device__ __forceinline__ void load(const In &a, uint32_t q, Raw &r) { r = a.st[q]; }
     static __device__ __forceinline__ void drift(const Raw &r, const S &s, float &x1, float &x2,
                                                  Own &o)
     {
-        o.u1 = r.u1;
-        o.u2 = r.u2;
-        x1 = r.p1 + s.dt * r.u1;
-        x2 = r.p2 + s.dt * r.u2;
+        o.u1 = r.z;
+        o.u2 = r.w;
+        x1 = r.x + s.dt * r.z;
+        x2 = r.y + s.dt * r.w;
     }
     template <bool NT>
     static __device__ __forceinline__ void update(const Out &o, uint32_t q, const S &s, float x1,
@@ -136,18 +118,15 @@ struct HalfF32 {
         float S_ = x1 * x1 + x2 * x2 - s.tau2 + s.T03;
         float k1 = s.alpha * sum1 + s.cl * x1 * S_;
         float k2 = s.alpha * sum2 + s.cl * x2 * S_;
-        st<NT>(o.u1 + q, w.u1 + s.dt * k1);
-        st<NT>(o.u2 + q, w.u2 + s.dt * k2);
-        st<NT>(o.p1 + q, x1);
-        st<NT>(o.p2 + q, x2);
-    }
-    template <bool NT>
-    static __device__ __forceinline__ void st(float *p, float v)
-    {
+        fvec4 r;
+        r.x = x1;
+        r.y = x2;
+        r.z = w.u1 + s.dt * k1;
+        r.w = w.u2 + s.dt * k2;
         if (NT)
-            __builtin_nontemporal_store(v, p);
+            store_nt(o.st + q, r);
         else
-            *p = v;
+            o.st[q] = r;
     }
 };
 
@@ -369,12 +348,10 @@ int launch_march3d(Context *c, Member &m, int src, int dst, const StepScalars &s
 {
     const int N = m.N;
     if (c->prec == AXS_FP64) {
-        ExactF64::In in{(const double *)m.state[src][PHI1], (const double *)m.state[src][PHI2],
-                        (const double *)m.state[src][VEL1], (const double *)m.state[src][VEL2],
-                        (const double *)m.state[src][KER1], (const double *)m.state[src][KER2]};
-        ExactF64::Out out{(double *)m.state[dst][PHI1], (double *)m.state[dst][PHI2],
-                          (double *)m.state[dst][VEL1], (double *)m.state[dst][VEL2],
-                          (double *)m.state[dst][KER1], (double *)m.state[dst][KER2]};
+        ExactF64::In in{(const dvec2 *)m.state[src][PHI], (const dvec2 *)m.state[src][VEL],
+                        (const dvec2 *)m.state[src][KER]};
+        ExactF64::Out out{(dvec2 *)m.state[dst][PHI], (dvec2 *)m.state[dst][VEL],
+                          (dvec2 *)m.state[dst][KER]};
         const int f = c->tune.flags & (kTunePrefetch | kTuneNT);
         const int rj = c->tune.rj64;
         if (rj == 2) {
@@ -393,10 +370,8 @@ int launch_march3d(Context *c, Member &m, int src, int dst, const StepScalars &s
         }
     }
     StepScalarsF f = to_float(s);
-    HalfF32::In in{(const float *)m.state[src][PHI1], (const float *)m.state[src][PHI2],
-                   (const float *)m.state[src][VEL1], (const float *)m.state[src][VEL2]};
-    HalfF32::Out out{(float *)m.state[dst][PHI1], (float *)m.state[dst][PHI2],
-                     (float *)m.state[dst][VEL1], (float *)m.state[dst][VEL2]};
+    HalfF32::In in{(const fvec4 *)m.state[src][PHI]};
+    HalfF32::Out out{(fvec4 *)m.state[dst][PHI]};
     const int fl = c->tune.flags & (kTunePrefetch | kTuneNT);
     if (c->tune.rj32 == 8 && N % 32 == 0) {
         switch (fl) {

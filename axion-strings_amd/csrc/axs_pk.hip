@@ -19,6 +19,7 @@
 #include <math.h>
 
 #include "axs_internal.hpp"
+#include "axs_lattice.hpp"
 
 namespace axs {
 
@@ -95,12 +96,16 @@ static int get_plan(Context *c, int N, FftPlan **out)
     return AXS_OK;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) to_double_kernel(uint32_t L, const T *__restrict__ x,
-                                                        double *__restrict__ y)
+// phi2 of the current state as a contiguous double array
+template <class P>
+__global__ void __launch_bounds__(256) phi2_kernel(uint32_t L, P phi, double *__restrict__ y)
 {
     uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < L) y[m] = (double)x[m];
+    if (m < L) {
+        double a, b;
+        phi.get(m, a, b);
+        y[m] = b;
+    }
 }
 
 struct KahanP {
@@ -231,11 +236,11 @@ int run_powerspec(Context *c, Member &m, int binning, int nbins, double *ks, dou
     uint32_t L = (uint32_t)m.L;
     unsigned g = (L + 255) / 256;
     if (c->prec == AXS_FP64)
-        hipLaunchKernelGGL(to_double_kernel<double>, dim3(g), dim3(256), 0, c->stream, L,
-                           (const double *)fv.phi2, plan->in);
+        hipLaunchKernelGGL(phi2_kernel<PhiF64>, dim3(g), dim3(256), 0, c->stream, L,
+                           PhiF64{(const dvec2 *)fv.phi}, plan->in);
     else
-        hipLaunchKernelGGL(to_double_kernel<float>, dim3(g), dim3(256), 0, c->stream, L,
-                           (const float *)fv.phi2, plan->in);
+        hipLaunchKernelGGL(phi2_kernel<PhiF32>, dim3(g), dim3(256), 0, c->stream, L,
+                           PhiF32{(const fvec4 *)fv.phi}, plan->in);
     AXS_HIP_TRY(hipGetLastError());
     if (rocfft_execution_info_set_stream(plan->info, c->stream) != rocfft_status_success) {
         set_error("rocfft_execution_info_set_stream failed");

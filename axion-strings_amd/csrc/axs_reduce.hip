@@ -64,10 +64,9 @@ __device__ __forceinline__ void block_reduce(Kahan (&k)[Q], Kahan (*sh)[kRedBloc
     }
 }
 
-template <typename T>
+template <class P, class V>
 __global__ void __launch_bounds__(kRedBlock)
-background_kernel(uint32_t L, const T *__restrict__ p1, const T *__restrict__ p2,
-                  const T *__restrict__ v1, const T *__restrict__ v2, Kahan *__restrict__ part)
+background_kernel(uint32_t L, P phi, V vel, Kahan *__restrict__ part)
 {
     __shared__ Kahan sh[6][kRedBlock / 64];
     Kahan k[6];
@@ -75,11 +74,13 @@ background_kernel(uint32_t L, const T *__restrict__ p1, const T *__restrict__ p2
     for (int q = 0; q < 6; ++q) k[q] = Kahan{0.0, 0.0};
     uint32_t stride = gridDim.x * kRedBlock;
     for (uint32_t m = blockIdx.x * kRedBlock + threadIdx.x; m < L; m += stride) {
-        double a = (double)p1[m], b = (double)p2[m];
+        double a, b, va, vb;
+        phi.get(m, a, b);
+        vel.get(m, va, vb);
         kadd(k[0], a);
         kadd(k[1], b);
-        kadd(k[2], (double)v1[m]);
-        kadd(k[3], (double)v2[m]);
+        kadd(k[2], va);
+        kadd(k[3], vb);
         kadd(k[4], fabs(atan2(a, b)));            // s_run.cpp:145
         kadd(k[5], sqrt(a * a + b * b));          // s_run.cpp:146
     }
@@ -88,11 +89,10 @@ background_kernel(uint32_t L, const T *__restrict__ p1, const T *__restrict__ p2
         for (int q = 0; q < 6; ++q) part[blockIdx.x * 6 + q] = k[q];
 }
 
-template <int ND, typename T>
+template <int ND, class P, class V>
 __global__ void __launch_bounds__(kRedBlock)
-energy_kernel(int N, uint32_t L, const T *__restrict__ p1, const T *__restrict__ p2,
-              const T *__restrict__ v1, const T *__restrict__ v2, double tau2, double T03,
-              double lam_4tau2, Kahan *__restrict__ part)
+energy_kernel(int N, uint32_t L, P phi, V vel, double tau2, double T03, double lam_4tau2,
+              Kahan *__restrict__ part)
 {
     __shared__ Kahan sh[3][kRedBlock / 64];
     Kahan k[3];
@@ -101,15 +101,18 @@ energy_kernel(int N, uint32_t L, const T *__restrict__ p1, const T *__restrict__
     uint32_t stride = gridDim.x * kRedBlock;
     for (uint32_t m = blockIdx.x * kRedBlock + threadIdx.x; m < L; m += stride) {
         Site<ND> s = decompose<ND>(m, N);
-        double a = (double)p1[m], b = (double)p2[m];
+        double a, b, va, vb;
+        phi.get(m, a, b);
+        vel.get(m, va, vb);
         double g = 0.0;
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
             uint32_t q = shifted<ND>(s, d, 1, N);
-            double d1 = (double)p1[q] - a, d2 = (double)p2[q] - b;
+            double n1, n2;
+            phi.get(q, n1, n2);
+            double d1 = n1 - a, d2 = n2 - b;
             g += d1 * d1 + d2 * d2;
         }
-        double va = (double)v1[m], vb = (double)v2[m];
         double S = a * a + b * b - tau2 + T03;
         kadd(k[0], 0.5 * (va * va + vb * vb));
         kadd(k[1], 0.5 * g);
@@ -147,13 +150,13 @@ int run_background(Context *c, Member &m, double out[6])
     Kahan *part = (Kahan *)c->scratch;
     double *dout = (double *)((char *)c->scratch + part_b);
     if (c->prec == AXS_FP64)
-        hipLaunchKernelGGL(background_kernel<double>, dim3(nb), dim3(kRedBlock), 0, c->stream, L,
-                           (const double *)fv.phi1, (const double *)fv.phi2,
-                           (const double *)fv.vel1, (const double *)fv.vel2, part);
+        hipLaunchKernelGGL((background_kernel<PhiF64, VelF64>), dim3(nb), dim3(kRedBlock), 0,
+                           c->stream, L, PhiF64{(const dvec2 *)fv.phi},
+                           VelF64{(const dvec2 *)fv.vel}, part);
     else
-        hipLaunchKernelGGL(background_kernel<float>, dim3(nb), dim3(kRedBlock), 0, c->stream, L,
-                           (const float *)fv.phi1, (const float *)fv.phi2,
-                           (const float *)fv.vel1, (const float *)fv.vel2, part);
+        hipLaunchKernelGGL((background_kernel<PhiF32, VelF32>), dim3(nb), dim3(kRedBlock), 0,
+                           c->stream, L, PhiF32{(const fvec4 *)fv.phi},
+                           VelF32{(const fvec2 *)fv.vel}, part);
     hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(64), 0, c->stream, part, nb, 6, dout);
     AXS_HIP_TRY(hipGetLastError());
     AXS_HIP_TRY(hipMemcpyAsync(out, dout, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -176,17 +179,15 @@ int run_energy(Context *c, Member &m, double out[3])
     double T03 = (T0 * T0) / 3.0;
     double lam_4tau2 = lam / (4.0 * tau2);
     if (c->prec == AXS_FP64) {
-        auto k = (c->nd == 2) ? energy_kernel<2, double> : energy_kernel<3, double>;
+        auto k = (c->nd == 2) ? energy_kernel<2, PhiF64, VelF64> : energy_kernel<3, PhiF64, VelF64>;
         hipLaunchKernelGGL(k, dim3(nb), dim3(kRedBlock), 0, c->stream, m.N, L,
-                           (const double *)fv.phi1, (const double *)fv.phi2,
-                           (const double *)fv.vel1, (const double *)fv.vel2, tau2, T03, lam_4tau2,
-                           part);
+                           PhiF64{(const dvec2 *)fv.phi}, VelF64{(const dvec2 *)fv.vel}, tau2, T03,
+                           lam_4tau2, part);
     } else {
-        auto k = (c->nd == 2) ? energy_kernel<2, float> : energy_kernel<3, float>;
+        auto k = (c->nd == 2) ? energy_kernel<2, PhiF32, VelF32> : energy_kernel<3, PhiF32, VelF32>;
         hipLaunchKernelGGL(k, dim3(nb), dim3(kRedBlock), 0, c->stream, m.N, L,
-                           (const float *)fv.phi1, (const float *)fv.phi2,
-                           (const float *)fv.vel1, (const float *)fv.vel2, tau2, T03, lam_4tau2,
-                           part);
+                           PhiF32{(const fvec4 *)fv.phi}, VelF32{(const fvec2 *)fv.vel}, tau2, T03,
+                           lam_4tau2, part);
     }
     hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(64), 0, c->stream, part, nb, 3, dout);
     AXS_HIP_TRY(hipGetLastError());

@@ -21,11 +21,12 @@ constexpr int kStrBlock = 256;
 constexpr int kStrIters = 8;                    // sites per thread per block
 constexpr int kStrSpan = kStrBlock * kStrIters; // sites per block
 
-template <typename T>
-__device__ __forceinline__ double corner_angle(const T *__restrict__ p1, const T *__restrict__ p2,
-                                               uint32_t q)
+template <class P>
+__device__ __forceinline__ double corner_angle(const P &phi, uint32_t q)
 {
-    double a = atan2((double)p1[q], (double)p2[q]);
+    double a1, a2;
+    phi.get(q, a1, a2);
+    double a = atan2(a1, a2);
     return (a + M_PI) / (2 * M_PI);
 }
 
@@ -37,9 +38,8 @@ __device__ __forceinline__ double edge_theta(double a, double b)
     return (e < d) ? e : d;
 }
 
-template <int ND, typename T>
-__device__ __forceinline__ bool plaquette_flag(const T *__restrict__ p1, const T *__restrict__ p2,
-                                               uint32_t m, int N, double accept)
+template <int ND, class P>
+__device__ __forceinline__ bool plaquette_flag(const P &phi, uint32_t m, int N, double accept)
 {
     Site<ND> s = decompose<ND>(m, N);
     if (ND == 2) {
@@ -48,10 +48,10 @@ __device__ __forceinline__ bool plaquette_flag(const T *__restrict__ p1, const T
         Site<ND> s2 = decompose<ND>(q2, N);
         uint32_t q3 = shifted<ND>(s2, 1, 1, N);
         uint32_t q4 = shifted<ND>(s, 1, 1, N);
-        double n1 = corner_angle(p1, p2, m);
-        double n2 = corner_angle(p1, p2, q2);
-        double n3 = corner_angle(p1, p2, q3);
-        double n4 = corner_angle(p1, p2, q4);
+        double n1 = corner_angle(phi, m);
+        double n2 = corner_angle(phi, q2);
+        double n3 = corner_angle(phi, q3);
+        double n4 = corner_angle(phi, q4);
         double sum = edge_theta(n1, n2) + edge_theta(n2, n3) + edge_theta(n3, n4);
         return sum > accept;
     } else {
@@ -63,12 +63,12 @@ __device__ __forceinline__ bool plaquette_flag(const T *__restrict__ p1, const T
         Site<ND> sj = decompose<ND>(qj, N);
         uint32_t qjk = shifted<ND>(sj, 2, 1, N);                   // (i,   j+1, k+1)
         uint32_t qk = shifted<ND>(s, 2, 1, N);                     // (i,   j,   k+1)
-        double n0 = corner_angle(p1, p2, m);
-        double ni = corner_angle(p1, p2, qi);
-        double nij = corner_angle(p1, p2, qij);
-        double nj = corner_angle(p1, p2, qj);
-        double njk = corner_angle(p1, p2, qjk);
-        double nk = corner_angle(p1, p2, qk);
+        double n0 = corner_angle(phi, m);
+        double ni = corner_angle(phi, qi);
+        double nij = corner_angle(phi, qij);
+        double nj = corner_angle(phi, qj);
+        double njk = corner_angle(phi, qjk);
+        double nk = corner_angle(phi, qk);
         double sa = edge_theta(n0, ni) + edge_theta(ni, nij) + edge_theta(nij, nj);
         double sb = edge_theta(n0, ni) + edge_theta(ni, njk) + edge_theta(njk, nk);
         double sc = edge_theta(n0, nj) + edge_theta(nj, njk) + edge_theta(njk, nk);
@@ -76,10 +76,10 @@ __device__ __forceinline__ bool plaquette_flag(const T *__restrict__ p1, const T
     }
 }
 
-template <int ND, typename T>
+template <int ND, class P>
 __global__ void __launch_bounds__(kStrBlock)
-strings_flag_kernel(int N, uint32_t L, const T *__restrict__ p1, const T *__restrict__ p2,
-                    double accept, uint64_t *__restrict__ words, uint32_t *__restrict__ bcount)
+strings_flag_kernel(int N, uint32_t L, P phi, double accept, uint64_t *__restrict__ words,
+                    uint32_t *__restrict__ bcount)
 {
     __shared__ uint32_t wave_tot[kStrBlock / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -87,7 +87,7 @@ strings_flag_kernel(int N, uint32_t L, const T *__restrict__ p1, const T *__rest
     uint32_t base = blockIdx.x * (uint32_t)kStrSpan;
     for (int it = 0; it < kStrIters; ++it) {
         uint32_t m = base + it * kStrBlock + threadIdx.x;
-        bool f = (m < L) && plaquette_flag<ND, T>(p1, p2, m, N, accept);
+        bool f = (m < L) && plaquette_flag<ND, P>(phi, m, N, accept);
         uint64_t w = __ballot(f);
         uint32_t wi = (base + it * kStrBlock) / 64 + wave;
         if (lane == 0 && (uint64_t)wi * 64 < L) words[wi] = w;
@@ -210,14 +210,13 @@ int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *co
 
     double accept = accept_threshold(c->p.thr);
     if (c->prec == AXS_FP64) {
-        auto k = (c->nd == 2) ? strings_flag_kernel<2, double> : strings_flag_kernel<3, double>;
+        auto k = (c->nd == 2) ? strings_flag_kernel<2, PhiF64> : strings_flag_kernel<3, PhiF64>;
         hipLaunchKernelGGL(k, dim3(nblocks), dim3(kStrBlock), 0, c->stream, m.N, L,
-                           (const double *)fv.phi1, (const double *)fv.phi2, accept, words,
-                           bcount);
+                           PhiF64{(const dvec2 *)fv.phi}, accept, words, bcount);
     } else {
-        auto k = (c->nd == 2) ? strings_flag_kernel<2, float> : strings_flag_kernel<3, float>;
+        auto k = (c->nd == 2) ? strings_flag_kernel<2, PhiF32> : strings_flag_kernel<3, PhiF32>;
         hipLaunchKernelGGL(k, dim3(nblocks), dim3(kStrBlock), 0, c->stream, m.N, L,
-                           (const float *)fv.phi1, (const float *)fv.phi2, accept, words, bcount);
+                           PhiF32{(const fvec4 *)fv.phi}, accept, words, bcount);
     }
     hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, c->stream, bcount, off,
                        nblocks);
