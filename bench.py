@@ -105,7 +105,7 @@ def cpu_baseline(np, seconds: float):
     A = O.Matrix(3, Ns)
     st.run(p, A, 1)                             # first touch
     n, t = 0, 0.0
-    while t < seconds and n < 200:
+    while t < seconds and n < 5000:
         t0 = time.perf_counter()
         st.run(p, A, 1)
         t += time.perf_counter() - t0
@@ -122,6 +122,27 @@ def cpu_baseline(np, seconds: float):
     }
 
 
+def copy_calibration(nbytes: int) -> int:
+    """One device-to-device copy of nbytes through the HIP runtime: a copy
+    kernel that reads and writes exactly nbytes, to scale FETCH_SIZE and
+    WRITE_SIZE readings against."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(nbytes)) != 0:
+            print("hipMalloc failed", file=sys.stderr)
+            return 1
+    hip.hipMemset(a, 0, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    rc = hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)      # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "rc": rc}), flush=True)
+    return 0 if rc == 0 else 1
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +151,11 @@ def main() -> int:
     ap.add_argument("--N", type=int, default=1024)
     ap.add_argument("--T-initial", type=float, default=4.0)
     ap.add_argument("--skip-fp64", action="store_true", help="time the FP32 form only")
+    ap.add_argument("--forms", default="fp32,fp64",
+                    help="forms to time (fp32 is the headline; fp64 alone for profiling)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (for calibrating HBM "
+                         "traffic counters) and exit")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     args = ap.parse_args()
@@ -139,14 +165,23 @@ def main() -> int:
               "(the multi-GPU slab decomposition is a design only)", file=sys.stderr)
         return 2
 
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
+
     import numpy as np
     import axs
 
     N, K, W = args.N, args.steps, args.warmup
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    if forms == ["fp64"]:
+        lat64, r64 = gpu_leg(np, axs, N, "fp64", W, K, args.T_initial)
+        lat64.close()
+        print(json.dumps({"form": "fp64", **r64}), flush=True)
+        return 0
     lat32, r32 = gpu_leg(np, axs, N, "fp32", W, K, args.T_initial)
     out64 = None
     acc = None
-    if not args.skip_fp64:
+    if not args.skip_fp64 and "fp64" in forms:
         lat64, r64 = gpu_leg(np, axs, N, "fp64", W, K, args.T_initial)
         log("[bench] comparing FP32 with FP64")
         acc = accuracy(np, lat32, lat64)
