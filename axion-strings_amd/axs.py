@@ -331,3 +331,67 @@ class Lattice:
                                              cnt.ctypes.data_as(POINTER(c_int32))),
                "axs_member_powerspec")
         return ks, pk, cnt
+
+
+# ---------------------------------------------------------------------------
+# host driver library (libaxs_host.so): parameter file, host initial
+# conditions and FFT, run loop
+
+HOST_LIB_PATH = os.path.join(_HERE, "libaxs_host.so")
+DRIVER_PATH = os.path.join(_HERE, "axs_main")
+_host = None
+
+
+def load_host_library(path: str = HOST_LIB_PATH) -> ctypes.CDLL:
+    global _host
+    if _host is not None:
+        return _host
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `make -C axion-strings_amd`")
+    h = ctypes.CDLL(path)
+    h.axs_host_params_json.argtypes = [ctypes.c_char_p, ctypes.c_char_p, c_int]
+    h.axs_host_params_json.restype = c_int
+    h.axs_host_init_fields.argtypes = [c_int, c_int, c_int, c_float, c_float, c_float,
+                                       ctypes.c_uint64] + [_DP] * 4
+    h.axs_host_init_fields.restype = c_int
+    h.axs_host_fft.argtypes = [_DP, c_int, c_int, c_int]
+    h.axs_host_fft.restype = c_int
+    h.axs_host_final_step.argtypes = [c_int, c_float, c_float]
+    h.axs_host_final_step.restype = c_int
+    h.axs_host_xi.argtypes = [c_int, c_int64, c_double, c_int, c_float]
+    h.axs_host_xi.restype = c_float
+    h.axs_host_run.argtypes = [ctypes.c_char_p]
+    h.axs_host_run.restype = c_int
+    _host = h
+    return h
+
+
+def read_params(path: str) -> dict:
+    import json
+    h = load_host_library()
+    buf = ctypes.create_string_buffer(1 << 14)
+    rc = h.axs_host_params_json(path.encode(), buf, len(buf))
+    if rc != 0:
+        raise ValueError(f"parameter file {path}: error {rc}")
+    return json.loads(buf.value.decode())
+
+
+def host_init(mode: str, ndims: int, N: int, space_step: float = 0.5, lam: float = 1.0,
+              T_initial: float = 4.0, seed: int = 1998):
+    """host initial fields: mode "kinv" or "thermal"; returns phi1, phi2, phidot1, phidot2"""
+    h = load_host_library()
+    L = N ** ndims
+    out = [np.empty(L) for _ in range(4)]
+    rc = h.axs_host_init_fields(0 if mode == "kinv" else 1, ndims, N, space_step, lam, T_initial,
+                                seed, *[_dp(a) for a in out])
+    if rc != 0:
+        raise ValueError(f"axs_host_init_fields: {rc}")
+    return out
+
+
+def host_fft(data, ndims: int, N: int, sign: int = -1) -> np.ndarray:
+    h = load_host_library()
+    d = np.ascontiguousarray(data, dtype=np.complex128).reshape(-1).copy()
+    if h.axs_host_fft(d.view(np.float64).ctypes.data_as(_DP), ndims, N, sign) != 0:
+        raise ValueError("axs_host_fft failed")
+    return d

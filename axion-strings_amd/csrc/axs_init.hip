@@ -270,7 +270,8 @@ int run_init(Context *c, Member &m, int mode, uint64_t seed)
     sp.N = m.N;
     sp.twopi_over_L = 2.0 * M_PI / ((double)m.N * dx);
     sp.kcut = M_PI / (4.0 * dx);
-    sp.meff2 = (double)(c->p.lambda * (T0f * T0f / 3.0f - 1.0f));      // physics.cpp:19
+    // physics.cpp:19: lambda * (pow_2(T0) / 3.0f - 1.0f), double, stored as float
+    sp.meff2 = (double)(float)((double)c->p.lambda * ((double)T0f * (double)T0f / 3.0 - 1.0));
     sp.T0 = (double)T0f;
     sp.seed = seed;
     sp.mode = mode;
