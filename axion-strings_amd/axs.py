@@ -46,7 +46,7 @@ EXPORTS = (
     "axs_string_counts", "axs_member_background", "axs_background",
     "axs_member_energy", "axs_energy", "axs_member_powerspec", "axs_powerspec",
     "axs_last_step_ms", "axs_bytes_per_site", "axs_strerror", "axs_last_error",
-    "axs_set_kernel_config", "axs_get_kernel_config",
+    "axs_set_kernel_config", "axs_get_kernel_config", "axs_autotune",
 )
 
 AXS_KERNEL_AUTO = 0
@@ -65,6 +65,7 @@ class AxsKernelConfig(ctypes.Structure):
 AXS_MARCH_PREFETCH = 1
 AXS_MARCH_NT_STORE = 2
 AXS_MARCH_JFAST = 4
+AXS_STEP_GRAPH = 8
 
 
 class AxsParams(ctypes.Structure):
@@ -131,6 +132,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "axs_bytes_per_site": ([c_void_p, _DP, _DP], c_int),
         "axs_set_kernel_config": ([c_void_p, POINTER(AxsKernelConfig)], c_int),
         "axs_get_kernel_config": ([c_void_p, POINTER(AxsKernelConfig)], c_int),
+        "axs_autotune": ([c_void_p, c_int, POINTER(AxsKernelConfig), _DP], c_int),
         "axs_strerror": ([c_int], ctypes.c_char_p),
         "axs_last_error": ([], ctypes.c_char_p),
     }
@@ -270,6 +272,15 @@ class Lattice:
                "axs_get_kernel_config")
         return {"kernel": cfg.kernel, "rows": cfg.rows, "chunk": cfg.chunk, "depth": cfg.depth,
                 "flags": cfg.flags}
+
+    def autotune(self, trials: int = 0):
+        """choose the fastest launch shape; returns (config dict, ms per step)"""
+        cfg = AxsKernelConfig()
+        ms = c_double()
+        _check(self.lib.axs_autotune(self._ctx, trials, ctypes.byref(cfg), ctypes.byref(ms)),
+               "axs_autotune")
+        return ({"kernel": cfg.kernel, "rows": cfg.rows, "chunk": cfg.chunk, "depth": cfg.depth,
+                 "flags": cfg.flags}, ms.value)
 
     @property
     def last_step_ms(self) -> float:

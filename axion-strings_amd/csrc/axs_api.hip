@@ -218,11 +218,13 @@ static void unpack32_fn(size_t lo, size_t hi, void *arg)
 }
 
 void destroy_plans(Context *c);
+void destroy_inverse_plans(Context *c);
 
 Context::~Context()
 {
     if (stream) (void)hipStreamSynchronize(stream);
     destroy_plans(this);
+    destroy_inverse_plans(this);
     for (Member &m : members) {
         for (int h = 0; h < 2; ++h)
             for (int a = 0; a < 3; ++a)
@@ -232,6 +234,8 @@ Context::~Context()
     }
     if (scratch) (void)hipFree(scratch);
     if (listbuf) (void)hipFree(listbuf);
+    for (int h = 0; h < 2; ++h)
+        if (batch_tab[h]) (void)hipFree(batch_tab[h]);
     if (pinned) (void)hipHostFree(pinned);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -576,16 +580,51 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
             return AXS_ERR_STATE;
         }
     if (nsteps == 0) return AXS_OK;
+    AXS_TRY(prepare_batch(c));
     const double dt = (double)c->p.time_step;
-    AXS_HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    for (int64_t n = 0; n < nsteps; ++n) {
+    const bool graph = (c->tune.flags & kTuneGraph) && nsteps > 1;
+    const double tau0 = c->tau;
+    const int cur0 = c->cur;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    if (graph) AXS_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = AXS_OK;
+    for (int64_t n = 0; n < nsteps && rc == AXS_OK; ++n) {
         double tau_next = c->tau + dt;                    // s_integrate.cpp:317
         StepScalars s = make_scalars(c->p, tau_next);
-        AXS_TRY(launch_step(c, s));
+        rc = launch_step(c, s);
         c->tau = tau_next;
+    }
+    if (graph) {
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc == AXS_OK && e == hipSuccess)
+            e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (rc == AXS_OK && e != hipSuccess) {
+            set_error("step graph capture failed: %s", hipGetErrorString(e));
+            rc = AXS_ERR_HIP;
+        }
+    }
+    if (rc != AXS_OK) {
+        if (ge) (void)hipGraphExecDestroy(ge);
+        if (g) (void)hipGraphDestroy(g);
+        c->tau = tau0;
+        c->cur = cur0;
+        return rc;
+    }
+    AXS_HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    if (graph) {
+        hipError_t e = hipGraphLaunch(ge, c->stream);
+        if (e != hipSuccess) {
+            (void)hipGraphExecDestroy(ge);
+            (void)hipGraphDestroy(g);
+            set_error("hipGraphLaunch failed: %s", hipGetErrorString(e));
+            return AXS_ERR_HIP;
+        }
     }
     AXS_HIP_TRY(hipEventRecord(c->ev1, c->stream));
     AXS_HIP_TRY(hipEventSynchronize(c->ev1));
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
     float ms = 0.f;
     AXS_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_step_ms = (double)ms / (double)nsteps;
@@ -706,6 +745,16 @@ int axs_get_kernel_config(const axs_ctx *ctx, axs_kernel_config *cfg)
     cfg->chunk = (c->prec == AXS_FP32) ? c->tune.ci32 : c->tune.ci64;
     cfg->depth = c->tune.depth;
     cfg->flags = c->tune.flags;
+    return AXS_OK;
+}
+
+int axs_autotune(axs_ctx *ctx, int trials, axs_kernel_config *best, double *ms_per_step)
+{
+    AXS_TRY(check_member(ctx, 0));
+    double ms = 0.0;
+    AXS_TRY(run_autotune(&ctx->impl, trials, &ms));
+    if (best) AXS_TRY(axs_get_kernel_config(ctx, best));
+    if (ms_per_step) *ms_per_step = ms;
     return AXS_OK;
 }
 

@@ -207,44 +207,66 @@ store_init32_kernel(uint32_t L, const double2 *__restrict__ z, const double2 *__
     ker[m] = k;
 }
 
-static int inverse_fft(Context *c, int N, double2 *buf)
-{
+// in-place complex inverse transform; plans and work buffers are cached per
+// side in the context (ensembles initialise many members of a few sides)
+struct InvPlan {
+    int N = 0, nd = 0;
     rocfft_plan plan = nullptr;
     rocfft_execution_info info = nullptr;
-    size_t lengths[3] = {(size_t)N, (size_t)N, (size_t)N};
-    if (rocfft_setup() != rocfft_status_success ||
-        rocfft_plan_create(&plan, rocfft_placement_inplace,
-                           rocfft_transform_type_complex_inverse, rocfft_precision_double,
-                           (size_t)c->nd, lengths, 1, nullptr) != rocfft_status_success) {
-        set_error("rocfft inverse plan failed (N=%d)", N);
+    void *work = nullptr;
+};
+
+void destroy_inverse_plans(Context *c)
+{
+    for (void *v : c->inv_plans) {
+        InvPlan *p = (InvPlan *)v;
+        if (p->plan) rocfft_plan_destroy(p->plan);
+        if (p->info) rocfft_execution_info_destroy(p->info);
+        if (p->work) (void)hipFree(p->work);
+        delete p;
+    }
+    c->inv_plans.clear();
+}
+
+static int inverse_fft(Context *c, int N, double2 *buf)
+{
+    InvPlan *P = nullptr;
+    for (void *v : c->inv_plans)
+        if (((InvPlan *)v)->N == N && ((InvPlan *)v)->nd == c->nd) P = (InvPlan *)v;
+    if (!P) {
+        P = new InvPlan();
+        P->N = N;
+        P->nd = c->nd;
+        size_t lengths[3] = {(size_t)N, (size_t)N, (size_t)N};
+        size_t wb = 0;
+        bool ok = rocfft_setup() == rocfft_status_success &&
+                  rocfft_plan_create(&P->plan, rocfft_placement_inplace,
+                                     rocfft_transform_type_complex_inverse,
+                                     rocfft_precision_double, (size_t)c->nd, lengths, 1,
+                                     nullptr) == rocfft_status_success &&
+                  rocfft_plan_get_work_buffer_size(P->plan, &wb) == rocfft_status_success &&
+                  rocfft_execution_info_create(&P->info) == rocfft_status_success;
+        if (ok && wb) {
+            ok = hipMalloc(&P->work, wb) == hipSuccess &&
+                 rocfft_execution_info_set_work_buffer(P->info, P->work, wb) ==
+                     rocfft_status_success;
+        }
+        c->inv_plans.push_back(P);
+        if (!ok) {
+            set_error("rocfft inverse plan failed (N=%d)", N);
+            return AXS_ERR_FFT;
+        }
+    }
+    if (rocfft_execution_info_set_stream(P->info, c->stream) != rocfft_status_success) {
+        set_error("rocfft_execution_info_set_stream failed");
         return AXS_ERR_FFT;
     }
-    size_t wb = 0;
-    void *work = nullptr;
-    int rc = AXS_OK;
-    if (rocfft_plan_get_work_buffer_size(plan, &wb) != rocfft_status_success ||
-        rocfft_execution_info_create(&info) != rocfft_status_success) {
-        rc = AXS_ERR_FFT;
-    } else {
-        if (wb) {
-            if (hipMalloc(&work, wb) != hipSuccess)
-                rc = AXS_ERR_OUT_OF_MEMORY;
-            else if (rocfft_execution_info_set_work_buffer(info, work, wb) != rocfft_status_success)
-                rc = AXS_ERR_FFT;
-        }
-        if (rc == AXS_OK &&
-            rocfft_execution_info_set_stream(info, c->stream) != rocfft_status_success)
-            rc = AXS_ERR_FFT;
-        void *ib[1] = {buf};
-        if (rc == AXS_OK && rocfft_execute(plan, ib, nullptr, info) != rocfft_status_success)
-            rc = AXS_ERR_FFT;
-        if (rc == AXS_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = AXS_ERR_HIP;
+    void *ib[1] = {buf};
+    if (rocfft_execute(P->plan, ib, nullptr, P->info) != rocfft_status_success) {
+        set_error("rocfft inverse transform failed (N=%d)", N);
+        return AXS_ERR_FFT;
     }
-    if (work) (void)hipFree(work);
-    if (info) rocfft_execution_info_destroy(info);
-    if (plan) rocfft_plan_destroy(plan);
-    if (rc != AXS_OK) set_error("rocfft inverse transform failed (N=%d)", N);
-    return rc;
+    return AXS_OK;
 }
 
 int run_init(Context *c, Member &m, int mode, uint64_t seed)

@@ -85,6 +85,15 @@ struct Member {
 
 struct FftPlan;
 
+// one member of an ensemble as seen by the one-launch-per-step kernels
+struct BatchMember {
+    int N, nTj, nTk, CI;     // 3D march tiling (2D: N only)
+    uint32_t item_begin;     // first work item: 3D march tile, 2D 256-site block
+    uint32_t L;
+    void *in[3];             // state[src] slots
+    void *out[3];            // state[dst] slots
+};
+
 // march-kernel variant switches (Tune::flags)
 constexpr int kTunePrefetch = 1;   // keep the next plane's loads in flight
 constexpr int kTuneNT = 2;         // non-temporal stores of the new state
@@ -99,6 +108,8 @@ struct Tune {
     int depth = 1;
     int flags = kTunePrefetch | kTuneNT;
 };
+
+constexpr int kTuneGraph = 8;      // capture the step loop in a HIP graph
 
 struct Context {
     axs_params p{};
@@ -128,6 +139,14 @@ struct Context {
     size_t pinned_bytes = 0;
 
     std::vector<FftPlan *> fft_plans;
+    std::vector<void *> inv_plans;   // device-init inverse transforms (axs_init.hip)
+
+    // ensemble tables (device), one per ping-pong parity, rebuilt when the
+    // launch shape changes
+    BatchMember *batch_tab[2] = {nullptr, nullptr};
+    uint32_t batch_items = 0;
+    int batch_key = -1;          // rows * 100000 + chunk the tables were built for
+    bool batch_march = false;    // 3D ensemble on the march kernel
 
     ~Context();
 };
@@ -148,6 +167,12 @@ int launch_step(Context *c, const StepScalars &s);
 // 3D 2.5D streaming step (stencil_setting 0, N a multiple of 64)
 bool march3d_supported(const Context *c, int N);
 int launch_march3d(Context *c, Member &m, int src, int dst, const StepScalars &s);
+// march geometry chosen for a member (rows per work-item, planes per chunk)
+int march_rows(const Context *c);
+int march_chunk(const Context *c, int N);
+// ensemble launches (one per step for all members)
+int launch_march3d_batch(Context *c, int src, const StepScalars &s);
+int prepare_batch(Context *c);
 
 // FP32: compute ker = F(phi, tau) and phidot = u - dt/2 ker into aux buffers
 int launch_fp32_canonical(Context *c, Member &m, const StepScalars &s, bool recompute_ker);
@@ -156,6 +181,7 @@ int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *co
                 int64_t *num_cores);
 int run_background(Context *c, Member &m, double out[6]);
 int run_init(Context *c, Member &m, int mode, uint64_t seed);
+int run_autotune(Context *c, int trials, double *best_ms);
 int run_energy(Context *c, Member &m, double out[3]);
 int run_powerspec(Context *c, Member &m, int binning, int nbins, double *ks, double *pk,
                   int32_t *count);

@@ -47,6 +47,14 @@ struct ExactF64 {
     };
     using Own = Raw;
     static __device__ __forceinline__ double c0(const S &s) { return s.c[0]; }
+    static __device__ __forceinline__ In make_in(void *const *p)
+    {
+        return In{(const dvec2 *)p[0], (const dvec2 *)p[1], (const dvec2 *)p[2]};
+    }
+    static __device__ __forceinline__ Out make_out(void *const *p)
+    {
+        return Out{(dvec2 *)p[0], (dvec2 *)p[1], (dvec2 *)p[2]};
+    }
     static __device__ __forceinline__ void load(const In &a, uint32_t q, Raw &r)
     {
         r.p = a.phi[q];
@@ -102,6 +110,8 @@ struct HalfF32 {
         float u1, u2;
     };
     static __device__ __forceinline__ float c0(const S &s) { return s.c[0]; }
+    static __device__ __forceinline__ In make_in(void *const *p) { return In{(const fvec4 *)p[0]}; }
+    static __device__ __forceinline__ Out make_out(void *const *p) { return Out{(fvec4 *)p[0]}; }
     static __device__ __forceinline__ void load(const In &a, uint32_t q, Raw &r) { r = a.st[q]; }
     static __device__ __forceinline__ void drift(const Raw &r, const S &s, float &x1, float &x2,
                                                  Own &o)
@@ -140,37 +150,30 @@ struct MarchGeom {
 
 // PF: prefetch plane i+2 while plane i is computed (two planes of raw loads
 // in flight per work-item).  NT: non-temporal stores of the new state.
+// One work-group marches tile (tj, tk) of the lattice of side N over planes
+// [ch*CI, ch*CI + CI).
+template <class Tr, int RJ>
+struct MarchLds {
+    static constexpr int TJ = kMW * RJ;
+    static constexpr int PL = (TJ + 2) * kLDK;    // one component of one LDS plane
+    typename Tr::T v[2][2][PL];
+};
+
 template <class Tr, int RJ, bool PF, bool NT>
-__global__ void __launch_bounds__(kMW * 64)
-march3d_kernel(MarchGeom g, typename Tr::In in, typename Tr::Out out, typename Tr::S s)
+__device__ __forceinline__ void march_tile(const int N, const int CI, const int tj, const int tk,
+                                           const int ch, const typename Tr::In &in,
+                                           const typename Tr::Out &out,
+                                           const typename Tr::S &s, MarchLds<Tr, RJ> &sh)
 {
     using T = typename Tr::T;
     using Raw = typename Tr::Raw;
     using Own = typename Tr::Own;
     constexpr int TJ = kMW * RJ;
-    constexpr int PL = (TJ + 2) * kLDK;          // one component of one LDS plane
-    __shared__ T lds[2][2][PL];
+    auto &lds = sh.v;
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int N = g.N;
     const uint32_t NN = (uint32_t)N * (uint32_t)N;
-
-    uint32_t b = blockIdx.x;
-    if (g.remap) b = (b & 7u) * (gridDim.x >> 3) + (b >> 3);
-    int tj, tk;
-    if (g.kfast) {
-        tk = (int)(b % (uint32_t)g.nTk);
-        b /= (uint32_t)g.nTk;
-        tj = (int)(b % (uint32_t)g.nTj);
-        b /= (uint32_t)g.nTj;
-    } else {
-        tj = (int)(b % (uint32_t)g.nTj);
-        b /= (uint32_t)g.nTj;
-        tk = (int)(b % (uint32_t)g.nTk);
-        b /= (uint32_t)g.nTk;
-    }
-    const int ch = (int)b;
-    const int j0 = tj * TJ, k0 = tk * kMK, i0 = ch * g.CI;
+    const int j0 = tj * TJ, k0 = tk * kMK, i0 = ch * CI;
     const int k = k0 + lane;
 
     // halo role of this work-item: waves 0/1 the rows j0-1 / j0+TJ, wave 2
@@ -257,11 +260,11 @@ march3d_kernel(MarchGeom g, typename Tr::In in, typename Tr::Out out, typename T
     __syncthreads();
 
     const T c0 = Tr::c0(s);
-    for (int it = 0; it < g.CI; ++it) {
+    for (int it = 0; it < CI; ++it) {
         const int i = i0 + it;
         const int buf = it & 1;
         if (PF) {
-            if (it + 1 < g.CI) load_plane(i + 2, r2, h2r);     // in flight during plane i
+            if (it + 1 < CI) load_plane(i + 2, r2, h2r);       // in flight during plane i
             drift_plane(r1, h1r, xn, on, buf ^ 1);            // plane i+1 (loaded earlier)
         } else {
             load_plane(i + 1, r1, h1r);
@@ -305,6 +308,59 @@ march3d_kernel(MarchGeom g, typename Tr::In in, typename Tr::Out out, typename T
     }
 }
 
+template <class Tr, int RJ, bool PF, bool NT>
+__global__ void __launch_bounds__(kMW * 64)
+march3d_kernel(MarchGeom g, typename Tr::In in, typename Tr::Out out, typename Tr::S s)
+{
+    __shared__ MarchLds<Tr, RJ> sh;
+    uint32_t b = blockIdx.x;
+    if (g.remap) b = (b & 7u) * (gridDim.x >> 3) + (b >> 3);
+    int tj, tk;
+    if (g.kfast) {
+        tk = (int)(b % (uint32_t)g.nTk);
+        b /= (uint32_t)g.nTk;
+        tj = (int)(b % (uint32_t)g.nTj);
+        b /= (uint32_t)g.nTj;
+    } else {
+        tj = (int)(b % (uint32_t)g.nTj);
+        b /= (uint32_t)g.nTj;
+        tk = (int)(b % (uint32_t)g.nTk);
+        b /= (uint32_t)g.nTk;
+    }
+    march_tile<Tr, RJ, PF, NT>(g.N, g.CI, tj, tk, (int)b, in, out, s, sh);
+}
+
+// Ensemble form: one persistent launch per step for every member.  Work
+// items are the (member, tile, chunk) triples of all members, members in
+// order, tiles k fastest; each work-group takes items blockIdx.x,
+// blockIdx.x + gridDim.x, ...  All members share tau and hence the scalars.
+template <class Tr, int RJ, bool PF, bool NT>
+__global__ void __launch_bounds__(kMW * 64)
+march3d_batch_kernel(const BatchMember *__restrict__ mem, int n_members, uint32_t n_items,
+                     typename Tr::S s)
+{
+    __shared__ MarchLds<Tr, RJ> sh;
+    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        // member owning the item: last r with mem[r].item_begin <= item
+        int lo = 0, hi = n_members - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (mem[mid].item_begin <= item)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const BatchMember &m = mem[lo];
+        uint32_t t = item - m.item_begin;
+        int tk = (int)(t % (uint32_t)m.nTk);
+        t /= (uint32_t)m.nTk;
+        int tj = (int)(t % (uint32_t)m.nTj);
+        int ch = (int)(t / (uint32_t)m.nTj);
+        march_tile<Tr, RJ, PF, NT>(m.N, m.CI, tj, tk, ch, Tr::make_in(m.in), Tr::make_out(m.out),
+                                   s, sh);
+    }
+}
+
 // ---------------------------------------------------------------------------
 
 static int pick_ci(int N, int tiles_per_plane)
@@ -321,7 +377,68 @@ static int pick_ci(int N, int tiles_per_plane)
 
 bool march3d_supported(const Context *c, int N)
 {
-    return c->nd == 3 && c->p.stencil_setting == 0 && N % kMK == 0 && N >= kMK;
+    return c->nd == 3 && c->p.stencil_setting == 0 && N % kMK == 0 && N >= kMK &&
+           N % (kMW * march_rows(c)) == 0;
+}
+
+int march_rows(const Context *c) { return (c->prec == AXS_FP32) ? c->tune.rj32 : c->tune.rj64; }
+
+int march_chunk(const Context *c, int N)
+{
+    int ci = (c->prec == AXS_FP32) ? c->tune.ci32 : c->tune.ci64;
+    if (ci > 0 && N % ci == 0) return ci;
+    int TJ = kMW * march_rows(c);
+    return pick_ci(N, (N / TJ) * (N / kMK));
+}
+
+template <class Tr, int RJ, bool PF, bool NT>
+static int launch_batch(Context *c, int src, const typename Tr::S &s)
+{
+    auto kern = march3d_batch_kernel<Tr, RJ, PF, NT>;
+    int per_cu = 0;
+    AXS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kMW * 64, 0));
+    uint32_t cap = (uint32_t)c->num_cu * (uint32_t)(per_cu > 0 ? per_cu : 1);
+    uint32_t grid = c->batch_items < cap ? c->batch_items : cap;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kMW * 64), 0, c->stream, c->batch_tab[src],
+                       (int)c->members.size(), c->batch_items, s);
+    return AXS_OK;
+}
+
+int launch_march3d_batch(Context *c, int src, const StepScalars &s)
+{
+    const int f = c->tune.flags & (kTunePrefetch | kTuneNT);
+    const int rj = march_rows(c);
+    if (c->prec == AXS_FP64) {
+        if (rj == 2) {
+            switch (f) {
+            case 0: return launch_batch<ExactF64, 2, false, false>(c, src, s);
+            case kTunePrefetch: return launch_batch<ExactF64, 2, true, false>(c, src, s);
+            case kTuneNT: return launch_batch<ExactF64, 2, false, true>(c, src, s);
+            default: return launch_batch<ExactF64, 2, true, true>(c, src, s);
+            }
+        }
+        switch (f) {
+        case 0: return launch_batch<ExactF64, 4, false, false>(c, src, s);
+        case kTunePrefetch: return launch_batch<ExactF64, 4, true, false>(c, src, s);
+        case kTuneNT: return launch_batch<ExactF64, 4, false, true>(c, src, s);
+        default: return launch_batch<ExactF64, 4, true, true>(c, src, s);
+        }
+    }
+    StepScalarsF fs = to_float(s);
+    if (rj == 8) {
+        switch (f) {
+        case 0: return launch_batch<HalfF32, 8, false, false>(c, src, fs);
+        case kTunePrefetch: return launch_batch<HalfF32, 8, true, false>(c, src, fs);
+        case kTuneNT: return launch_batch<HalfF32, 8, false, true>(c, src, fs);
+        default: return launch_batch<HalfF32, 8, true, true>(c, src, fs);
+        }
+    }
+    switch (f) {
+    case 0: return launch_batch<HalfF32, 4, false, false>(c, src, fs);
+    case kTunePrefetch: return launch_batch<HalfF32, 4, true, false>(c, src, fs);
+    case kTuneNT: return launch_batch<HalfF32, 4, false, true>(c, src, fs);
+    default: return launch_batch<HalfF32, 4, true, true>(c, src, fs);
+    }
 }
 
 template <class Tr, int RJ, bool PF, bool NT>

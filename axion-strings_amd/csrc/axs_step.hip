@@ -41,11 +41,9 @@ __device__ __forceinline__ dvec2 drift64(const In64 &a, uint32_t q, double dt)
 }
 
 template <int ND>
-__global__ void __launch_bounds__(256)
-step_fp64_kernel(int N, uint32_t L, In64 in, Out64 out, StepScalars s)
+__device__ __forceinline__ void site_fp64(int N, uint32_t m, const In64 &in, const Out64 &out,
+                                          const StepScalars &s)
 {
-    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= L) return;
     Site<ND> site = decompose<ND>(m, N);
 
     dvec2 x = drift64(in, m, s.dt);
@@ -84,11 +82,16 @@ step_fp64_kernel(int N, uint32_t L, In64 in, Out64 out, StepScalars s)
 
 template <int ND>
 __global__ void __launch_bounds__(256)
-step_fp32_kernel(int N, uint32_t L, const fvec4 *__restrict__ in, fvec4 *__restrict__ out,
-                 StepScalarsF s)
+step_fp64_kernel(int N, uint32_t L, In64 in, Out64 out, StepScalars s)
 {
     uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= L) return;
+    if (m < L) site_fp64<ND>(N, m, in, out, s);
+}
+
+template <int ND>
+__device__ __forceinline__ void site_fp32(int N, uint32_t m, const fvec4 *__restrict__ in,
+                                          fvec4 *__restrict__ out, const StepScalarsF &s)
+{
     Site<ND> site = decompose<ND>(m, N);
 
     fvec4 c = in[m];
@@ -116,6 +119,93 @@ step_fp32_kernel(int N, uint32_t L, const fvec4 *__restrict__ in, fvec4 *__restr
     o.z = c.z + s.dt * k1;
     o.w = c.w + s.dt * k2;
     out[m] = o;
+}
+
+template <int ND>
+__global__ void __launch_bounds__(256)
+step_fp32_kernel(int N, uint32_t L, const fvec4 *__restrict__ in, fvec4 *__restrict__ out,
+                 StepScalarsF s)
+{
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < L) site_fp32<ND>(N, m, in, out, s);
+}
+
+// Ensemble form for lattices the march kernel does not take (2D, higher-order
+// stencils, other sides): one launch per step over the 256-site blocks of all
+// members; each block finds its member by binary search over the members'
+// first block.
+template <int ND, bool F64>
+__global__ void __launch_bounds__(256)
+batch_site_kernel(const BatchMember *__restrict__ mem, int n_members, StepScalars s,
+                  StepScalarsF sf)
+{
+    uint32_t b = blockIdx.x;
+    int lo = 0, hi = n_members - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (mem[mid].item_begin <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const BatchMember &mb = mem[lo];
+    uint32_t m = (b - mb.item_begin) * 256u + threadIdx.x;
+    if (m >= mb.L) return;
+    if (F64) {
+        In64 in{(const dvec2 *)mb.in[PHI], (const dvec2 *)mb.in[VEL], (const dvec2 *)mb.in[KER]};
+        Out64 out{(dvec2 *)mb.out[PHI], (dvec2 *)mb.out[VEL], (dvec2 *)mb.out[KER]};
+        site_fp64<ND>(mb.N, m, in, out, s);
+    } else {
+        site_fp32<ND>(mb.N, m, (const fvec4 *)mb.in[PHI], (fvec4 *)mb.out[PHI], sf);
+    }
+}
+
+// (Re)builds the ensemble tables for both ping-pong parities when the launch
+// shape changed.  Not called during graph capture (it copies to the device).
+int prepare_batch(Context *c)
+{
+    if (c->members.size() < 2) return AXS_OK;
+    bool march = c->tune.kernel != AXS_KERNEL_SIMPLE;
+    for (const Member &m : c->members) march = march && march3d_supported(c, m.N);
+    const int rows = march_rows(c);
+    int key = march ? rows * 100000 + ((c->prec == AXS_FP32) ? c->tune.ci32 : c->tune.ci64) : 0;
+    if (key == c->batch_key && c->batch_tab[0]) return AXS_OK;
+    std::vector<BatchMember> tab[2];
+    uint32_t items = 0;
+    for (const Member &m : c->members) {
+        BatchMember b{};
+        b.N = m.N;
+        b.L = (uint32_t)m.L;
+        b.item_begin = items;
+        if (march) {
+            b.nTj = m.N / (4 * rows);
+            b.nTk = m.N / 64;
+            b.CI = march_chunk(c, m.N);
+            items += (uint32_t)b.nTj * b.nTk * (m.N / b.CI);
+        } else {
+            items += (uint32_t)((m.L + 255) / 256);
+        }
+        for (int h = 0; h < 2; ++h) {
+            for (int a = 0; a < 3; ++a) {
+                b.in[a] = m.state[h][a];
+                b.out[a] = m.state[h ^ 1][a];
+            }
+            tab[h].push_back(b);
+        }
+    }
+    for (int h = 0; h < 2; ++h) {
+        if (!c->batch_tab[h]) {
+            AXS_HIP_TRY(hipMalloc(&c->batch_tab[h], c->members.size() * sizeof(BatchMember)));
+        }
+        AXS_HIP_TRY(hipMemcpyAsync(c->batch_tab[h], tab[h].data(),
+                                   tab[h].size() * sizeof(BatchMember), hipMemcpyHostToDevice,
+                                   c->stream));
+    }
+    AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+    c->batch_items = items;
+    c->batch_key = key;
+    c->batch_march = march;
+    return AXS_OK;
 }
 
 // FP32: ker = F(phi, tau) at the current state and phidot = u - dt/2 ker
@@ -161,6 +251,26 @@ static inline unsigned grid_for(uint32_t L, unsigned block) { return (L + block 
 int launch_step(Context *c, const StepScalars &s)
 {
     const int src = c->cur, dst = c->cur ^ 1;
+    if (c->members.size() > 1 && c->batch_tab[src]) {
+        // one launch for the whole ensemble (tables built by prepare_batch)
+        if (c->batch_march) {
+            AXS_TRY(launch_march3d_batch(c, src, s));
+        } else {
+            StepScalarsF f = to_float(s);
+            const int nm = (int)c->members.size();
+            dim3 grid(c->batch_items), block(256);
+            if (c->prec == AXS_FP64) {
+                auto k = (c->nd == 2) ? batch_site_kernel<2, true> : batch_site_kernel<3, true>;
+                hipLaunchKernelGGL(k, grid, block, 0, c->stream, c->batch_tab[src], nm, s, f);
+            } else {
+                auto k = (c->nd == 2) ? batch_site_kernel<2, false> : batch_site_kernel<3, false>;
+                hipLaunchKernelGGL(k, grid, block, 0, c->stream, c->batch_tab[src], nm, s, f);
+            }
+        }
+        AXS_HIP_TRY(hipGetLastError());
+        c->cur = dst;
+        return AXS_OK;
+    }
     for (Member &m : c->members) {
         uint32_t L = (uint32_t)m.L;
         unsigned grid = grid_for(L, 256);

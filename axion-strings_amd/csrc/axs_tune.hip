@@ -1,0 +1,218 @@
+// axs_tune.hip -- autotuner of the step kernel's launch shape.
+//
+// Candidates: for 3D lattices the march kernel with rows per work-item,
+// planes per chunk and the prefetch / non-temporal-store switches, plus the
+// one-site-per-work-item kernel; for every lattice, with and without a HIP
+// graph around the step loop.  A trial launches the step repeatedly on the
+// current state, writing only the spare ping-pong half and restoring the
+// state index, so tuning never advances the simulation.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "axs_internal.hpp"
+
+namespace axs {
+
+namespace {
+
+std::mutex g_tune_mu;
+std::map<std::string, std::pair<Tune, double>> g_tune_cache;
+
+std::string tune_key(const Context *c)
+{
+    hipDeviceProp_t prop;
+    std::string dev = "gfx";
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) dev = prop.gcnArchName;
+    std::string k = dev + " nd" + std::to_string(c->nd) + (c->prec == AXS_FP32 ? " fp32" : " fp64") +
+                    " st" + std::to_string(c->p.stencil_setting) + " N";
+    std::vector<int> Ns;
+    for (const Member &m : c->members) Ns.push_back(m.N);
+    std::sort(Ns.begin(), Ns.end());
+    for (size_t i = 0; i < Ns.size(); ++i) {
+        // run-length: "128x16" for sixteen members of side 128
+        size_t j = i;
+        while (j + 1 < Ns.size() && Ns[j + 1] == Ns[i]) ++j;
+        k += (i ? "," : "") + std::to_string(Ns[i]) + "x" + std::to_string(j - i + 1);
+        i = j;
+    }
+    return k;
+}
+
+void load_file_cache()
+{
+    const char *path = getenv("AXS_TUNE_CACHE");
+    if (!path || !*path) return;
+    FILE *f = fopen(path, "r");
+    if (!f) return;
+    char line[1024];
+    while (fgets(line, sizeof line, f)) {
+        // key | kernel rj32 rj64 ci32 ci64 flags | ms
+        char *bar1 = strchr(line, '|');
+        if (!bar1) continue;
+        char *bar2 = strchr(bar1 + 1, '|');
+        if (!bar2) continue;
+        std::string key(line, bar1 - line);
+        while (!key.empty() && key.back() == ' ') key.pop_back();
+        Tune t;
+        double ms = 0;
+        if (sscanf(bar1 + 1, "%d %d %d %d %d %d", &t.kernel, &t.rj32, &t.rj64, &t.ci32, &t.ci64,
+                   &t.flags) == 6 &&
+            sscanf(bar2 + 1, "%lf", &ms) == 1)
+            g_tune_cache[key] = {t, ms};
+    }
+    fclose(f);
+}
+
+void append_file_cache(const std::string &key, const Tune &t, double ms)
+{
+    const char *path = getenv("AXS_TUNE_CACHE");
+    if (!path || !*path) return;
+    FILE *f = fopen(path, "a");
+    if (!f) return;
+    fprintf(f, "%s | %d %d %d %d %d %d | %.6f\n", key.c_str(), t.kernel, t.rj32, t.rj64, t.ci32,
+            t.ci64, t.flags, ms);
+    fclose(f);
+}
+
+// time `trials` steps of the current configuration without changing state
+int time_trial(Context *c, int trials, double *ms)
+{
+    AXS_TRY(prepare_batch(c));
+    const int cur0 = c->cur;
+    StepScalars s = make_scalars(c->p, c->tau + (double)c->p.time_step);
+    AXS_TRY(launch_step(c, s));                 // warm-up (code objects, caches)
+    c->cur = cur0;
+    const bool graph = (c->tune.flags & kTuneGraph) != 0;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    if (graph) {
+        AXS_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = AXS_OK;
+        for (int t = 0; t < trials && rc == AXS_OK; ++t) {
+            rc = launch_step(c, s);
+            c->cur = cur0;
+        }
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc != AXS_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        AXS_HIP_TRY(e);
+        AXS_HIP_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    }
+    AXS_HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    if (graph) {
+        AXS_HIP_TRY(hipGraphLaunch(ge, c->stream));
+    } else {
+        for (int t = 0; t < trials; ++t) {
+            AXS_TRY(launch_step(c, s));
+            c->cur = cur0;
+        }
+    }
+    AXS_HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    AXS_HIP_TRY(hipEventSynchronize(c->ev1));
+    float f = 0.f;
+    AXS_HIP_TRY(hipEventElapsedTime(&f, c->ev0, c->ev1));
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+    *ms = (double)f / trials;
+    return AXS_OK;
+}
+
+}  // namespace
+
+int run_autotune(Context *c, int trials, double *best_ms)
+{
+    const std::string key = tune_key(c);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        static bool loaded = false;
+        if (!loaded) {
+            load_file_cache();
+            loaded = true;
+        }
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            c->tune = it->second.first;
+            if (best_ms) *best_ms = it->second.second;
+            return AXS_OK;
+        }
+    }
+    if (trials <= 0) trials = 8;
+    for (Member &m : c->members)
+        if (!m.has_state) {
+            set_error("autotune needs a state on every member");
+            return AXS_ERR_STATE;
+        }
+
+    std::vector<Tune> cands;
+    int minN = 1 << 30;
+    for (const Member &m : c->members) minN = std::min(minN, m.N);
+    const bool march_ok = c->nd == 3 && c->p.stencil_setting == 0;
+    for (int graph = 0; graph < 2; ++graph) {
+        Tune t;
+        t.kernel = AXS_KERNEL_SIMPLE;
+        t.flags = graph ? kTuneGraph : 0;
+        cands.push_back(t);
+        if (!march_ok) continue;
+        const int rows_opts32[2] = {4, 8}, rows_opts64[2] = {2, 4};
+        const int flag_opts[3] = {kTunePrefetch | kTuneNT, kTuneNT, kTunePrefetch};
+        const int chunk_opts[4] = {0, 32, 64, 128};
+        for (int ri = 0; ri < 2; ++ri)
+            for (int fi = 0; fi < 3; ++fi)
+                for (int ci = 0; ci < 4; ++ci) {
+                    int chunk = chunk_opts[ci];
+                    bool ok = true;
+                    for (const Member &m : c->members) ok = ok && (chunk == 0 || m.N % chunk == 0);
+                    if (!ok) continue;
+                    Tune u;
+                    u.kernel = AXS_KERNEL_MARCH;
+                    u.rj32 = rows_opts32[ri];
+                    u.rj64 = rows_opts64[ri];
+                    u.ci32 = u.ci64 = chunk;
+                    u.flags = flag_opts[fi] | (graph ? kTuneGraph : 0);
+                    cands.push_back(u);
+                }
+    }
+    const Tune saved = c->tune;
+    Tune best = saved;
+    double best_t = 1e300;
+    for (const Tune &t : cands) {
+        c->tune = t;
+        if (t.kernel == AXS_KERNEL_MARCH) {
+            bool ok = true;
+            for (const Member &m : c->members) ok = ok && march3d_supported(c, m.N);
+            if (!ok) continue;
+        }
+        double ms = 0.0;
+        int rc = time_trial(c, trials, &ms);
+        if (rc != AXS_OK) {
+            c->tune = saved;
+            c->batch_key = -1;
+            return rc;
+        }
+        if (ms < best_t) {
+            best_t = ms;
+            best = t;
+        }
+    }
+    c->tune = best;
+    c->batch_key = -1;          // tables follow the chosen shape
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[key] = {best, best_t};
+        append_file_cache(key, best, best_t);
+    }
+    if (best_ms) *best_ms = best_t;
+    (void)minN;
+    return AXS_OK;
+}
+
+}  // namespace axs

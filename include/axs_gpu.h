@@ -182,6 +182,7 @@ int axs_powerspec(axs_ctx *ctx, int binning, int nbins,
 #define AXS_MARCH_PREFETCH 1   /* next plane's loads in flight during compute */
 #define AXS_MARCH_NT_STORE 2   /* non-temporal stores of the new state        */
 #define AXS_MARCH_JFAST    4   /* tiles ordered j fastest (default k fastest) */
+#define AXS_STEP_GRAPH     8   /* axs_step captures its launches in a HIP graph */
 
 typedef struct axs_kernel_config {
     int kernel;   /* AXS_KERNEL_*                                           */
@@ -193,6 +194,15 @@ typedef struct axs_kernel_config {
 
 int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg);
 int axs_get_kernel_config(const axs_ctx *ctx, axs_kernel_config *cfg);
+
+/* Autotuner: times every candidate launch shape of the step kernel on this
+ * context's lattices (trial steps read the current state and write only the
+ * spare half, so the state and tau are unchanged), keeps the fastest and
+ * returns it in *best (may be NULL) with its time per step in *ms_per_step
+ * (may be NULL).  Results are cached per (ndims, sides, form, device) for
+ * the process and, if the environment variable AXS_TUNE_CACHE names a file,
+ * across processes.  trials <= 0 picks a default.                        */
+int axs_autotune(axs_ctx *ctx, int trials, axs_kernel_config *best, double *ms_per_step);
 
 /* ---- diagnostics ------------------------------------------------------- */
 

@@ -51,3 +51,77 @@ def test_uneven_3d_batch_fp32_members_match_single():
         a, b = lat.get_state(member=r), s.get_state()
         for k in a:
             np.testing.assert_array_equal(a[k], b[k], err_msg=f"member {r} {k}")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_3d_batch_on_march_kernel_matches_single(precision):
+    # sides that the persistent march kernel takes (multiples of 64), one
+    # launch per step for the ensemble; every member equals its lone run
+    Ns = [64, 128, 192, 64]
+    lat = axs.Lattice(3, members=Ns, precision=precision)
+    singles = []
+    for r, N in enumerate(Ns):
+        p1, p2 = kinv_field(3, N, seed=3000 + r)
+        z = np.zeros(N ** 3)
+        lat.set_state(p1, p2, z, z, member=r)
+        s = axs.Lattice(3, N, precision=precision)
+        s.set_kernel("simple")
+        s.set_state(p1, p2, z, z, tau=1.0)
+        singles.append(s)
+    lat.tau = 1.0
+    lat.step(6)
+    npl, _ = lat.string_counts()
+    for r, s in enumerate(singles):
+        s.step(6)
+        a, b = lat.get_state(member=r), s.get_state()
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k], err_msg=f"member {r} {k}")
+        assert npl[r] == s.strings(coords=False)[0]
+
+
+@pytest.mark.parametrize("nd,Ns", [(2, [64, 128, 32]), (3, [64, 64])])
+def test_graph_captured_steps_match_direct(nd, Ns):
+    runs = []
+    for graph in (False, True):
+        lat = axs.Lattice(nd, members=Ns)
+        flags = axs.AXS_MARCH_PREFETCH | axs.AXS_MARCH_NT_STORE | (axs.AXS_STEP_GRAPH if graph else 0)
+        lat.set_kernel("auto", flags=flags)
+        for r, N in enumerate(Ns):
+            p1, p2 = kinv_field(nd, N, seed=40 + r)
+            z = np.zeros(N ** nd)
+            lat.set_state(p1, p2, z, z, member=r)
+        lat.tau = 1.0
+        lat.step(7)
+        lat.step(5)
+        runs.append([lat.get_state(member=r) for r in range(len(Ns))])
+        assert lat.tau == pytest.approx(1.0 + 12 * float(np.float32(0.1)))
+    for a, b in zip(*runs):
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k])
+
+
+@pytest.mark.parametrize("nd,Ns,precision", [(3, [128], "fp32"), (3, [64, 128], "fp64"),
+                                             (2, [128, 256], "fp32")])
+def test_autotune_keeps_state_and_results(nd, Ns, precision):
+    lat = axs.Lattice(nd, members=Ns, precision=precision)
+    ref = axs.Lattice(nd, members=Ns, precision=precision)
+    for r, N in enumerate(Ns):
+        p1, p2 = kinv_field(nd, N, seed=50 + r)
+        z = np.zeros(N ** nd)
+        lat.set_state(p1, p2, z, z, member=r)
+        ref.set_state(p1, p2, z, z, member=r)
+    lat.tau = ref.tau = 1.0
+    before = [lat.get_state(member=r) for r in range(len(Ns))]
+    cfg, ms = lat.autotune(trials=3)
+    assert ms > 0 and cfg["kernel"] in (1, 2)
+    after = [lat.get_state(member=r) for r in range(len(Ns))]
+    for a, b in zip(before, after):
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k])
+    assert lat.tau == 1.0
+    lat.step(10)
+    ref.step(10)
+    for r in range(len(Ns)):
+        a, b = lat.get_state(member=r), ref.get_state(member=r)
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k])

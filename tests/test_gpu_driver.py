@@ -123,7 +123,8 @@ def test_driver_outputs_match_oracle(tmp_path, nd, N, steps, init):
             k_, p_, n_ = row.split(",")
             assert int(n_) == cnt[b]
             assert float(k_) == pytest.approx(ks[b], rel=1e-5)
-            assert float(p_) == pytest.approx(pk[b], rel=1e-5, abs=1e-300)
+            # empty-spectrum bins hold only transform rounding (~1e-26)
+            assert float(p_) == pytest.approx(pk[b], rel=1e-5, abs=1e-12 * pk[sel].max())
     log = open(out / "output.txt").read()
     assert "Running with" in log and "Time taken:" in log and "Writing snapshot 0:" in log
 
