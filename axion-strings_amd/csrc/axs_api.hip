@@ -711,7 +711,7 @@ int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
                    (cfg->rows == 8 && c->prec == AXS_FP32) ||
                    (cfg->rows == 2 && c->prec == AXS_FP64);
     if (cfg->kernel < AXS_KERNEL_AUTO || cfg->kernel > AXS_KERNEL_MARCH || !rows_ok ||
-        cfg->chunk < 0 || (cfg->depth != 0 && cfg->depth != 1) || (cfg->flags & ~7) != 0) {
+        cfg->chunk < 0 || (cfg->depth != 0 && cfg->depth != 1) || (cfg->flags & ~15) != 0) {
         set_error("unsupported kernel configuration (kernel %d rows %d chunk %d depth %d "
                   "flags %d)", cfg->kernel, cfg->rows, cfg->chunk, cfg->depth, cfg->flags);
         return AXS_ERR_UNSUPPORTED;

@@ -89,6 +89,51 @@ def accuracy(np, lat32, lat64):
     }
 
 
+ENSEMBLES = {
+    # 512 2D lattices: 128 each of 128^2, 256^2, 512^2, 1024^2; 2000 steps
+    "ensemble_2d": (2, [(128, 128), (256, 128), (512, 128), (1024, 128)], 2000),
+    # 48 3D lattices: 16 each of 128^3, 192^3, 256^3; 1500 steps
+    "ensemble_3d": (3, [(128, 16), (192, 16), (256, 16)], 1500),
+}
+
+
+def ensemble_leg(np, axs, name, T0, steps_override=0):
+    """One launch per step for every member (persistent work-list kernel for
+    3D, flat batched kernel for 2D), launch shape from the autotuner; the
+    FP32 form timed, then per-realization string counts against the FP64
+    form after the same steps."""
+    nd, groups, steps = ENSEMBLES[name]
+    if steps_override:
+        steps = steps_override
+    Ns = [N for N, cnt in groups for _ in range(cnt)]
+    sites = sum(N ** nd for N in Ns)
+    out = {"members": len(Ns), "sides": [N for N, _ in groups], "sites": sites, "steps": steps}
+    counts = {}
+    for prec in ("fp32", "fp64"):
+        lat = axs.Lattice(nd, members=Ns, precision=prec, T_initial=T0)
+        for r in range(len(Ns)):
+            lat.init("kinv", seed=1998 + r, member=r)
+        lat.tau = 1.0
+        cfg, tuned_ms = lat.autotune(trials=3)
+        log(f"[bench] {name} {prec}: {len(Ns)} members, tuned {cfg} ({tuned_ms:.3f} ms)")
+        t0 = time.perf_counter()
+        lat.step(steps)
+        wall = time.perf_counter() - t0
+        counts[prec] = lat.string_counts()[0]
+        out[prec] = {"site_updates_per_s": sites * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                     "kernel_ms": lat.last_step_ms, "config": cfg}
+        lat.close()
+        log(f"[bench] {name} {prec}: {1e3 * wall / steps:.3f} ms/step")
+    c32 = counts["fp32"].astype(np.float64)
+    c64 = counts["fp64"].astype(np.float64)
+    rel = np.abs(c32 - c64) / np.maximum(c64, 1.0)
+    out["string_count_rel_diff_max"] = float(rel.max())
+    out["string_count_rel_diff_mean"] = float(rel.mean())
+    out["string_count_total_fp32"] = int(c32.sum())
+    out["string_count_total_fp64"] = int(c64.sum())
+    return out
+
+
 def cpu_baseline(np, seconds: float):
     """The oracle's reference-structured FP64 step on a 256^3 sub-sample of
     the workload (same per-site work), all host threads."""
@@ -156,6 +201,10 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (for calibrating HBM "
                          "traffic counters) and exit")
+    ap.add_argument("--skip-ensembles", action="store_true",
+                    help="skip the 2D (512-member) and 3D (48-member) ensemble legs")
+    ap.add_argument("--ensemble-steps", type=int, default=0,
+                    help="override the ensembles' step counts (2000 / 1500)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     args = ap.parse_args()
@@ -207,6 +256,15 @@ def main() -> int:
         except (ValueError, OSError):
             traffic = None
 
+    ens = None
+    if not args.skip_ensembles:
+        ens = {}
+        for name in ENSEMBLES:
+            try:
+                ens[name] = ensemble_leg(np, axs, name, args.T_initial, args.ensemble_steps)
+            except Exception as e:          # reported, the headline line still prints
+                ens[name] = {"error": str(e)}
+
     cpu = None
     if not args.skip_cpu:
         log("[bench] CPU baseline (oracle)")
@@ -243,6 +301,7 @@ def main() -> int:
         "fp64": out64,
         "fp32_speedup_over_fp64": (r32["site_updates_per_s"] / out64["value"]) if out64 else None,
         "accuracy_fp32_vs_fp64": acc,
+        "ensembles": ens,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
