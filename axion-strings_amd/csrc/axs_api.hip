@@ -587,13 +587,17 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
     const int cur0 = c->cur;
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
-    if (graph) AXS_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    // the events bracket the launches themselves, or the graph launch
+    if (graph)
+        AXS_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    else
+        AXS_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     int rc = AXS_OK;
     for (int64_t n = 0; n < nsteps && rc == AXS_OK; ++n) {
         double tau_next = c->tau + dt;                    // s_integrate.cpp:317
         StepScalars s = make_scalars(c->p, tau_next);
         rc = launch_step(c, s);
-        c->tau = tau_next;
+        if (rc == AXS_OK) c->tau = tau_next;
     }
     if (graph) {
         hipError_t e = hipStreamEndCapture(c->stream, &g);
@@ -605,14 +609,18 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
         }
     }
     if (rc != AXS_OK) {
+        // a captured loop never ran: back to the state before the call; a
+        // direct loop keeps the steps that were launched
         if (ge) (void)hipGraphExecDestroy(ge);
         if (g) (void)hipGraphDestroy(g);
-        c->tau = tau0;
-        c->cur = cur0;
+        if (graph) {
+            c->tau = tau0;
+            c->cur = cur0;
+        }
         return rc;
     }
-    AXS_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     if (graph) {
+        AXS_HIP_TRY(hipEventRecord(c->ev0, c->stream));
         hipError_t e = hipGraphLaunch(ge, c->stream);
         if (e != hipSuccess) {
             (void)hipGraphExecDestroy(ge);
