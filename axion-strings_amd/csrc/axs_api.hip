@@ -377,6 +377,9 @@ static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ct
         set_error("stream synchronize failed after allocation");
         return fail(AXS_ERR_HIP);
     }
+    // FP32 3D standard stencil: two steps per pass unless configured otherwise
+    c->tune.depth = 2;
+    if (!two_step_active(c)) c->tune.depth = 1;
     (void)rc;
     *out = h;
     return AXS_OK;
@@ -593,11 +596,20 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
     else
         AXS_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     int rc = AXS_OK;
-    for (int64_t n = 0; n < nsteps && rc == AXS_OK; ++n) {
+    const bool pairs = two_step_active(c);
+    for (int64_t n = 0; n < nsteps && rc == AXS_OK;) {
         double tau_next = c->tau + dt;                    // s_integrate.cpp:317
         StepScalars s = make_scalars(c->p, tau_next);
-        rc = launch_step(c, s);
-        if (rc == AXS_OK) c->tau = tau_next;
+        if (pairs && nsteps - n >= 2) {
+            double tau_after = tau_next + dt;
+            rc = launch_two_steps(c, s, make_scalars(c->p, tau_after));
+            if (rc == AXS_OK) c->tau = tau_after;
+            n += 2;
+        } else {
+            rc = launch_step(c, s);
+            if (rc == AXS_OK) c->tau = tau_next;
+            n += 1;
+        }
     }
     if (graph) {
         hipError_t e = hipStreamEndCapture(c->stream, &g);
@@ -715,11 +727,24 @@ int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
         return AXS_ERR_INVALID_ARG;
     }
     Context *c = &ctx->impl;
-    bool rows_ok = cfg->rows == 0 || cfg->rows == 4 ||
-                   (cfg->rows == 8 && c->prec == AXS_FP32) ||
-                   (cfg->rows == 2 && c->prec == AXS_FP64);
+    const int depth = cfg->depth ? cfg->depth : 1;
+    bool rows_ok;
+    if (depth == 2) {
+        // two steps per pass: FP32 3D standard stencil on every member
+        rows_ok = cfg->rows == 0 || cfg->rows == 4 || cfg->rows == 6 || cfg->rows == 8;
+        bool fits = cfg->kernel != AXS_KERNEL_SIMPLE;
+        for (const Member &m : c->members) fits = fits && march2_supported(c, m.N);
+        if (!fits) {
+            set_error("depth 2 needs the FP32 form, 3D, stencil 0, sides >= 64 and a "
+                      "non-simple kernel");
+            return AXS_ERR_UNSUPPORTED;
+        }
+    } else {
+        rows_ok = cfg->rows == 0 || cfg->rows == 4 || (cfg->rows == 8 && c->prec == AXS_FP32) ||
+                  (cfg->rows == 2 && c->prec == AXS_FP64);
+    }
     if (cfg->kernel < AXS_KERNEL_AUTO || cfg->kernel > AXS_KERNEL_MARCH || !rows_ok ||
-        cfg->chunk < 0 || (cfg->depth != 0 && cfg->depth != 1) || (cfg->flags & ~15) != 0) {
+        cfg->chunk < 0 || depth < 1 || depth > 2 || (cfg->flags & ~15) != 0) {
         set_error("unsupported kernel configuration (kernel %d rows %d chunk %d depth %d "
                   "flags %d)", cfg->kernel, cfg->rows, cfg->chunk, cfg->depth, cfg->flags);
         return AXS_ERR_UNSUPPORTED;
@@ -730,13 +755,15 @@ int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
             return AXS_ERR_INVALID_ARG;
         }
     c->tune.kernel = cfg->kernel;
-    if (c->prec == AXS_FP32)
+    if (depth == 2)
+        c->tune.rtb = cfg->rows;
+    else if (c->prec == AXS_FP32)
         c->tune.rj32 = cfg->rows ? cfg->rows : 8;
     else
         c->tune.rj64 = cfg->rows ? cfg->rows : 4;
     c->tune.ci32 = cfg->chunk;
     c->tune.ci64 = cfg->chunk;
-    c->tune.depth = 1;
+    c->tune.depth = depth;
     c->tune.flags = cfg->flags;
     return AXS_OK;
 }
@@ -749,7 +776,8 @@ int axs_get_kernel_config(const axs_ctx *ctx, axs_kernel_config *cfg)
     }
     const Context *c = &ctx->impl;
     cfg->kernel = c->tune.kernel;
-    cfg->rows = (c->prec == AXS_FP32) ? c->tune.rj32 : c->tune.rj64;
+    cfg->rows = (c->tune.depth == 2) ? c->tune.rtb
+                                     : ((c->prec == AXS_FP32) ? c->tune.rj32 : c->tune.rj64);
     cfg->chunk = (c->prec == AXS_FP32) ? c->tune.ci32 : c->tune.ci64;
     cfg->depth = c->tune.depth;
     cfg->flags = c->tune.flags;
@@ -785,8 +813,10 @@ int axs_bytes_per_site(const axs_ctx *ctx, double *scheme_bytes, double *alg_byt
     const Context *c = &ctx->impl;
     double w = (double)c->elem;
     if (alg_bytes) *alg_bytes = 2.0 * 4.0 * w;     // read + write phi1, phi2, vel1, vel2
-    if (scheme_bytes)   // FP64 also moves the two kernel arrays
-        *scheme_bytes = (c->prec == AXS_FP64) ? 2.0 * 6.0 * w : 2.0 * 4.0 * w;
+    if (scheme_bytes) {   // FP64 also moves the two kernel arrays
+        double b = (c->prec == AXS_FP64) ? 2.0 * 6.0 * w : 2.0 * 4.0 * w;
+        *scheme_bytes = two_step_active(c) ? b / 2.0 : b;
+    }
     return AXS_OK;
 }
 

@@ -105,7 +105,8 @@ struct Tune {
     int rj64 = 4;      // rows per work-item, FP64 march
     int ci32 = 0;      // planes per chunk, 0 = automatic
     int ci64 = 0;
-    int depth = 1;
+    int depth = 1;     // time steps per pass over the lattice: 1, or 2 (FP32 3D)
+    int rtb = 0;       // rows per work-item of the two-step kernel, 0 = by side
     int flags = kTunePrefetch | kTuneNT;
 };
 
@@ -167,6 +168,13 @@ int launch_step(Context *c, const StepScalars &s);
 // 3D 2.5D streaming step (stencil_setting 0, N a multiple of 64)
 bool march3d_supported(const Context *c, int N);
 int launch_march3d(Context *c, Member &m, int src, int dst, const StepScalars &s);
+bool march2_supported(const Context *c, int N);
+int launch_march2(Context *c, Member &m, int src, int dst, const StepScalars &a,
+                  const StepScalars &b);
+// depth-2 configuration that applies to every member of the context
+bool two_step_active(const Context *c);
+// two steps (scalars of tau+dt, tau+2dt) with one pass per member
+int launch_two_steps(Context *c, const StepScalars &a, const StepScalars &b);
 // march geometry chosen for a member (rows per work-item, planes per chunk)
 int march_rows(const Context *c);
 int march_chunk(const Context *c, int N);

@@ -62,8 +62,8 @@ void load_file_cache()
         while (!key.empty() && key.back() == ' ') key.pop_back();
         Tune t;
         double ms = 0;
-        if (sscanf(bar1 + 1, "%d %d %d %d %d %d", &t.kernel, &t.rj32, &t.rj64, &t.ci32, &t.ci64,
-                   &t.flags) == 6 &&
+        if (sscanf(bar1 + 1, "%d %d %d %d %d %d %d %d", &t.kernel, &t.rj32, &t.rj64, &t.ci32,
+                   &t.ci64, &t.flags, &t.depth, &t.rtb) == 8 &&
             sscanf(bar2 + 1, "%lf", &ms) == 1)
             g_tune_cache[key] = {t, ms};
     }
@@ -76,8 +76,8 @@ void append_file_cache(const std::string &key, const Tune &t, double ms)
     if (!path || !*path) return;
     FILE *f = fopen(path, "a");
     if (!f) return;
-    fprintf(f, "%s | %d %d %d %d %d %d | %.6f\n", key.c_str(), t.kernel, t.rj32, t.rj64, t.ci32,
-            t.ci64, t.flags, ms);
+    fprintf(f, "%s | %d %d %d %d %d %d %d %d | %.6f\n", key.c_str(), t.kernel, t.rj32, t.rj64,
+            t.ci32, t.ci64, t.flags, t.depth, t.rtb, ms);
     fclose(f);
 }
 
@@ -86,8 +86,13 @@ int time_trial(Context *c, int trials, double *ms)
 {
     AXS_TRY(prepare_batch(c));
     const int cur0 = c->cur;
-    StepScalars s = make_scalars(c->p, c->tau + (double)c->p.time_step);
-    AXS_TRY(launch_step(c, s));                 // warm-up (code objects, caches)
+    const double dt = (double)c->p.time_step;
+    const StepScalars s = make_scalars(c->p, c->tau + dt);
+    const StepScalars s2 = make_scalars(c->p, c->tau + dt + dt);
+    // one trial = one launch: a step, or a pair of steps at depth 2
+    const bool pairs = two_step_active(c);
+    auto one = [&]() { return pairs ? launch_two_steps(c, s, s2) : launch_step(c, s); };
+    AXS_TRY(one());                             // warm-up (code objects, caches)
     c->cur = cur0;
     const bool graph = (c->tune.flags & kTuneGraph) != 0;
     hipGraph_t g = nullptr;
@@ -96,7 +101,7 @@ int time_trial(Context *c, int trials, double *ms)
         AXS_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int rc = AXS_OK;
         for (int t = 0; t < trials && rc == AXS_OK; ++t) {
-            rc = launch_step(c, s);
+            rc = one();
             c->cur = cur0;
         }
         hipError_t e = hipStreamEndCapture(c->stream, &g);
@@ -112,7 +117,7 @@ int time_trial(Context *c, int trials, double *ms)
         AXS_HIP_TRY(hipGraphLaunch(ge, c->stream));
     } else {
         for (int t = 0; t < trials; ++t) {
-            AXS_TRY(launch_step(c, s));
+            AXS_TRY(one());
             c->cur = cur0;
         }
     }
@@ -122,7 +127,7 @@ int time_trial(Context *c, int trials, double *ms)
     AXS_HIP_TRY(hipEventElapsedTime(&f, c->ev0, c->ev1));
     if (ge) (void)hipGraphExecDestroy(ge);
     if (g) (void)hipGraphDestroy(g);
-    *ms = (double)f / trials;
+    *ms = (double)f / trials / (pairs ? 2 : 1);
     return AXS_OK;
 }
 
@@ -180,13 +185,34 @@ int run_autotune(Context *c, int trials, double *best_ms)
                     u.flags = flag_opts[fi] | (graph ? kTuneGraph : 0);
                     cands.push_back(u);
                 }
+        if (c->prec != AXS_FP32) continue;
+        // two steps per pass
+        const int rtb_opts[3] = {4, 6, 8};
+        const int chunk_opts2[4] = {0, 16, 32, 64};
+        for (int ri = 0; ri < 3; ++ri)
+            for (int pf = 0; pf < 2; ++pf)
+                for (int ci = 0; ci < 4; ++ci) {
+                    int chunk = chunk_opts2[ci];
+                    bool ok = true;
+                    for (const Member &m : c->members) ok = ok && (chunk == 0 || m.N % chunk == 0);
+                    if (!ok) continue;
+                    Tune u;
+                    u.kernel = AXS_KERNEL_MARCH;
+                    u.depth = 2;
+                    u.rtb = rtb_opts[ri];
+                    u.ci32 = u.ci64 = chunk;
+                    u.flags = (pf ? kTunePrefetch : 0) | kTuneNT | (graph ? kTuneGraph : 0);
+                    cands.push_back(u);
+                }
     }
     const Tune saved = c->tune;
     Tune best = saved;
     double best_t = 1e300;
     for (const Tune &t : cands) {
         c->tune = t;
-        if (t.kernel == AXS_KERNEL_MARCH) {
+        if (t.depth == 2) {
+            if (!two_step_active(c)) continue;
+        } else if (t.kernel == AXS_KERNEL_MARCH) {
             bool ok = true;
             for (const Member &m : c->members) ok = ok && march3d_supported(c, m.N);
             if (!ok) continue;

@@ -11,10 +11,15 @@ steps are followed by exactly K timed steps, with the stream synchronised
 before and after.
 
 The JSON line also carries
-  roofline     : the step kernel's achieved HBM rate, from the algorithmic
-                 bytes per launch (32 B/site FP32: read + write of phi1, phi2
-                 and the two velocities) over the kernel's average duration
-                 measured with HIP events on the library's stream;
+  roofline     : the step kernel's achieved rate: algorithmic bytes per
+                 launch (32 B per site-update in FP32: read + write of phi1,
+                 phi2 and the two velocities, times the site-updates of one
+                 launch: N^3 per time step it advances) over the launch's
+                 average duration, measured with HIP events on the library's
+                 stream.  The default FP32 kernel advances two time steps per
+                 launch, reading and writing the state once, so this figure
+                 can exceed the HBM peak; `traffic` (measured HBM bytes per
+                 launch) and `traffic_rate` give the bytes actually moved;
   cpu_baseline : the CPU oracle (reference-structured FP64 step: drift,
                  two CSR sparse-matrix Laplacians, potential, kick, copy, with
                  OpenMP) timed on this host over a bounded sample;
@@ -33,17 +38,25 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "axion-strings_amd"))
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_step_fp32_3d_1024.json")
+# measured HBM bytes per launch of the FP32 step kernels at 1024^3, by time
+# steps per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)
+TRAFFIC_FILES = {1: os.path.join(ROOT, "profiles", "traffic_march_fp32_3d_1024.json"),
+                 2: os.path.join(ROOT, "profiles", "traffic_two_step_fp32_3d_1024.json")}
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def gpu_leg(np, axs, N, precision, warmup, steps, T0):
+def gpu_leg(np, axs, N, precision, warmup, steps, T0, depth=0):
     lat = axs.Lattice(3, N, precision=precision, T_initial=T0)
     lat.init("kinv", seed=1998)
     lat.tau = 1.0
+    if depth:
+        try:
+            lat.set_kernel("auto", depth=depth)
+        except axs.AxsError as e:           # not available for this form
+            log(f"[bench] {precision}: depth {depth} not applied ({e})")
     log(f"[bench] {precision}: {N}^3 initialised, {warmup} warm-up steps")
     if warmup:
         lat.step(warmup)
@@ -54,7 +67,10 @@ def gpu_leg(np, axs, N, precision, warmup, steps, T0):
     kern_ms = lat.last_step_ms       # HIP-event average over the same steps
     scheme_b, alg_b = lat.bytes_per_site()
     L = N ** 3
+    cfg = lat.kernel_config()
     res = {
+        "kernel_config": cfg,
+        "steps_per_launch": max(1, cfg["depth"]),
         "wall_s": wall,
         "ms_per_step": 1e3 * wall / steps,
         "kernel_ms": kern_ms,
@@ -205,6 +221,8 @@ def main() -> int:
                     help="skip the 2D (512-member) and 3D (48-member) ensemble legs")
     ap.add_argument("--ensemble-steps", type=int, default=0,
                     help="override the ensembles' step counts (2000 / 1500)")
+    ap.add_argument("--depth", type=int, default=0,
+                    help="time steps per step-kernel launch (0: the library's default)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--skip-cpu", action="store_true")
     args = ap.parse_args()
@@ -223,15 +241,15 @@ def main() -> int:
     N, K, W = args.N, args.steps, args.warmup
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     if forms == ["fp64"]:
-        lat64, r64 = gpu_leg(np, axs, N, "fp64", W, K, args.T_initial)
+        lat64, r64 = gpu_leg(np, axs, N, "fp64", W, K, args.T_initial, args.depth)
         lat64.close()
         print(json.dumps({"form": "fp64", **r64}), flush=True)
         return 0
-    lat32, r32 = gpu_leg(np, axs, N, "fp32", W, K, args.T_initial)
+    lat32, r32 = gpu_leg(np, axs, N, "fp32", W, K, args.T_initial, args.depth)
     out64 = None
     acc = None
     if not args.skip_fp64 and "fp64" in forms:
-        lat64, r64 = gpu_leg(np, axs, N, "fp64", W, K, args.T_initial)
+        lat64, r64 = gpu_leg(np, axs, N, "fp64", W, K, args.T_initial, args.depth)
         log("[bench] comparing FP32 with FP64")
         acc = accuracy(np, lat32, lat64)
         lat64.close()
@@ -242,16 +260,20 @@ def main() -> int:
             "roofline_frac_alg": (r64["alg_bytes_per_site"] * N ** 3 / (r64["kernel_ms"] * 1e-3))
             / (HBM_PEAK_GBS * 1e9),
             "scheme_bytes_per_site": r64["scheme_bytes_per_site"],
+            "kernel_config": r64["kernel_config"],
         }
     lat32.close()
 
     L = N ** 3
-    achieved = r32["alg_bytes_per_site"] * L / (r32["kernel_ms"] * 1e-3) / 1e9
+    spl = r32["steps_per_launch"]
+    launch_ms = r32["kernel_ms"] * spl
+    achieved = r32["alg_bytes_per_site"] * L * spl / (launch_ms * 1e-3) / 1e9
     traffic = None
-    if os.path.exists(TRAFFIC_FILE):
+    tfile = TRAFFIC_FILES.get(spl)
+    if tfile and os.path.exists(tfile):
         try:
-            t = json.load(open(TRAFFIC_FILE))
-            if int(t.get("N", -1)) == N:
+            t = json.load(open(tfile))
+            if int(t.get("N", -1)) == N and int(t.get("steps_per_launch", 1)) == spl:
                 traffic = t.get("bytes_per_launch")
         except (ValueError, OSError):
             traffic = None
@@ -296,7 +318,12 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "steps_per_launch": spl,
+            "launch_ms": launch_ms,
+            "traffic_rate": (traffic / (launch_ms * 1e-3) / 1e9) if traffic else None,
+            "traffic_frac": (traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
         },
+        "kernel_config": r32["kernel_config"],
         "kernel_ms_per_step": r32["kernel_ms"],
         "fp64": out64,
         "fp32_speedup_over_fp64": (r32["site_updates_per_s"] / out64["value"]) if out64 else None,

@@ -186,9 +186,13 @@ int axs_powerspec(axs_ctx *ctx, int binning, int nbins,
 
 typedef struct axs_kernel_config {
     int kernel;   /* AXS_KERNEL_*                                           */
-    int rows;     /* rows per work-item: FP32 4 or 8, FP64 2 or 4           */
+    int rows;     /* rows per work-item: FP32 4 or 8, FP64 2 or 4; depth 2:
+                     4, 6 or 8                                              */
     int chunk;    /* planes marched per work-group (divides N)              */
-    int depth;    /* time steps per pass (1)                                */
+    int depth;    /* time steps per pass over the lattice: 1 (0 = 1), or 2
+                     (FP32 form, 3D, stencil 0, every side >= 64: a new
+                     context's default there).  At depth 2 an odd step
+                     count ends with a one-step launch.                     */
     int flags;    /* AXS_MARCH_* bits                                       */
 } axs_kernel_config;
 
@@ -210,9 +214,10 @@ int axs_autotune(axs_ctx *ctx, int trials, axs_kernel_config *best, double *ms_p
  * axs_step call, measured with HIP events on the context's stream.        */
 int axs_last_step_ms(const axs_ctx *ctx, double *ms_per_step);
 
-/* Bytes each step kernel must move per site-update in this form (the
- * scheme bytes) and the path's algorithmic bytes (read+write of phi and its
- * velocity).                                                              */
+/* Bytes the configured step kernel must move per site-update (the scheme
+ * bytes: FP64 96, FP32 32, FP32 at depth 2 16 as one pass reads and writes
+ * the state once for two updates) and the path's algorithmic bytes per
+ * site-update (read+write of phi and its velocity: FP64 64, FP32 32).     */
 int axs_bytes_per_site(const axs_ctx *ctx, double *scheme_bytes, double *alg_bytes);
 
 const char *axs_strerror(int status);

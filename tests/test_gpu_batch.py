@@ -59,6 +59,7 @@ def test_3d_batch_on_march_kernel_matches_single(precision):
     # launch per step for the ensemble; every member equals its lone run
     Ns = [64, 128, 192, 64]
     lat = axs.Lattice(3, members=Ns, precision=precision)
+    lat.set_kernel("march")
     singles = []
     for r, N in enumerate(Ns):
         p1, p2 = kinv_field(3, N, seed=3000 + r)
@@ -125,3 +126,27 @@ def test_autotune_keeps_state_and_results(nd, Ns, precision):
         a, b = lat.get_state(member=r), ref.get_state(member=r)
         for k in a:
             np.testing.assert_array_equal(a[k], b[k])
+
+
+def test_two_step_kernel_on_ensemble_matches_single():
+    # depth 2 on an uneven FP32 ensemble: every member equals its lone
+    # one-step run
+    Ns = [64, 128, 96]
+    lat = axs.Lattice(3, members=Ns, precision="fp32")
+    lat.set_kernel("march", depth=2)
+    singles = []
+    for r, N in enumerate(Ns):
+        p1, p2 = kinv_field(3, N, seed=6000 + r)
+        z = np.zeros(N ** 3)
+        lat.set_state(p1, p2, z, z, member=r)
+        s = axs.Lattice(3, N, precision="fp32")
+        s.set_kernel("simple")
+        s.set_state(p1, p2, z, z, tau=1.0)
+        singles.append(s)
+    lat.tau = 1.0
+    lat.step(9)
+    for r, s in enumerate(singles):
+        s.step(9)
+        a, b = lat.get_state(member=r), s.get_state()
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k], err_msg=f"member {r} {k}")

@@ -161,3 +161,44 @@ def test_march_kernel_fp64_oracle_192():
     lat.set_state(*state, tau=1.0)
     lat.step(nsteps)
     _assert_same(lat, st)
+
+
+@pytest.mark.parametrize("N,rows,chunk,flags,nsteps", [
+    (64, 4, 0, 3, 8), (64, 8, 16, 2, 7), (100, 6, 0, 3, 9), (128, 6, 32, 3, 12),
+    (128, 8, 0, 2, 6), (192, 4, 64, 3, 5), (128, 6, 0, 11, 10)])
+def test_two_step_kernel_matches_one_step(N, rows, chunk, flags, nsteps):
+    # two time steps per pass (temporal blocking) evaluate the same
+    # expressions as two one-step launches: identical bits on every side
+    # (also sides the tiles do not divide), launch shape and step count (an
+    # odd count ends with a one-step launch)
+    state = random_state(3, N, seed=11, ker=True)
+    a = axs.Lattice(3, N, precision="fp32")
+    a.set_kernel("simple")
+    a.set_state(*state, tau=1.0)
+    a.step(nsteps)
+    b = axs.Lattice(3, N, precision="fp32")
+    b.set_kernel("march", rows=rows, chunk=chunk, depth=2, flags=flags)
+    assert b.kernel_config()["depth"] == 2 and b.kernel_config()["rows"] == rows
+    b.set_state(*state, tau=1.0)
+    b.step(nsteps)
+    ga, gb = a.get_state(), b.get_state()
+    for k in ga:
+        np.testing.assert_array_equal(ga[k], gb[k], err_msg=k)
+    assert a.tau == b.tau
+    na, _ = a.strings(coords=False)
+    nb, _ = b.strings(coords=False)
+    assert na == nb
+
+
+@pytest.mark.parametrize("nd,N,precision,kernel", [
+    (3, 64, "fp64", "march"), (2, 128, "fp32", "auto"), (3, 48, "fp32", "march"),
+    (3, 64, "fp32", "simple")])
+def test_two_step_kernel_rejects_other_forms(nd, N, precision, kernel):
+    # a rejected configuration leaves the context's own one in place (the
+    # FP32 3D 64^3 context starts at depth 2)
+    lat = axs.Lattice(nd, N, precision=precision)
+    before = lat.kernel_config()
+    assert before["depth"] == (2 if (nd, N, precision) == (3, 64, "fp32") else 1)
+    with pytest.raises(axs.AxsError, match="depth 2"):
+        lat.set_kernel(kernel, depth=2)
+    assert lat.kernel_config() == before
