@@ -66,6 +66,8 @@ AXS_MARCH_PREFETCH = 1
 AXS_MARCH_NT_STORE = 2
 AXS_MARCH_JFAST = 4
 AXS_STEP_GRAPH = 8
+AXS_MARCH_WG512 = 16
+AXS_MARCH_WG1024 = 32
 
 
 class AxsParams(ctypes.Structure):

@@ -236,6 +236,8 @@ Context::~Context()
     if (listbuf) (void)hipFree(listbuf);
     for (int h = 0; h < 2; ++h)
         if (batch_tab[h]) (void)hipFree(batch_tab[h]);
+    for (int h = 0; h < 2; ++h)
+        if (mbatch_tab[h]) (void)hipFree(mbatch_tab[h]);
     if (pinned) (void)hipHostFree(pinned);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -377,9 +379,16 @@ static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ct
         set_error("stream synchronize failed after allocation");
         return fail(AXS_ERR_HIP);
     }
-    // FP32 3D standard stencil: two steps per pass unless configured otherwise
-    c->tune.depth = 2;
-    if (!two_step_active(c)) c->tune.depth = 1;
+    // FP32 3D standard stencil: several steps per pass with 16-wave
+    // work-groups unless configured otherwise; four steps from side 256 on,
+    // three below (the fastest shapes measured at 128^3, 512^3, 1024^3)
+    int min_side = 1 << 30;
+    for (const Member &m : c->members) min_side = std::min(min_side, m.N);
+    c->tune.depth = (min_side >= 256) ? 4 : 3;
+    if (multi_step_depth(c) == 1)
+        c->tune.depth = 1;
+    else
+        c->tune.flags |= kTuneWG1024;
     (void)rc;
     *out = h;
     return AXS_OK;
@@ -596,15 +605,20 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
     else
         AXS_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     int rc = AXS_OK;
-    const bool pairs = two_step_active(c);
+    const int depth = multi_step_depth(c);
     for (int64_t n = 0; n < nsteps && rc == AXS_OK;) {
         double tau_next = c->tau + dt;                    // s_integrate.cpp:317
         StepScalars s = make_scalars(c->p, tau_next);
-        if (pairs && nsteps - n >= 2) {
-            double tau_after = tau_next + dt;
-            rc = launch_two_steps(c, s, make_scalars(c->p, tau_after));
-            if (rc == AXS_OK) c->tau = tau_after;
-            n += 2;
+        if (depth > 1 && nsteps - n >= depth) {
+            StepScalars ss[4];
+            double t = c->tau;
+            for (int l = 0; l < depth; ++l) {
+                t = t + dt;                               // the same sequence of sums
+                ss[l] = make_scalars(c->p, t);
+            }
+            rc = launch_multi_steps(c, ss, depth);
+            if (rc == AXS_OK) c->tau = t;
+            n += depth;
         } else {
             rc = launch_step(c, s);
             if (rc == AXS_OK) c->tau = tau_next;
@@ -729,14 +743,17 @@ int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
     Context *c = &ctx->impl;
     const int depth = cfg->depth ? cfg->depth : 1;
     bool rows_ok;
-    if (depth == 2) {
-        // two steps per pass: FP32 3D standard stencil on every member
-        rows_ok = cfg->rows == 0 || cfg->rows == 4 || cfg->rows == 6 || cfg->rows == 8;
-        bool fits = cfg->kernel != AXS_KERNEL_SIMPLE;
+    if (depth >= 2) {
+        // several steps per pass: FP32 3D standard stencil on every member
+        const int W = multi_waves(cfg->flags);
+        rows_ok = !((cfg->flags & AXS_MARCH_WG512) && (cfg->flags & AXS_MARCH_WG1024)) &&
+                  depth <= 4 && (cfg->rows == 0 || multi_shape_built(depth, W, cfg->rows)) &&
+                  (W != 4 || depth <= 3);
+        bool fits = cfg->kernel != AXS_KERNEL_SIMPLE && depth <= 4;
         for (const Member &m : c->members) fits = fits && march2_supported(c, m.N);
         if (!fits) {
-            set_error("depth 2 needs the FP32 form, 3D, stencil 0, sides >= 64 and a "
-                      "non-simple kernel");
+            set_error("depth %d needs the FP32 form, 3D, stencil 0, sides >= 64, a "
+                      "non-simple kernel and depth <= 4", depth);
             return AXS_ERR_UNSUPPORTED;
         }
     } else {
@@ -744,7 +761,7 @@ int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
                   (cfg->rows == 2 && c->prec == AXS_FP64);
     }
     if (cfg->kernel < AXS_KERNEL_AUTO || cfg->kernel > AXS_KERNEL_MARCH || !rows_ok ||
-        cfg->chunk < 0 || depth < 1 || depth > 2 || (cfg->flags & ~15) != 0) {
+        cfg->chunk < 0 || depth < 1 || depth > 4 || (cfg->flags & ~63) != 0) {
         set_error("unsupported kernel configuration (kernel %d rows %d chunk %d depth %d "
                   "flags %d)", cfg->kernel, cfg->rows, cfg->chunk, cfg->depth, cfg->flags);
         return AXS_ERR_UNSUPPORTED;
@@ -755,7 +772,7 @@ int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
             return AXS_ERR_INVALID_ARG;
         }
     c->tune.kernel = cfg->kernel;
-    if (depth == 2)
+    if (depth >= 2)
         c->tune.rtb = cfg->rows;
     else if (c->prec == AXS_FP32)
         c->tune.rj32 = cfg->rows ? cfg->rows : 8;
@@ -776,7 +793,7 @@ int axs_get_kernel_config(const axs_ctx *ctx, axs_kernel_config *cfg)
     }
     const Context *c = &ctx->impl;
     cfg->kernel = c->tune.kernel;
-    cfg->rows = (c->tune.depth == 2) ? c->tune.rtb
+    cfg->rows = (c->tune.depth >= 2) ? c->tune.rtb
                                      : ((c->prec == AXS_FP32) ? c->tune.rj32 : c->tune.rj64);
     cfg->chunk = (c->prec == AXS_FP32) ? c->tune.ci32 : c->tune.ci64;
     cfg->depth = c->tune.depth;
@@ -815,7 +832,7 @@ int axs_bytes_per_site(const axs_ctx *ctx, double *scheme_bytes, double *alg_byt
     if (alg_bytes) *alg_bytes = 2.0 * 4.0 * w;     // read + write phi1, phi2, vel1, vel2
     if (scheme_bytes) {   // FP64 also moves the two kernel arrays
         double b = (c->prec == AXS_FP64) ? 2.0 * 6.0 * w : 2.0 * 4.0 * w;
-        *scheme_bytes = two_step_active(c) ? b / 2.0 : b;
+        *scheme_bytes = b / (double)multi_step_depth(c);
     }
     return AXS_OK;
 }

@@ -105,12 +105,14 @@ struct Tune {
     int rj64 = 4;      // rows per work-item, FP64 march
     int ci32 = 0;      // planes per chunk, 0 = automatic
     int ci64 = 0;
-    int depth = 1;     // time steps per pass over the lattice: 1, or 2 (FP32 3D)
-    int rtb = 0;       // rows per work-item of the two-step kernel, 0 = by side
+    int depth = 1;     // time steps per pass over the lattice: 1, or 2-4 (FP32 3D)
+    int rtb = 0;       // rows per work-item of the multi-step kernel, 0 = by side
     int flags = kTunePrefetch | kTuneNT;
 };
 
 constexpr int kTuneGraph = 8;      // capture the step loop in a HIP graph
+constexpr int kTuneWG512 = 16;     // multi-step kernel: 8 waves per work-group
+constexpr int kTuneWG1024 = 32;    // multi-step kernel: 16 waves per work-group
 
 struct Context {
     axs_params p{};
@@ -148,6 +150,10 @@ struct Context {
     uint32_t batch_items = 0;
     int batch_key = -1;          // rows * 100000 + chunk the tables were built for
     bool batch_march = false;    // 3D ensemble on the march kernel
+    // the same for the multi-step kernel (tiles of its own shape)
+    BatchMember *mbatch_tab[2] = {nullptr, nullptr};
+    uint32_t mbatch_items = 0;
+    int mbatch_key = -1;
 
     ~Context();
 };
@@ -169,12 +175,16 @@ int launch_step(Context *c, const StepScalars &s);
 bool march3d_supported(const Context *c, int N);
 int launch_march3d(Context *c, Member &m, int src, int dst, const StepScalars &s);
 bool march2_supported(const Context *c, int N);
-int launch_march2(Context *c, Member &m, int src, int dst, const StepScalars &a,
-                  const StepScalars &b);
-// depth-2 configuration that applies to every member of the context
-bool two_step_active(const Context *c);
-// two steps (scalars of tau+dt, tau+2dt) with one pass per member
-int launch_two_steps(Context *c, const StepScalars &a, const StepScalars &b);
+// time steps per launch the configuration gives every member (1 to 4)
+int multi_step_depth(const Context *c);
+// the multi-step kernel is built for depth D, W waves, R rows per wave
+bool multi_shape_built(int D, int W, int R);
+int multi_waves(int flags);
+// `depth` steps (scalars s[0..depth-1] of tau+dt, tau+2dt, ...) in one pass
+// per member
+int launch_multi_steps(Context *c, const StepScalars *s, int depth);
+// work-item tables of the multi-step kernel for an ensemble
+int prepare_multi_batch(Context *c);
 // march geometry chosen for a member (rows per work-item, planes per chunk)
 int march_rows(const Context *c);
 int march_chunk(const Context *c, int N);

@@ -1,83 +1,78 @@
-// axs_march2.hip -- FP32 3D step with temporal blocking: two time steps per
-// pass over HBM (standard 7-point stencil).
+// axs_march2.hip -- FP32 3D step with temporal blocking: D = 2, 3 or 4 time
+// steps per pass over HBM (standard 7-point stencil).
 //
-// A work-group owns an output tile of (4R-4) rows (j) x 60 columns (k) and
-// marches along i over a chunk of CI planes.  Its four waves cover 4R rows
-// of 64 consecutive k: the output tile plus a two-site halo on every side,
-// read from HBM once per chunk.  For each i the pipeline is
+// A work-group of W waves covers ER = W*R rows (j) of 64 consecutive k and
+// marches along i over a chunk of CI planes.  Level 0 is the state loaded
+// from HBM; level l (1..D) is the lattice after l steps.  Level l is
+// computed on the loaded region minus an l-site rim, so the work-group
+// writes an (ER-2D) x (64-2D) output tile and reads a D-site halo on every
+// side (re-read by the neighbouring tiles).  Per plane i the pipeline is
 //
-//   load the state (phi, u) of plane i+2 on the 4R x 64 region,
-//      drift x1 = phi + dt u                               (registers, LDS X1)
-//   first step at plane i+1 on the region minus a one-site rim:
-//      ker1 = F(x1, tau1), u1 = u + dt ker1, x2 = x1 + dt u1   (registers, LDS X2)
-//   second step at plane i on the output tile:
-//      ker2 = F(x2, tau2), u2 = u1 + dt ker2, write (x2, u2)
+//   load (phi, u) of plane i+D, drift x1 = phi + dt u        (level 0 -> x1)
+//   for l = 1..D: at plane q = i+D-l
+//      ker = F(x_l, tau_l);  u_l = u_{l-1} + dt ker;  x_{l+1} = x_l + dt u_l
+//   write (x_D, u_D) of plane i
 //
-// with the drifted planes i-1, i, i+1 of each level in registers (the i
-// neighbours) and the middle plane in LDS (the j and k neighbours).  Every
-// intermediate value is computed by the same expressions as in the one-step
-// kernels (Laplacian order centre, +-i, +-j, +-k; no contraction), so one
-// pass here is bit-identical to two one-step launches.  Tiles at the
-// lattice edge overhang it: loads wrap periodically, stores are masked.
-//
-// LDS: X1 double-buffered, X2 triple-buffered, so one barrier per plane
-// separates every write from the reads of the same buffer.
+// Each level keeps x_l of planes q-1 and q and u_{l-1} of plane q in
+// registers (the i neighbours come from registers; plane q+1 is the value
+// the level below has just produced) and x_l of plane q in LDS (the j and k
+// neighbours): one LDS plane per level, written after a barrier that ends
+// the plane's reads and read after a second barrier.  Every value is
+// computed by the same expressions as the one-step kernels (Laplacian order
+// centre, +-i, +-j, +-k; no contraction), so a pass is bit-identical to D
+// one-step launches.  Tiles at the lattice edge overhang it: loads wrap
+// periodically and stores are masked.
 #include "axs_internal.hpp"
 #include "axs_lattice.hpp"
 
 namespace axs {
 
-constexpr int kTBW = 64;            // columns loaded per row (one wave)
-constexpr int kTBOut = kTBW - 4;    // columns written per row
+constexpr int kTBW = 64;   // columns loaded per row (one wave)
 
 struct TBGeom {
     int N, CI, nTj, nTk, remap;
 };
 
-template <int R>
-struct TBLds {
-    static constexpr int P = 4 * R * kTBW;   // one component of one plane
-    float x1[2][2][P];
-    float x2[3][2][P];
+template <int D>
+struct TBScalars {
+    StepScalarsF s[D];     // steps to tau+dt, tau+2dt, ...
 };
 
-template <int R, bool PF>
-__global__ void __launch_bounds__(256)
-march2_f32_kernel(TBGeom g, const fvec4 *__restrict__ in, fvec4 *__restrict__ out,
-                  StepScalarsF s1, StepScalarsF s2)
+template <int D, int W, int R>
+using TBLds = float[D][2][W * R * kTBW];   // x_l of the plane level l works on
+
+// One work-group: tile (tj, tk) of the lattice of side N, planes
+// [ch*CI, ch*CI + CI).
+template <int D, int W, int R, bool PF>
+__device__ __forceinline__ void marchd_tile(const int N, const int CI, const int tj, const int tk,
+                                            const int ch, const fvec4 *__restrict__ in,
+                                            fvec4 *__restrict__ out, const TBScalars<D> &sc,
+                                            TBLds<D, W, R> &X)
 {
-    constexpr int ER = 4 * R;           // rows loaded
-    constexpr int TJ = ER - 4;          // rows written
-    __shared__ TBLds<R> sh;
+    constexpr int ER = W * R;            // rows loaded
+    constexpr int TJ = ER - 2 * D;       // rows written
+    constexpr int TK = kTBW - 2 * D;     // columns written
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int N = g.N;
     const uint32_t NN = (uint32_t)N * (uint32_t)N;
+    const int j0 = tj * TJ, k0 = tk * TK, i0 = ch * CI;
 
-    uint32_t b = blockIdx.x;
-    if (g.remap) b = (b & 7u) * (gridDim.x >> 3) + (b >> 3);
-    const int tk = (int)(b % (uint32_t)g.nTk);
-    b /= (uint32_t)g.nTk;
-    const int tj = (int)(b % (uint32_t)g.nTj);
-    const int ch = (int)(b / (uint32_t)g.nTj);
-    const int j0 = tj * TJ, k0 = tk * kTBOut, i0 = ch * g.CI;
-
-    // column of this lane, wrapped; -2 <= k0 - 2 + lane < 2N since N >= 64
-    int kk = k0 - 2 + lane;
+    // column of this lane, wrapped: -D <= k0 - D + lane < 2N for N >= 64
+    int kk = k0 - D + lane;
     kk = (kk < 0) ? kk + N : ((kk >= N) ? kk - N : kk);
-    const bool col1 = lane >= 1 && lane < kTBW - 1;
-    const bool col2 = lane >= 2 && lane < kTBW - 2 && (k0 + lane - 2) < N;
+    const bool colout = lane >= D && lane < kTBW - D && (k0 + lane - D) < N;
 
     uint32_t rowoff[R];
-    bool row1[R], row2[R];
+    int erow[R];
+    bool rowout[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int e = wave * R + r;                 // row within the loaded region
-        int j = j0 - 2 + e;                         // -2 <= j < 2N since TJ < N
+        int j = j0 - D + e;                         // -D <= j < 2N (ER <= 64 <= N)
         j = (j < 0) ? j + N : ((j >= N) ? j - N : j);
         rowoff[r] = (uint32_t)j * (uint32_t)N + (uint32_t)kk;
-        row1[r] = e >= 1 && e < ER - 1;
-        row2[r] = e >= 2 && e < ER - 2 && (j0 + e - 2) < N;
+        erow[r] = e;
+        rowout[r] = e >= D && e < ER - D && (j0 + e - D) < N;
     }
     auto plane = [&](int i) -> uint32_t {
         int w = (i < 0) ? i + N : ((i >= N) ? i - N : i);
@@ -88,148 +83,175 @@ march2_f32_kernel(TBGeom g, const fvec4 *__restrict__ in, fvec4 *__restrict__ ou
 #pragma unroll
         for (int r = 0; r < R; ++r) raw[r] = in[po + rowoff[r]];
     };
+    // site (row e, this lane) lies in level l's region
+    auto inside = [&](int e, int l) -> bool {
+        return e >= l && e < ER - l && lane >= l && lane < kTBW - l;
+    };
 
-    // x1 of planes (i, i+1, i+2) = (x1a, x1b, x1c); u of planes i+1, i+2;
-    // x2 of planes (i-1, i, i+1) = (x2a, x2b, x2c); u1 of planes i, i+1
-    float x1a[R][2], x1b[R][2], x1c[R][2];
-    float u0b[R][2], u0c[R][2];
-    float x2a[R][2], x2b[R][2], x2c[R][2];
-    float u1b[R][2], u1c[R][2];
+    // per level l = 1..D (index l-1): x_l at planes q-1 (xm) and q (x0),
+    // u_{l-1} at plane q (uq)
+    float xm[D][R][2], x0[D][R][2], uq[D][R][2];
+#pragma unroll
+    for (int l = 0; l < D; ++l)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int q = 0; q < 2; ++q) xm[l][r][q] = x0[l][r][q] = uq[l][r][q] = 0.f;
     fvec4 raw[R], pf[R];
 
-    const float c0a = s1.c[0], c0b = s2.c[0];
-
-    // prologue: planes i0-2 (registers) and i0-1 (registers, X1)
-    load(i0 - 2, raw);
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        x1a[r][0] = raw[r].x + s1.dt * raw[r].z;
-        x1a[r][1] = raw[r].y + s1.dt * raw[r].w;
-    }
-    load(i0 - 1, raw);
-    {
-        float *W1 = sh.x1[(i0 - 1) & 1][0], *W2 = sh.x1[(i0 - 1) & 1][1];
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            x1b[r][0] = raw[r].x + s1.dt * raw[r].z;
-            x1b[r][1] = raw[r].y + s1.dt * raw[r].w;
-            u0b[r][0] = raw[r].z;
-            u0b[r][1] = raw[r].w;
-            const int o = (wave * R + r) * kTBW + lane;
-            W1[o] = x1b[r][0];
-            W2[o] = x1b[r][1];
-        }
-    }
-    if (PF) load(i0, pf);
-    __syncthreads();
-
-    int w2 = 0;   // X2 slot of plane i+1; plane i sits in slot w2 - 1 (mod 3)
-    for (int it = -2; it < g.CI; ++it) {
+    if (PF) load(i0 - D, pf);
+    for (int it = -2 * D; it < CI; ++it) {
         const int i = i0 + it;
-        // plane i+2: x1 and u
+        // level 0: plane i+D
         if (PF) {
 #pragma unroll
             for (int r = 0; r < R; ++r) raw[r] = pf[r];
-            if (it + 1 < g.CI) load(i + 3, pf);
+            if (it + 1 < CI) load(i + D + 1, pf);
         } else {
-            load(i + 2, raw);
+            load(i + D, raw);
         }
-        {
-            float *W1 = sh.x1[(i + 2) & 1][0], *W2 = sh.x1[(i + 2) & 1][1];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                x1c[r][0] = raw[r].x + s1.dt * raw[r].z;
-                x1c[r][1] = raw[r].y + s1.dt * raw[r].w;
-                u0c[r][0] = raw[r].z;
-                u0c[r][1] = raw[r].w;
-                const int o = (wave * R + r) * kTBW + lane;
-                W1[o] = x1c[r][0];
-                W2[o] = x1c[r][1];
-            }
-        }
-        // first step at plane i+1 (its x1 has been in X1 since the last barrier)
-        {
-            const float *L1 = sh.x1[(i + 1) & 1][0], *L2 = sh.x1[(i + 1) & 1][1];
-            float *W1 = sh.x2[w2][0], *W2 = sh.x2[w2][1];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                if (!(row1[r] && col1)) continue;
-                const int o = (wave * R + r) * kTBW + lane;
-                float a = c0a * x1b[r][0], c = c0a * x1b[r][1];
-                a = a + x1c[r][0];
-                c = c + x1c[r][1];        // i+1 neighbour
-                a = a + x1a[r][0];
-                c = c + x1a[r][1];        // i-1 neighbour
-                a = a + L1[o + kTBW];
-                c = c + L2[o + kTBW];     // j+1
-                a = a + L1[o - kTBW];
-                c = c + L2[o - kTBW];     // j-1
-                a = a + L1[o + 1];
-                c = c + L2[o + 1];        // k+1
-                a = a + L1[o - 1];
-                c = c + L2[o - 1];        // k-1
-                const float x = x1b[r][0], y = x1b[r][1];
-                const float S = x * x + y * y - s1.tau2 + s1.T03;
-                const float k1 = s1.alpha * a + s1.cl * x * S;
-                const float k2 = s1.alpha * c + s1.cl * y * S;
-                const float ua = u0b[r][0] + s1.dt * k1;
-                const float ub = u0b[r][1] + s1.dt * k2;
-                u1c[r][0] = ua;
-                u1c[r][1] = ub;
-                x2c[r][0] = x + s2.dt * ua;
-                x2c[r][1] = y + s2.dt * ub;
-                W1[o] = x2c[r][0];
-                W2[o] = x2c[r][1];
-            }
-        }
-        __syncthreads();
-        // second step at plane i (its x2 was written one plane ago)
-        if (it >= 0) {
-            const int rs = (w2 == 0) ? 2 : w2 - 1;
-            const float *L1 = sh.x2[rs][0], *L2 = sh.x2[rs][1];
-            const uint32_t po = plane(i);
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                if (!(row2[r] && col2)) continue;
-                const int o = (wave * R + r) * kTBW + lane;
-                float a = c0b * x2b[r][0], c = c0b * x2b[r][1];
-                a = a + x2c[r][0];
-                c = c + x2c[r][1];
-                a = a + x2a[r][0];
-                c = c + x2a[r][1];
-                a = a + L1[o + kTBW];
-                c = c + L2[o + kTBW];
-                a = a + L1[o - kTBW];
-                c = c + L2[o - kTBW];
-                a = a + L1[o + 1];
-                c = c + L2[o + 1];
-                a = a + L1[o - 1];
-                c = c + L2[o - 1];
-                const float x = x2b[r][0], y = x2b[r][1];
-                const float S = x * x + y * y - s2.tau2 + s2.T03;
-                const float k1 = s2.alpha * a + s2.cl * x * S;
-                const float k2 = s2.alpha * c + s2.cl * y * S;
-                fvec4 v;
-                v.x = x;
-                v.y = y;
-                v.z = u1b[r][0] + s2.dt * k1;
-                v.w = u1b[r][1] + s2.dt * k2;
-                store_nt(out + po + rowoff[r], v);
-            }
-        }
-        w2 = (w2 == 2) ? 0 : w2 + 1;
+        // values handed up the levels: x_l and u_{l-1} at plane q+1
+        float fx[R][2], fu[R][2];
+        float wx[D][R][2];                 // x_l of plane q+1, for the LDS writes
 #pragma unroll
         for (int r = 0; r < R; ++r) {
+            const StepScalarsF &s = sc.s[0];
+            fx[r][0] = raw[r].x + s.dt * raw[r].z;
+            fx[r][1] = raw[r].y + s.dt * raw[r].w;
+            fu[r][0] = raw[r].z;
+            fu[r][1] = raw[r].w;
+            wx[0][r][0] = fx[r][0];
+            wx[0][r][1] = fx[r][1];
+        }
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                x1a[r][q] = x1b[r][q];
-                x1b[r][q] = x1c[r][q];
-                u0b[r][q] = u0c[r][q];
-                x2a[r][q] = x2b[r][q];
-                x2b[r][q] = x2c[r][q];
-                u1b[r][q] = u1c[r][q];
+        for (int l = 1; l <= D; ++l) {
+            const StepScalarsF &s = sc.s[l - 1];
+            const float *L1 = X[l - 1][0], *L2 = X[l - 1][1];
+            const bool active = it >= 2 * l - 2 * D;   // x_l of planes q-1, q, q+1 exist
+            float nx[R][2], nu[R][2];
+#pragma unroll
+            for (int r = 0; r < R; ++r) nx[r][0] = nx[r][1] = nu[r][0] = nu[r][1] = 0.f;
+            if (active) {
+                const uint32_t po = plane(i + D - l);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (!inside(erow[r], l)) continue;
+                    const int o = erow[r] * kTBW + lane;
+                    const float x = x0[l - 1][r][0], y = x0[l - 1][r][1];
+                    float a = s.c[0] * x, c = s.c[0] * y;
+                    a = a + fx[r][0];
+                    c = c + fx[r][1];        // i+1
+                    a = a + xm[l - 1][r][0];
+                    c = c + xm[l - 1][r][1]; // i-1
+                    a = a + L1[o + kTBW];
+                    c = c + L2[o + kTBW];    // j+1
+                    a = a + L1[o - kTBW];
+                    c = c + L2[o - kTBW];    // j-1
+                    a = a + L1[o + 1];
+                    c = c + L2[o + 1];       // k+1
+                    a = a + L1[o - 1];
+                    c = c + L2[o - 1];       // k-1
+                    const float S = x * x + y * y - s.tau2 + s.T03;
+                    const float k1 = s.alpha * a + s.cl * x * S;
+                    const float k2 = s.alpha * c + s.cl * y * S;
+                    const float ua = uq[l - 1][r][0] + s.dt * k1;
+                    const float ub = uq[l - 1][r][1] + s.dt * k2;
+                    if (l == D) {
+                        if (it >= 0 && rowout[r] && colout) {
+                            fvec4 v;
+                            v.x = x;
+                            v.y = y;
+                            v.z = ua;
+                            v.w = ub;
+                            store_nt(out + po + rowoff[r], v);
+                        }
+                    } else {
+                        const StepScalarsF &sn = sc.s[l < D ? l : D - 1];
+                        nx[r][0] = x + sn.dt * ua;
+                        nx[r][1] = y + sn.dt * ub;
+                        nu[r][0] = ua;
+                        nu[r][1] = ub;
+                    }
+                }
+            }
+            // rotate level l: q-1 <- q <- q+1 (the value handed up)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    xm[l - 1][r][q] = x0[l - 1][r][q];
+                    x0[l - 1][r][q] = fx[r][q];
+                    uq[l - 1][r][q] = fu[r][q];
+                }
+            }
+            if (l < D) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        fx[r][q] = nx[r][q];
+                        fu[r][q] = nu[r][q];
+                        wx[l][r][q] = nx[r][q];
+                    }
+                }
             }
         }
+        __syncthreads();   // every read of this plane's LDS values is done
+#pragma unroll
+        for (int l = 0; l < D; ++l) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int o = erow[r] * kTBW + lane;
+                X[l][0][o] = wx[l][r][0];
+                X[l][1][o] = wx[l][r][1];
+            }
+        }
+        __syncthreads();   // the next plane's values are visible
+    }
+}
+
+template <int D, int W, int R, bool PF>
+__global__ void __launch_bounds__(64 * W)
+marchd_f32_kernel(TBGeom g, const fvec4 *__restrict__ in, fvec4 *__restrict__ out,
+                  TBScalars<D> sc)
+{
+    __shared__ TBLds<D, W, R> X;
+    uint32_t b = blockIdx.x;
+    if (g.remap) b = (b & 7u) * (gridDim.x >> 3) + (b >> 3);   // XCD-aware order
+    const int tk = (int)(b % (uint32_t)g.nTk);
+    b /= (uint32_t)g.nTk;
+    const int tj = (int)(b % (uint32_t)g.nTj);
+    const int ch = (int)(b / (uint32_t)g.nTj);
+    marchd_tile<D, W, R, PF>(g.N, g.CI, tj, tk, ch, in, out, sc, X);
+}
+
+// Ensemble form: one persistent launch for every member.  Work items are
+// the (member, tile, chunk) triples, members in order, tiles k fastest;
+// work-group b takes items b, b + gridDim.x, ...  The first barrier of a
+// tile orders its LDS writes after the previous tile's reads.
+template <int D, int W, int R, bool PF>
+__global__ void __launch_bounds__(64 * W)
+marchd_batch_kernel(const BatchMember *__restrict__ mem, int n_members, uint32_t n_items,
+                    TBScalars<D> sc)
+{
+    __shared__ TBLds<D, W, R> X;
+    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        int lo = 0, hi = n_members - 1;        // last member starting at or before item
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (mem[mid].item_begin <= item)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const BatchMember &m = mem[lo];
+        uint32_t t = item - m.item_begin;
+        const int tk = (int)(t % (uint32_t)m.nTk);
+        t /= (uint32_t)m.nTk;
+        const int tj = (int)(t % (uint32_t)m.nTj);
+        const int ch = (int)(t / (uint32_t)m.nTj);
+        marchd_tile<D, W, R, PF>(m.N, m.CI, tj, tk, ch, (const fvec4 *)m.in[PHI],
+                                 (fvec4 *)m.out[PHI], sc, X);
     }
 }
 
@@ -238,65 +260,196 @@ bool march2_supported(const Context *c, int N)
     return c->nd == 3 && c->prec == AXS_FP32 && c->p.stencil_setting == 0 && N >= 64;
 }
 
-template <int R>
-static int launch_tb(Context *c, int N, const fvec4 *in, fvec4 *out, const StepScalarsF &a,
-                     const StepScalarsF &b, int ci, bool pf)
+bool multi_shape_built(int D, int W, int R)
 {
-    constexpr int TJ = 4 * R - 4;
-    TBGeom g;
-    g.N = N;
-    g.nTj = (N + TJ - 1) / TJ;
-    g.nTk = (N + kTBOut - 1) / kTBOut;
-    if (ci <= 0 || N % ci != 0) {
-        // short chunks keep neighbouring tiles in step (measured at 256^3
-        // and 1024^3: 16 and 32 planes beat 64 and 128)
-        ci = (N >= 512) ? 32 : 16;
-        while (N % ci != 0) --ci;
+    switch (W) {
+    case 4: return (D == 2 && (R == 4 || R == 6 || R == 8)) || (D == 3 && (R == 4 || R == 6));
+    case 8: return D >= 2 && D <= 4 && R >= 2 && R <= 4;
+    case 16: return D >= 2 && D <= 4 && (R == 2 || R == 3);
+    default: return false;
     }
-    g.CI = ci;
-    const uint32_t nb = (uint32_t)g.nTj * (uint32_t)g.nTk * (uint32_t)(N / ci);
-    g.remap = (nb % 8 == 0) ? 1 : 0;
-    if (pf)
-        hipLaunchKernelGGL((march2_f32_kernel<R, true>), dim3(nb), dim3(256), 0, c->stream, g, in,
-                           out, a, b);
-    else
-        hipLaunchKernelGGL((march2_f32_kernel<R, false>), dim3(nb), dim3(256), 0, c->stream, g,
-                           in, out, a, b);
+}
+
+int multi_waves(int flags)
+{
+    return (flags & kTuneWG1024) ? 16 : ((flags & kTuneWG512) ? 8 : 4);
+}
+
+// rows per wave: the configured count, else the default for the shape
+static int pick_rows(const Context *c, int N, int D, int W)
+{
+    const int r = c->tune.rtb;
+    if (r && multi_shape_built(D, W, r)) return r;
+    if (W == 4) return (D == 2 && N >= 512) ? 6 : 4;
+    return (W == 8) ? 3 : 2;
+}
+
+// planes per chunk: the configured count if it divides N, else long enough
+// to amortise the 2D halo planes and short enough to give the chip work
+// (measured best: 64 at 512^3 and 1024^3, 32 at 256^3, 8 at 128^3)
+static int pick_chunk(const Context *c, int N)
+{
+    int ci = c->tune.ci32;
+    if (ci > 0 && N % ci == 0) return ci;
+    ci = (N >= 512) ? 64 : ((N >= 256) ? 32 : 8);
+    while (N % ci != 0) --ci;
+    return ci;
+}
+
+// calls f.run<D, W, R>() for the configured shape
+template <int D, class F>
+static int dispatch_shape(int W, int R, F &f)
+{
+    if (W == 16) {
+        if (R == 3) return f.template run<D, 16, 3>();
+        return f.template run<D, 16, 2>();
+    }
+    if (W == 8) {
+        if (R == 2) return f.template run<D, 8, 2>();
+        if (R == 3) return f.template run<D, 8, 3>();
+        return f.template run<D, 8, 4>();
+    }
+    if constexpr (D == 2) {
+        if (R == 8) return f.template run<D, 4, 8>();
+        if (R == 4) return f.template run<D, 4, 4>();
+        return f.template run<D, 4, 6>();
+    } else if constexpr (D == 3) {
+        if (R == 4) return f.template run<D, 4, 4>();
+        return f.template run<D, 4, 6>();
+    } else {
+        set_error("depth %d is not built for 4 waves per work-group", D);
+        return AXS_ERR_UNSUPPORTED;
+    }
+}
+
+template <int D>
+struct SingleLaunch {
+    Context *c;
+    int N;
+    const fvec4 *in;
+    fvec4 *out;
+    TBScalars<D> sc;
+    bool pf;
+    template <int DD, int W, int R>
+    int run()
+    {
+        constexpr int TJ = W * R - 2 * D;
+        constexpr int TK = kTBW - 2 * D;
+        static_assert(TJ > 0 && W * R <= 64, "tile shape");
+        TBGeom g;
+        g.N = N;
+        g.nTj = (N + TJ - 1) / TJ;
+        g.nTk = (N + TK - 1) / TK;
+        g.CI = pick_chunk(c, N);
+        const uint32_t nb = (uint32_t)g.nTj * (uint32_t)g.nTk * (uint32_t)(N / g.CI);
+        g.remap = (nb % 8 == 0) ? 1 : 0;
+        if (pf)
+            hipLaunchKernelGGL((marchd_f32_kernel<D, W, R, true>), dim3(nb), dim3(64 * W), 0,
+                               c->stream, g, in, out, sc);
+        else
+            hipLaunchKernelGGL((marchd_f32_kernel<D, W, R, false>), dim3(nb), dim3(64 * W), 0,
+                               c->stream, g, in, out, sc);
+        return AXS_OK;
+    }
+};
+
+template <int D>
+struct BatchLaunch {
+    Context *c;
+    int src;
+    TBScalars<D> sc;
+    bool pf;
+    template <int DD, int W, int R>
+    int run()
+    {
+        auto kern = pf ? marchd_batch_kernel<D, W, R, true> : marchd_batch_kernel<D, W, R, false>;
+        int per_cu = 0;
+        AXS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, 0));
+        const uint32_t cap = (uint32_t)c->num_cu * (uint32_t)(per_cu > 0 ? per_cu : 1);
+        const uint32_t grid = c->mbatch_items < cap ? c->mbatch_items : cap;
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * W), 0, c->stream, c->mbatch_tab[src],
+                           (int)c->members.size(), c->mbatch_items, sc);
+        return AXS_OK;
+    }
+};
+
+template <int D>
+static int launch_depth(Context *c, int src, int dst, const StepScalars *s)
+{
+    TBScalars<D> sc;
+    for (int l = 0; l < D; ++l) sc.s[l] = to_float(s[l]);
+    const bool pf = (c->tune.flags & kTunePrefetch) != 0;
+    const int W = multi_waves(c->tune.flags);
+    if (c->members.size() > 1 && c->mbatch_tab[src]) {
+        BatchLaunch<D> f{c, src, sc, pf};
+        return dispatch_shape<D>(W, pick_rows(c, c->members[0].N, D, W), f);
+    }
+    for (Member &m : c->members) {
+        SingleLaunch<D> f{c, m.N, (const fvec4 *)m.state[src][PHI], (fvec4 *)m.state[dst][PHI],
+                          sc, pf};
+        AXS_TRY(dispatch_shape<D>(W, pick_rows(c, m.N, D, W), f));
+    }
     return AXS_OK;
 }
 
-int launch_march2(Context *c, Member &m, int src, int dst, const StepScalars &a,
-                  const StepScalars &b)
+int multi_step_depth(const Context *c)
 {
-    const StepScalarsF fa = to_float(a), fb = to_float(b);
-    const fvec4 *in = (const fvec4 *)m.state[src][PHI];
-    fvec4 *out = (fvec4 *)m.state[dst][PHI];
-    const bool pf = (c->tune.flags & kTunePrefetch) != 0;
-    int rows = c->tune.rtb;
-    if (rows == 0) rows = (m.N >= 512) ? 6 : 4;   // measured best at 1024^3 / 256^3
-    switch (rows) {
-    case 4: return launch_tb<4>(c, m.N, in, out, fa, fb, c->tune.ci32, pf);
-    case 8: return launch_tb<8>(c, m.N, in, out, fa, fb, c->tune.ci32, pf);
-    default: return launch_tb<6>(c, m.N, in, out, fa, fb, c->tune.ci32, pf);
-    }
-}
-
-}  // namespace axs
-
-namespace axs {
-
-bool two_step_active(const Context *c)
-{
-    if (c->tune.depth != 2 || c->tune.kernel == AXS_KERNEL_SIMPLE) return false;
+    if (c->tune.depth < 2 || c->tune.kernel == AXS_KERNEL_SIMPLE) return 1;
     for (const Member &m : c->members)
-        if (!march2_supported(c, m.N)) return false;
-    return true;
+        if (!march2_supported(c, m.N)) return 1;
+    return c->tune.depth;
 }
 
-int launch_two_steps(Context *c, const StepScalars &a, const StepScalars &b)
+int prepare_multi_batch(Context *c)
+{
+    if (c->members.size() < 2) return AXS_OK;
+    const int D = multi_step_depth(c);
+    if (D < 2) return AXS_OK;
+    const int W = multi_waves(c->tune.flags);
+    // one row count for the ensemble: that of its first member's side
+    const int R = pick_rows(c, c->members[0].N, D, W);
+    const int key = ((D * 100 + W) * 100 + R) * 10000 + c->tune.ci32;
+    if (key == c->mbatch_key && c->mbatch_tab[0]) return AXS_OK;
+    const int TJ = W * R - 2 * D, TK = kTBW - 2 * D;
+    std::vector<BatchMember> tab[2];
+    uint32_t items = 0;
+    for (const Member &m : c->members) {
+        BatchMember b{};
+        b.N = m.N;
+        b.L = (uint32_t)m.L;
+        b.item_begin = items;
+        b.nTj = (m.N + TJ - 1) / TJ;
+        b.nTk = (m.N + TK - 1) / TK;
+        b.CI = pick_chunk(c, m.N);
+        items += (uint32_t)b.nTj * b.nTk * (m.N / b.CI);
+        for (int h = 0; h < 2; ++h) {
+            b.in[PHI] = m.state[h][PHI];
+            b.out[PHI] = m.state[h ^ 1][PHI];
+            tab[h].push_back(b);
+        }
+    }
+    for (int h = 0; h < 2; ++h) {
+        if (!c->mbatch_tab[h])
+            AXS_HIP_TRY(hipMalloc(&c->mbatch_tab[h], c->members.size() * sizeof(BatchMember)));
+        AXS_HIP_TRY(hipMemcpyAsync(c->mbatch_tab[h], tab[h].data(),
+                                   tab[h].size() * sizeof(BatchMember), hipMemcpyHostToDevice,
+                                   c->stream));
+    }
+    AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+    c->mbatch_items = items;
+    c->mbatch_key = key;
+    return AXS_OK;
+}
+
+int launch_multi_steps(Context *c, const StepScalars *s, int depth)
 {
     const int src = c->cur, dst = c->cur ^ 1;
-    for (Member &m : c->members) AXS_TRY(launch_march2(c, m, src, dst, a, b));
+    if (depth == 4)
+        AXS_TRY(launch_depth<4>(c, src, dst, s));
+    else if (depth == 3)
+        AXS_TRY(launch_depth<3>(c, src, dst, s));
+    else
+        AXS_TRY(launch_depth<2>(c, src, dst, s));
     AXS_HIP_TRY(hipGetLastError());
     c->cur = dst;
     return AXS_OK;

@@ -169,7 +169,8 @@ int prepare_batch(Context *c)
     for (const Member &m : c->members) march = march && march3d_supported(c, m.N);
     const int rows = march_rows(c);
     int key = march ? rows * 100000 + ((c->prec == AXS_FP32) ? c->tune.ci32 : c->tune.ci64) : 0;
-    if (key == c->batch_key && c->batch_tab[0]) return AXS_OK;
+    if (key == c->batch_key && c->batch_tab[0])
+        return (multi_step_depth(c) > 1) ? prepare_multi_batch(c) : AXS_OK;
     std::vector<BatchMember> tab[2];
     uint32_t items = 0;
     for (const Member &m : c->members) {
@@ -205,6 +206,7 @@ int prepare_batch(Context *c)
     c->batch_items = items;
     c->batch_key = key;
     c->batch_march = march;
+    if (multi_step_depth(c) > 1) AXS_TRY(prepare_multi_batch(c));
     return AXS_OK;
 }
 

@@ -87,11 +87,16 @@ int time_trial(Context *c, int trials, double *ms)
     AXS_TRY(prepare_batch(c));
     const int cur0 = c->cur;
     const double dt = (double)c->p.time_step;
-    const StepScalars s = make_scalars(c->p, c->tau + dt);
-    const StepScalars s2 = make_scalars(c->p, c->tau + dt + dt);
-    // one trial = one launch: a step, or a pair of steps at depth 2
-    const bool pairs = two_step_active(c);
-    auto one = [&]() { return pairs ? launch_two_steps(c, s, s2) : launch_step(c, s); };
+    const int depth = multi_step_depth(c);
+    StepScalars ss[4];
+    double t = c->tau;
+    for (int l = 0; l < 4; ++l) {
+        t = t + dt;
+        ss[l] = make_scalars(c->p, t);
+    }
+    const StepScalars &s = ss[0];
+    // one trial = one launch of one or several steps
+    auto one = [&]() { return depth > 1 ? launch_multi_steps(c, ss, depth) : launch_step(c, s); };
     AXS_TRY(one());                             // warm-up (code objects, caches)
     c->cur = cur0;
     const bool graph = (c->tune.flags & kTuneGraph) != 0;
@@ -127,7 +132,7 @@ int time_trial(Context *c, int trials, double *ms)
     AXS_HIP_TRY(hipEventElapsedTime(&f, c->ev0, c->ev1));
     if (ge) (void)hipGraphExecDestroy(ge);
     if (g) (void)hipGraphDestroy(g);
-    *ms = (double)f / trials / (pairs ? 2 : 1);
+    *ms = (double)f / trials / depth;
     return AXS_OK;
 }
 
@@ -186,22 +191,25 @@ int run_autotune(Context *c, int trials, double *best_ms)
                     cands.push_back(u);
                 }
         if (c->prec != AXS_FP32) continue;
-        // two steps per pass
-        const int rtb_opts[3] = {4, 6, 8};
-        const int chunk_opts2[4] = {0, 16, 32, 64};
-        for (int ri = 0; ri < 3; ++ri)
+        // several steps per pass: (depth, waves per work-group, rows per wave)
+        const int shapes[][3] = {{2, 4, 4}, {2, 8, 2}, {2, 8, 3}, {3, 4, 4}, {3, 8, 2},
+                                 {3, 8, 3}, {3, 8, 4}, {3, 16, 2}, {4, 8, 3}, {4, 8, 4},
+                                 {4, 16, 2}, {4, 16, 3}};
+        const int chunk_opts2[3] = {0, 16, 32};
+        for (const auto &sh : shapes)
             for (int pf = 0; pf < 2; ++pf)
-                for (int ci = 0; ci < 4; ++ci) {
+                for (int ci = 0; ci < 3; ++ci) {
                     int chunk = chunk_opts2[ci];
                     bool ok = true;
                     for (const Member &m : c->members) ok = ok && (chunk == 0 || m.N % chunk == 0);
                     if (!ok) continue;
                     Tune u;
                     u.kernel = AXS_KERNEL_MARCH;
-                    u.depth = 2;
-                    u.rtb = rtb_opts[ri];
+                    u.depth = sh[0];
+                    u.rtb = sh[2];
                     u.ci32 = u.ci64 = chunk;
-                    u.flags = (pf ? kTunePrefetch : 0) | kTuneNT | (graph ? kTuneGraph : 0);
+                    u.flags = (pf ? kTunePrefetch : 0) | kTuneNT | (sh[1] == 8 ? kTuneWG512 : 0) |
+                              (sh[1] == 16 ? kTuneWG1024 : 0) | (graph ? kTuneGraph : 0);
                     cands.push_back(u);
                 }
     }
@@ -210,8 +218,8 @@ int run_autotune(Context *c, int trials, double *best_ms)
     double best_t = 1e300;
     for (const Tune &t : cands) {
         c->tune = t;
-        if (t.depth == 2) {
-            if (!two_step_active(c)) continue;
+        if (t.depth >= 2) {
+            if (multi_step_depth(c) != t.depth) continue;
         } else if (t.kernel == AXS_KERNEL_MARCH) {
             bool ok = true;
             for (const Member &m : c->members) ok = ok && march3d_supported(c, m.N);
@@ -222,6 +230,7 @@ int run_autotune(Context *c, int trials, double *best_ms)
         if (rc != AXS_OK) {
             c->tune = saved;
             c->batch_key = -1;
+            c->mbatch_key = -1;
             return rc;
         }
         if (ms < best_t) {
@@ -231,6 +240,7 @@ int run_autotune(Context *c, int trials, double *best_ms)
     }
     c->tune = best;
     c->batch_key = -1;          // tables follow the chosen shape
+    c->mbatch_key = -1;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tune_cache[key] = {best, best_t};

@@ -16,10 +16,11 @@ The JSON line also carries
                  phi2 and the two velocities, times the site-updates of one
                  launch: N^3 per time step it advances) over the launch's
                  average duration, measured with HIP events on the library's
-                 stream.  The default FP32 kernel advances two time steps per
-                 launch, reading and writing the state once, so this figure
-                 can exceed the HBM peak; `traffic` (measured HBM bytes per
-                 launch) and `traffic_rate` give the bytes actually moved;
+                 stream.  The default FP32 kernel advances four time steps
+                 per launch, reading and writing the state once, so this
+                 figure exceeds the HBM peak; `traffic` (measured HBM bytes
+                 per launch) and `traffic_rate` give the bytes actually
+                 moved;
   cpu_baseline : the CPU oracle (reference-structured FP64 step: drift,
                  two CSR sparse-matrix Laplacians, potential, kick, copy, with
                  OpenMP) timed on this host over a bounded sample;
@@ -41,7 +42,7 @@ HBM_PEAK_GBS = 8000.0
 # measured HBM bytes per launch of the FP32 step kernels at 1024^3, by time
 # steps per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)
 TRAFFIC_FILES = {1: os.path.join(ROOT, "profiles", "traffic_march_fp32_3d_1024.json"),
-                 2: os.path.join(ROOT, "profiles", "traffic_two_step_fp32_3d_1024.json")}
+                 4: os.path.join(ROOT, "profiles", "traffic_multi_step_fp32_3d_1024.json")}
 
 
 def log(msg: str) -> None:

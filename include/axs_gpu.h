@@ -183,16 +183,22 @@ int axs_powerspec(axs_ctx *ctx, int binning, int nbins,
 #define AXS_MARCH_NT_STORE 2   /* non-temporal stores of the new state        */
 #define AXS_MARCH_JFAST    4   /* tiles ordered j fastest (default k fastest) */
 #define AXS_STEP_GRAPH     8   /* axs_step captures its launches in a HIP graph */
+#define AXS_MARCH_WG512   16   /* depth >= 2: 8 waves per work-group (else 4)  */
+#define AXS_MARCH_WG1024  32   /* depth >= 2: 16 waves per work-group          */
 
 typedef struct axs_kernel_config {
     int kernel;   /* AXS_KERNEL_*                                           */
-    int rows;     /* rows per work-item: FP32 4 or 8, FP64 2 or 4; depth 2:
-                     4, 6 or 8                                              */
+    int rows;     /* rows per work-item: FP32 4 or 8, FP64 2 or 4; depth >= 2:
+                     4 or 6 (8 at depth 2; depth 4 needs 8 or 16 waves),
+                     with AXS_MARCH_WG512 2, 3 or 4, with AXS_MARCH_WG1024
+                     2 or 3                                                 */
     int chunk;    /* planes marched per work-group (divides N)              */
     int depth;    /* time steps per pass over the lattice: 1 (0 = 1), or 2
-                     (FP32 form, 3D, stencil 0, every side >= 64: a new
-                     context's default there).  At depth 2 an odd step
-                     count ends with a one-step launch.                     */
+                     to 4 (FP32 form, 3D, stencil 0, every side >= 64; a new
+                     context there starts at depth 4, or 3 if a side is
+                     below 256, with AXS_MARCH_WG1024).
+                     Step counts that are not a multiple end with one-step
+                     launches.                                              */
     int flags;    /* AXS_MARCH_* bits                                       */
 } axs_kernel_config;
 
@@ -215,8 +221,8 @@ int axs_autotune(axs_ctx *ctx, int trials, axs_kernel_config *best, double *ms_p
 int axs_last_step_ms(const axs_ctx *ctx, double *ms_per_step);
 
 /* Bytes the configured step kernel must move per site-update (the scheme
- * bytes: FP64 96, FP32 32, FP32 at depth 2 16 as one pass reads and writes
- * the state once for two updates) and the path's algorithmic bytes per
+ * bytes: FP64 96, FP32 32 / depth, as one pass reads and writes the state
+ * once for `depth` updates) and the path's algorithmic bytes per
  * site-update (read+write of phi and its velocity: FP64 64, FP32 32).     */
 int axs_bytes_per_site(const axs_ctx *ctx, double *scheme_bytes, double *alg_bytes);
 

@@ -128,12 +128,13 @@ def test_autotune_keeps_state_and_results(nd, Ns, precision):
             np.testing.assert_array_equal(a[k], b[k])
 
 
-def test_two_step_kernel_on_ensemble_matches_single():
-    # depth 2 on an uneven FP32 ensemble: every member equals its lone
-    # one-step run
+@pytest.mark.parametrize("depth,flags,rows", [(2, 3, 0), (3, 35, 0), (4, 19, 3), (3, 2, 4)])
+def test_multi_step_kernel_on_ensemble_matches_single(depth, flags, rows):
+    # several steps per launch on an uneven FP32 ensemble (one persistent
+    # launch for all members): every member equals its lone one-step run
     Ns = [64, 128, 96]
     lat = axs.Lattice(3, members=Ns, precision="fp32")
-    lat.set_kernel("march", depth=2)
+    lat.set_kernel("march", depth=depth, flags=flags, rows=rows)
     singles = []
     for r, N in enumerate(Ns):
         p1, p2 = kinv_field(3, N, seed=6000 + r)

@@ -163,22 +163,29 @@ def test_march_kernel_fp64_oracle_192():
     _assert_same(lat, st)
 
 
-@pytest.mark.parametrize("N,rows,chunk,flags,nsteps", [
-    (64, 4, 0, 3, 8), (64, 8, 16, 2, 7), (100, 6, 0, 3, 9), (128, 6, 32, 3, 12),
-    (128, 8, 0, 2, 6), (192, 4, 64, 3, 5), (128, 6, 0, 11, 10)])
-def test_two_step_kernel_matches_one_step(N, rows, chunk, flags, nsteps):
-    # two time steps per pass (temporal blocking) evaluate the same
-    # expressions as two one-step launches: identical bits on every side
-    # (also sides the tiles do not divide), launch shape and step count (an
-    # odd count ends with a one-step launch)
+@pytest.mark.parametrize("N,depth,rows,chunk,flags,nsteps", [
+    (64, 2, 4, 0, 3, 8), (64, 2, 8, 16, 2, 7), (100, 2, 6, 0, 3, 9), (128, 2, 6, 32, 3, 12),
+    (192, 2, 4, 64, 3, 5), (128, 2, 6, 0, 11, 10), (128, 2, 3, 0, 19, 6), (96, 2, 2, 0, 18, 5),
+    (64, 3, 4, 0, 3, 9), (100, 3, 6, 0, 3, 10), (128, 3, 4, 32, 2, 8), (128, 3, 4, 0, 19, 7),
+    (192, 3, 3, 0, 27, 6), (128, 3, 6, 16, 3, 3), (80, 3, 2, 0, 18, 11), (128, 3, 2, 0, 34, 7),
+    (128, 4, 3, 0, 19, 9), (100, 4, 2, 0, 35, 8), (192, 4, 4, 32, 18, 7), (64, 2, 3, 16, 35, 6),
+    (64, 4, 3, 0, 34, 12)])
+def test_multi_step_kernel_matches_one_step(N, depth, rows, chunk, flags, nsteps):
+    # two to four time steps per pass (temporal blocking) evaluate the same
+    # expressions as one-step launches: identical bits on every side (also
+    # sides the tiles do not divide), launch shape (4, 8 or 16 waves per
+    # work-group: AXS_MARCH_WG512 / AXS_MARCH_WG1024) and step count (a
+    # count that is not a multiple of the depth ends with one-step launches)
     state = random_state(3, N, seed=11, ker=True)
     a = axs.Lattice(3, N, precision="fp32")
     a.set_kernel("simple")
     a.set_state(*state, tau=1.0)
     a.step(nsteps)
     b = axs.Lattice(3, N, precision="fp32")
-    b.set_kernel("march", rows=rows, chunk=chunk, depth=2, flags=flags)
-    assert b.kernel_config()["depth"] == 2 and b.kernel_config()["rows"] == rows
+    b.set_kernel("march", rows=rows, chunk=chunk, depth=depth, flags=flags)
+    cfg = b.kernel_config()
+    assert cfg["depth"] == depth and cfg["rows"] == rows
+    assert b.bytes_per_site() == (32.0 / depth, 32.0)
     b.set_state(*state, tau=1.0)
     b.step(nsteps)
     ga, gb = a.get_state(), b.get_state()
@@ -194,11 +201,21 @@ def test_two_step_kernel_matches_one_step(N, rows, chunk, flags, nsteps):
     (3, 64, "fp64", "march"), (2, 128, "fp32", "auto"), (3, 48, "fp32", "march"),
     (3, 64, "fp32", "simple")])
 def test_two_step_kernel_rejects_other_forms(nd, N, precision, kernel):
-    # a rejected configuration leaves the context's own one in place (the
-    # FP32 3D 64^3 context starts at depth 2)
+    # a rejected configuration leaves the context's own one in place (an
+    # FP32 3D context below side 256 starts at depth 3)
     lat = axs.Lattice(nd, N, precision=precision)
     before = lat.kernel_config()
-    assert before["depth"] == (2 if (nd, N, precision) == (3, 64, "fp32") else 1)
+    assert before["depth"] == (3 if (nd, N, precision) == (3, 64, "fp32") else 1)
     with pytest.raises(axs.AxsError, match="depth 2"):
         lat.set_kernel(kernel, depth=2)
+    assert lat.kernel_config() == before
+
+
+@pytest.mark.parametrize("depth,rows,flags", [(4, 0, 3), (5, 0, 19), (3, 8, 3), (2, 6, 19),
+                                              (2, 5, 3), (3, 4, 35), (2, 2, 51)])
+def test_multi_step_kernel_rejects_unbuilt_shapes(depth, rows, flags):
+    lat = axs.Lattice(3, 64, precision="fp32")
+    before = lat.kernel_config()
+    with pytest.raises(axs.AxsError):
+        lat.set_kernel("march", rows=rows, depth=depth, flags=flags)
     assert lat.kernel_config() == before
