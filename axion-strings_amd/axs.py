@@ -36,6 +36,12 @@ AXS_BIN_LINEAR = 0
 AXS_BIN_LOG = 1
 AXS_INIT_KINV = 0
 AXS_INIT_THERMAL = 1
+AXS_EOM_FIELD_RESCALED = 0
+AXS_EOM_HAMILTONIAN = 1
+AXS_LAPLACIAN_STENCIL = 0
+AXS_LAPLACIAN_SPECTRAL = 1
+EOMS = {"field_rescaled": AXS_EOM_FIELD_RESCALED, "hamiltonian": AXS_EOM_HAMILTONIAN}
+LAPLACIANS = {"stencil": AXS_LAPLACIAN_STENCIL, "spectral": AXS_LAPLACIAN_SPECTRAL}
 
 # every entry point declared in include/axs_gpu.h
 EXPORTS = (
@@ -46,7 +52,7 @@ EXPORTS = (
     "axs_string_counts", "axs_member_background", "axs_background",
     "axs_member_energy", "axs_energy", "axs_member_powerspec", "axs_powerspec",
     "axs_last_step_ms", "axs_bytes_per_site", "axs_strerror", "axs_last_error",
-    "axs_set_kernel_config", "axs_get_kernel_config", "axs_autotune",
+    "axs_set_kernel_config", "axs_get_kernel_config", "axs_autotune", "axs_member_laplacian",
 )
 
 AXS_KERNEL_AUTO = 0
@@ -84,6 +90,8 @@ class AxsParams(ctypes.Structure):
         ("thr", c_int),
         ("precision", c_int),
         ("device", c_int),
+        ("eom", c_int),
+        ("laplacian", c_int),
     ]
 
 
@@ -132,6 +140,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "axs_powerspec": ([c_void_p, c_int, c_int, _DP, _DP, POINTER(c_int32)], c_int),
         "axs_last_step_ms": ([c_void_p, _DP], c_int),
         "axs_bytes_per_site": ([c_void_p, _DP, _DP], c_int),
+        "axs_member_laplacian": ([c_void_p, c_int, c_int, _DP, _DP], c_int),
         "axs_set_kernel_config": ([c_void_p, POINTER(AxsKernelConfig)], c_int),
         "axs_get_kernel_config": ([c_void_p, POINTER(AxsKernelConfig)], c_int),
         "axs_autotune": ([c_void_p, c_int, POINTER(AxsKernelConfig), _DP], c_int),
@@ -180,11 +189,13 @@ class Lattice:
     def __init__(self, ndims: int, N: int = 0, *, space_step: float = 0.5,
                  time_step: float = 0.1, lam: float = 1.0, T_initial: float = 4.0,
                  stencil_setting: int = 0, thr: int = 1, precision: str = "fp64",
-                 device: int = 0, members=None):
+                 device: int = 0, members=None, eom: str = "field_rescaled",
+                 laplacian: str = "stencil"):
         self.lib = load_library()
         self.params = AxsParams(ndims, N if members is None else int(members[0]),
                                 space_step, time_step, lam, T_initial, stencil_setting,
-                                thr, PRECISIONS[precision], device)
+                                thr, PRECISIONS[precision], device, EOMS[eom],
+                                LAPLACIANS[laplacian])
         self.ndims = ndims
         self.precision = precision
         self._ctx = c_void_p()
@@ -327,6 +338,15 @@ class Lattice:
         _check(self.lib.axs_member_background(self._ctx, member, _dp(out)),
                "axs_member_background")
         return out
+
+    def laplacian(self, mode: str = "stencil", member: int = 0):
+        """(lap1, lap2) of the current field: the stencil operator
+        (1/dx^2) A phi or the spectral one (compute_laplacian_fft)."""
+        L = self.Ns[member] ** self.ndims
+        l1, l2 = np.empty(L), np.empty(L)
+        _check(self.lib.axs_member_laplacian(self._ctx, member, LAPLACIANS[mode], _dp(l1),
+                                             _dp(l2)), "axs_member_laplacian")
+        return l1, l2
 
     def energy(self, member: int = 0) -> np.ndarray:
         out = np.zeros(3, np.float64)

@@ -63,11 +63,22 @@ StepScalars make_scalars(const axs_params &p, double tau)
     StepScalars s{};
     double dx = (double)p.space_step;
     s.dt = (double)p.time_step;
-    s.alpha = (double)1.0f / (dx * dx);
-    s.cl = ((double)(-1.0f) / (tau * tau)) * (double)p.lambda;
-    s.tau2 = tau * tau;
     double T0 = (double)p.T_initial;
-    s.T03 = (T0 * T0) / (double)3.0f;
+    if (p.eom == AXS_EOM_HAMILTONIAN) {
+        // s_integrate.cpp:364-380, tau already advanced
+        double t2 = tau * tau;
+        s.dcoef = s.dt / t2;
+        s.hdt = (double)(0.5f * p.time_step);
+        s.alpha = t2 / (dx * dx);
+        s.cl = (-t2) * (double)p.lambda;
+        s.tau2 = 1.0;                                  // the "- 1.0f" term
+        s.T03 = (T0 * T0) / ((double)3.0f * t2);
+    } else {
+        s.alpha = (double)1.0f / (dx * dx);
+        s.cl = ((double)(-1.0f) / (tau * tau)) * (double)p.lambda;
+        s.tau2 = tau * tau;
+        s.T03 = (T0 * T0) / (double)3.0f;
+    }
     s.ncoef = stencil_coefficients(p.stencil_setting, p.ndims, s.c);
     return s;
 }
@@ -225,6 +236,7 @@ Context::~Context()
     if (stream) (void)hipStreamSynchronize(stream);
     destroy_plans(this);
     destroy_inverse_plans(this);
+    destroy_spec_plans(this);
     for (Member &m : members) {
         for (int h = 0; h < 2; ++h)
             for (int a = 0; a < 3; ++a)
@@ -295,6 +307,21 @@ static int validate_params(const axs_params *p, int N)
     if (!(p->time_step > 0.0f) || !(p->space_step > 0.0f)) {
         set_error("time_step and space_step must be positive");
         return AXS_ERR_INVALID_ARG;
+    }
+    if ((p->eom != AXS_EOM_FIELD_RESCALED && p->eom != AXS_EOM_HAMILTONIAN) ||
+        (p->laplacian != AXS_LAPLACIAN_STENCIL && p->laplacian != AXS_LAPLACIAN_SPECTRAL)) {
+        set_error("eom must be AXS_EOM_* and laplacian AXS_LAPLACIAN_* (got %d, %d)", p->eom,
+                  p->laplacian);
+        return AXS_ERR_INVALID_ARG;
+    }
+    if ((p->eom != AXS_EOM_FIELD_RESCALED || p->laplacian != AXS_LAPLACIAN_STENCIL) &&
+        p->precision != AXS_FP64) {
+        set_error("the Hamiltonian form and the spectral Laplacian need AXS_FP64");
+        return AXS_ERR_UNSUPPORTED;
+    }
+    if (p->eom == AXS_EOM_HAMILTONIAN && p->laplacian == AXS_LAPLACIAN_SPECTRAL) {
+        set_error("the reference has no Hamiltonian-form step with the spectral Laplacian");
+        return AXS_ERR_UNSUPPORTED;
     }
     return AXS_OK;
 }
@@ -594,7 +621,9 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
     if (nsteps == 0) return AXS_OK;
     AXS_TRY(prepare_batch(c));
     const double dt = (double)c->p.time_step;
-    const bool graph = (c->tune.flags & kTuneGraph) && nsteps > 1;
+    // the spectral step's rocFFT calls stay out of graph capture
+    const bool graph = (c->tune.flags & kTuneGraph) && nsteps > 1 &&
+                       c->p.laplacian == AXS_LAPLACIAN_STENCIL;
     const double tau0 = c->tau;
     const int cur0 = c->cur;
     hipGraph_t g = nullptr;
@@ -713,6 +742,10 @@ int axs_member_energy(axs_ctx *ctx, int member, double out[3])
         set_error("null output");
         return AXS_ERR_INVALID_ARG;
     }
+    if (ctx->impl.p.eom != AXS_EOM_FIELD_RESCALED) {
+        set_error("the energy functional is defined for the field-rescaled form");
+        return AXS_ERR_UNSUPPORTED;
+    }
     return run_energy(&ctx->impl, ctx->impl.members[member], out);
 }
 
@@ -819,6 +852,16 @@ int axs_last_step_ms(const axs_ctx *ctx, double *ms)
     }
     *ms = ctx->impl.last_step_ms;
     return AXS_OK;
+}
+
+int axs_member_laplacian(axs_ctx *ctx, int member, int mode, double *lap1, double *lap2)
+{
+    AXS_TRY(check_member(ctx, member));
+    if (mode != AXS_LAPLACIAN_STENCIL && mode != AXS_LAPLACIAN_SPECTRAL) {
+        set_error("mode must be AXS_LAPLACIAN_STENCIL or AXS_LAPLACIAN_SPECTRAL");
+        return AXS_ERR_INVALID_ARG;
+    }
+    return run_laplacian(&ctx->impl, ctx->impl.members[member], mode, lap1, lap2);
 }
 
 int axs_bytes_per_site(const axs_ctx *ctx, double *scheme_bytes, double *alg_bytes)

@@ -44,6 +44,11 @@ struct Status {
 // in the reference's expression order (s_integrate.cpp:308-334).
 
 struct StepScalars {
+    // Hamiltonian form (s_integrate.cpp:358-395): alpha = tau^2 / dx^2,
+    // cl = -tau^2 * lambda, tau2 = 1, T03 = T0^2 / (3 tau^2), and the
+    // drift phi + dcoef * (pi + hdt * ker) with dcoef = dt / tau^2,
+    // hdt = 0.5f * dt (a float product)
+    double dcoef, hdt;
     double dt;      // (double) parameters.time_step
     double alpha;   // 1.0f / pow_2(dx)
     double cl;      // (-1.0f / pow_2(tau)) * lambda, tau after the update
@@ -143,6 +148,7 @@ struct Context {
 
     std::vector<FftPlan *> fft_plans;
     std::vector<void *> inv_plans;   // device-init inverse transforms (axs_init.hip)
+    std::vector<void *> spec_plans;  // spectral-Laplacian transforms (axs_spectral.hip)
 
     // ensemble tables (device), one per ping-pong parity, rebuilt when the
     // launch shape changes
@@ -185,6 +191,11 @@ int multi_waves(int flags);
 int launch_multi_steps(Context *c, const StepScalars *s, int depth);
 // work-item tables of the multi-step kernel for an ensemble
 int prepare_multi_batch(Context *c);
+// field-rescaled step with the spectral Laplacian, one member
+int run_spectral_step(Context *c, Member &m, int src, int dst, const StepScalars &s);
+// Laplacian of the current field (AXS_LAPLACIAN_*) into host arrays
+int run_laplacian(Context *c, Member &m, int mode, double *h1, double *h2);
+void destroy_spec_plans(Context *c);
 // march geometry chosen for a member (rows per work-item, planes per chunk)
 int march_rows(const Context *c);
 int march_chunk(const Context *c, int N);

@@ -33,7 +33,8 @@ constexpr int kMK = 64;          // tile width along k = wave width
 constexpr int kMW = 4;           // waves per work-group
 constexpr int kLDK = kMK + 2;    // LDS row stride (halo column each side)
 
-struct ExactF64 {
+template <bool HAM>
+struct ExactF64T {
     using T = double;
     using S = StepScalars;
     struct In {
@@ -61,13 +62,18 @@ struct ExactF64 {
         r.v = a.vel[q];
         r.k = a.ker[q];
     }
-    // drift, s_integrate.cpp:313-314
+    // drift, s_integrate.cpp:313-314; Hamiltonian form :368-369
     static __device__ __forceinline__ void drift(const Raw &r, const S &s, double &x1, double &x2,
                                                  Own &o)
     {
         o = r;
-        x1 = r.p.x + s.dt * (r.v.x + 0.5 * r.k.x * s.dt);
-        x2 = r.p.y + s.dt * (r.v.y + 0.5 * r.k.y * s.dt);
+        if (HAM) {
+            x1 = r.p.x + s.dcoef * (r.v.x + s.hdt * r.k.x);
+            x2 = r.p.y + s.dcoef * (r.v.y + s.hdt * r.k.y);
+        } else {
+            x1 = r.p.x + s.dt * (r.v.x + 0.5 * r.k.x * s.dt);
+            x2 = r.p.y + s.dt * (r.v.y + 0.5 * r.k.y * s.dt);
+        }
     }
     // s_integrate.cpp:322-340 for one site, sums already in COO order
     template <bool NT>
@@ -95,6 +101,8 @@ struct ExactF64 {
         }
     }
 };
+
+using ExactF64 = ExactF64T<false>;
 
 struct HalfF32 {
     using T = float;
@@ -377,8 +385,8 @@ static int pick_ci(int N, int tiles_per_plane)
 
 bool march3d_supported(const Context *c, int N)
 {
-    return c->nd == 3 && c->p.stencil_setting == 0 && N % kMK == 0 && N >= kMK &&
-           N % (kMW * march_rows(c)) == 0;
+    return c->nd == 3 && c->p.stencil_setting == 0 && c->p.laplacian == AXS_LAPLACIAN_STENCIL &&
+           N % kMK == 0 && N >= kMK && N % (kMW * march_rows(c)) == 0;
 }
 
 int march_rows(const Context *c) { return (c->prec == AXS_FP32) ? c->tune.rj32 : c->tune.rj64; }
@@ -461,30 +469,39 @@ static int launch_march(Context *c, int N, typename Tr::In in, typename Tr::Out 
     return AXS_OK;
 }
 
+template <class Tr>
+static int launch_march_f64(Context *c, Member &m, int src, int dst, const StepScalars &s)
+{
+    const int N = m.N;
+    typename Tr::In in{(const dvec2 *)m.state[src][PHI], (const dvec2 *)m.state[src][VEL],
+                       (const dvec2 *)m.state[src][KER]};
+    typename Tr::Out out{(dvec2 *)m.state[dst][PHI], (dvec2 *)m.state[dst][VEL],
+                         (dvec2 *)m.state[dst][KER]};
+    const int f = c->tune.flags & (kTunePrefetch | kTuneNT);
+    const int ci = c->tune.ci64;
+    if (c->tune.rj64 == 2) {
+        switch (f) {
+        case 0: return launch_march<Tr, 2, false, false>(c, N, in, out, s, ci);
+        case kTunePrefetch: return launch_march<Tr, 2, true, false>(c, N, in, out, s, ci);
+        case kTuneNT: return launch_march<Tr, 2, false, true>(c, N, in, out, s, ci);
+        default: return launch_march<Tr, 2, true, true>(c, N, in, out, s, ci);
+        }
+    }
+    switch (f) {
+    case 0: return launch_march<Tr, 4, false, false>(c, N, in, out, s, ci);
+    case kTunePrefetch: return launch_march<Tr, 4, true, false>(c, N, in, out, s, ci);
+    case kTuneNT: return launch_march<Tr, 4, false, true>(c, N, in, out, s, ci);
+    default: return launch_march<Tr, 4, true, true>(c, N, in, out, s, ci);
+    }
+}
+
 int launch_march3d(Context *c, Member &m, int src, int dst, const StepScalars &s)
 {
     const int N = m.N;
     if (c->prec == AXS_FP64) {
-        ExactF64::In in{(const dvec2 *)m.state[src][PHI], (const dvec2 *)m.state[src][VEL],
-                        (const dvec2 *)m.state[src][KER]};
-        ExactF64::Out out{(dvec2 *)m.state[dst][PHI], (dvec2 *)m.state[dst][VEL],
-                          (dvec2 *)m.state[dst][KER]};
-        const int f = c->tune.flags & (kTunePrefetch | kTuneNT);
-        const int rj = c->tune.rj64;
-        if (rj == 2) {
-            switch (f) {
-            case 0: return launch_march<ExactF64, 2, false, false>(c, N, in, out, s, c->tune.ci64);
-            case kTunePrefetch: return launch_march<ExactF64, 2, true, false>(c, N, in, out, s, c->tune.ci64);
-            case kTuneNT: return launch_march<ExactF64, 2, false, true>(c, N, in, out, s, c->tune.ci64);
-            default: return launch_march<ExactF64, 2, true, true>(c, N, in, out, s, c->tune.ci64);
-            }
-        }
-        switch (f) {
-        case 0: return launch_march<ExactF64, 4, false, false>(c, N, in, out, s, c->tune.ci64);
-        case kTunePrefetch: return launch_march<ExactF64, 4, true, false>(c, N, in, out, s, c->tune.ci64);
-        case kTuneNT: return launch_march<ExactF64, 4, false, true>(c, N, in, out, s, c->tune.ci64);
-        default: return launch_march<ExactF64, 4, true, true>(c, N, in, out, s, c->tune.ci64);
-        }
+        if (c->p.eom == AXS_EOM_HAMILTONIAN)
+            return launch_march_f64<ExactF64T<true>>(c, m, src, dst, s);
+        return launch_march_f64<ExactF64>(c, m, src, dst, s);
     }
     StepScalarsF f = to_float(s);
     HalfF32::In in{(const fvec4 *)m.state[src][PHI]};

@@ -30,23 +30,33 @@ struct Out64 {
     dvec2 *__restrict__ phi, *__restrict__ vel, *__restrict__ ker;
 };
 
-// drift, s_integrate.cpp:313-314: phi += dt * (phidot + 0.5f * ker * dt)
-__device__ __forceinline__ dvec2 drift64(const In64 &a, uint32_t q, double dt)
+// drift, s_integrate.cpp:313-314: phi += dt * (phidot + 0.5f * ker * dt);
+// Hamiltonian form :368-369: phi += dt / pow_2(tau) * (pi + 0.5f * dt * ker)
+template <bool HAM>
+__device__ __forceinline__ dvec2 drift64(const In64 &a, uint32_t q, const StepScalars &s)
 {
     dvec2 p = a.phi[q], v = a.vel[q], k = a.ker[q];
     dvec2 x;
-    x.x = p.x + dt * (v.x + 0.5 * k.x * dt);
-    x.y = p.y + dt * (v.y + 0.5 * k.y * dt);
+    if (HAM) {
+        x.x = p.x + s.dcoef * (v.x + s.hdt * k.x);
+        x.y = p.y + s.dcoef * (v.y + s.hdt * k.y);
+    } else {
+        x.x = p.x + s.dt * (v.x + 0.5 * k.x * s.dt);
+        x.y = p.y + s.dt * (v.y + 0.5 * k.y * s.dt);
+    }
     return x;
 }
 
-template <int ND>
+// The Hamiltonian form differs only in the drift and in the scalars (its
+// Laplacian, potential and kick have the same expression trees with
+// alpha = tau^2/dx^2, cl = -tau^2 lambda, tau2 = 1, T03 = T0^2/(3 tau^2)).
+template <int ND, bool HAM>
 __device__ __forceinline__ void site_fp64(int N, uint32_t m, const In64 &in, const Out64 &out,
                                           const StepScalars &s)
 {
     Site<ND> site = decompose<ND>(m, N);
 
-    dvec2 x = drift64(in, m, s.dt);
+    dvec2 x = drift64<HAM>(in, m, s);
 
     // Laplacian in COO row order (centre first), then alpha (s_integrate.cpp:322-324)
     double s1 = s.c[0] * x.x, s2 = s.c[0] * x.y;
@@ -54,10 +64,10 @@ __device__ __forceinline__ void site_fp64(int N, uint32_t m, const In64 &in, con
         double cl = s.c[l];
 #pragma unroll
         for (int a = 0; a < ND; ++a) {
-            dvec2 y = drift64(in, shifted<ND>(site, a, l, N), s.dt);
+            dvec2 y = drift64<HAM>(in, shifted<ND>(site, a, l, N), s);
             s1 = s1 + cl * y.x;
             s2 = s2 + cl * y.y;
-            y = drift64(in, shifted<ND>(site, a, -l, N), s.dt);
+            y = drift64<HAM>(in, shifted<ND>(site, a, -l, N), s);
             s1 = s1 + cl * y.x;
             s2 = s2 + cl * y.y;
         }
@@ -80,12 +90,12 @@ __device__ __forceinline__ void site_fp64(int N, uint32_t m, const In64 &in, con
     out.ker[m] = kn;
 }
 
-template <int ND>
+template <int ND, bool HAM>
 __global__ void __launch_bounds__(256)
 step_fp64_kernel(int N, uint32_t L, In64 in, Out64 out, StepScalars s)
 {
     uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < L) site_fp64<ND>(N, m, in, out, s);
+    if (m < L) site_fp64<ND, HAM>(N, m, in, out, s);
 }
 
 template <int ND>
@@ -134,7 +144,7 @@ step_fp32_kernel(int N, uint32_t L, const fvec4 *__restrict__ in, fvec4 *__restr
 // stencils, other sides): one launch per step over the 256-site blocks of all
 // members; each block finds its member by binary search over the members'
 // first block.
-template <int ND, bool F64>
+template <int ND, bool F64, bool HAM>
 __global__ void __launch_bounds__(256)
 batch_site_kernel(const BatchMember *__restrict__ mem, int n_members, StepScalars s,
                   StepScalarsF sf)
@@ -154,7 +164,7 @@ batch_site_kernel(const BatchMember *__restrict__ mem, int n_members, StepScalar
     if (F64) {
         In64 in{(const dvec2 *)mb.in[PHI], (const dvec2 *)mb.in[VEL], (const dvec2 *)mb.in[KER]};
         Out64 out{(dvec2 *)mb.out[PHI], (dvec2 *)mb.out[VEL], (dvec2 *)mb.out[KER]};
-        site_fp64<ND>(mb.N, m, in, out, s);
+        site_fp64<ND, HAM>(mb.N, m, in, out, s);
     } else {
         site_fp32<ND>(mb.N, m, (const fvec4 *)mb.in[PHI], (fvec4 *)mb.out[PHI], sf);
     }
@@ -165,7 +175,10 @@ batch_site_kernel(const BatchMember *__restrict__ mem, int n_members, StepScalar
 int prepare_batch(Context *c)
 {
     if (c->members.size() < 2) return AXS_OK;
-    bool march = c->tune.kernel != AXS_KERNEL_SIMPLE;
+    // the spectral step runs member by member (rocFFT per lattice)
+    if (c->p.laplacian != AXS_LAPLACIAN_STENCIL) return AXS_OK;
+    // the Hamiltonian form's ensembles use the site kernel
+    bool march = c->tune.kernel != AXS_KERNEL_SIMPLE && c->p.eom == AXS_EOM_FIELD_RESCALED;
     for (const Member &m : c->members) march = march && march3d_supported(c, m.N);
     const int rows = march_rows(c);
     int key = march ? rows * 100000 + ((c->prec == AXS_FP32) ? c->tune.ci32 : c->tune.ci64) : 0;
@@ -253,7 +266,7 @@ static inline unsigned grid_for(uint32_t L, unsigned block) { return (L + block 
 int launch_step(Context *c, const StepScalars &s)
 {
     const int src = c->cur, dst = c->cur ^ 1;
-    if (c->members.size() > 1 && c->batch_tab[src]) {
+    if (c->members.size() > 1 && c->batch_tab[src] && c->p.laplacian == AXS_LAPLACIAN_STENCIL) {
         // one launch for the whole ensemble (tables built by prepare_batch)
         if (c->batch_march) {
             AXS_TRY(launch_march3d_batch(c, src, s));
@@ -262,10 +275,15 @@ int launch_step(Context *c, const StepScalars &s)
             const int nm = (int)c->members.size();
             dim3 grid(c->batch_items), block(256);
             if (c->prec == AXS_FP64) {
-                auto k = (c->nd == 2) ? batch_site_kernel<2, true> : batch_site_kernel<3, true>;
+                auto k = (c->p.eom == AXS_EOM_HAMILTONIAN)
+                             ? ((c->nd == 2) ? batch_site_kernel<2, true, true>
+                                             : batch_site_kernel<3, true, true>)
+                             : ((c->nd == 2) ? batch_site_kernel<2, true, false>
+                                             : batch_site_kernel<3, true, false>);
                 hipLaunchKernelGGL(k, grid, block, 0, c->stream, c->batch_tab[src], nm, s, f);
             } else {
-                auto k = (c->nd == 2) ? batch_site_kernel<2, false> : batch_site_kernel<3, false>;
+                auto k = (c->nd == 2) ? batch_site_kernel<2, false, false>
+                                      : batch_site_kernel<3, false, false>;
                 hipLaunchKernelGGL(k, grid, block, 0, c->stream, c->batch_tab[src], nm, s, f);
             }
         }
@@ -276,6 +294,10 @@ int launch_step(Context *c, const StepScalars &s)
     for (Member &m : c->members) {
         uint32_t L = (uint32_t)m.L;
         unsigned grid = grid_for(L, 256);
+        if (c->p.laplacian == AXS_LAPLACIAN_SPECTRAL) {
+            AXS_TRY(run_spectral_step(c, m, src, dst, s));
+            continue;
+        }
         if (c->tune.kernel != AXS_KERNEL_SIMPLE && march3d_supported(c, m.N)) {
             AXS_TRY(launch_march3d(c, m, src, dst, s));
             continue;
@@ -285,12 +307,10 @@ int launch_step(Context *c, const StepScalars &s)
                     (const dvec2 *)m.state[src][KER]};
             Out64 out{(dvec2 *)m.state[dst][PHI], (dvec2 *)m.state[dst][VEL],
                       (dvec2 *)m.state[dst][KER]};
-            if (c->nd == 2)
-                hipLaunchKernelGGL(step_fp64_kernel<2>, dim3(grid), dim3(256), 0, c->stream, m.N,
-                                   L, in, out, s);
-            else
-                hipLaunchKernelGGL(step_fp64_kernel<3>, dim3(grid), dim3(256), 0, c->stream, m.N,
-                                   L, in, out, s);
+            const bool ham = c->p.eom == AXS_EOM_HAMILTONIAN;
+            auto k = (c->nd == 2) ? (ham ? step_fp64_kernel<2, true> : step_fp64_kernel<2, false>)
+                                  : (ham ? step_fp64_kernel<3, true> : step_fp64_kernel<3, false>);
+            hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, c->stream, m.N, L, in, out, s);
         } else {
             StepScalarsF f = to_float(s);
             const fvec4 *in = (const fvec4 *)m.state[src][PHI];

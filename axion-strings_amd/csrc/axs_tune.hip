@@ -31,7 +31,8 @@ std::string tune_key(const Context *c)
     std::string dev = "gfx";
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) dev = prop.gcnArchName;
     std::string k = dev + " nd" + std::to_string(c->nd) + (c->prec == AXS_FP32 ? " fp32" : " fp64") +
-                    " st" + std::to_string(c->p.stencil_setting) + " N";
+                    " st" + std::to_string(c->p.stencil_setting) + " eom" +
+                    std::to_string(c->p.eom) + " lap" + std::to_string(c->p.laplacian) + " N";
     std::vector<int> Ns;
     for (const Member &m : c->members) Ns.push_back(m.N);
     std::sort(Ns.begin(), Ns.end());

@@ -65,6 +65,8 @@ std::vector<Tag> tag_table(parameters *p)
         {"device", Type::INT, &p->device, 0, false, false},
         {"checkpoint_path", Type::STRING, p->checkpoint_path, AXS_MAX_LEN, false, false},
         {"restart_path", Type::STRING, p->restart_path, AXS_MAX_LEN, false, false},
+        {"eom", Type::STRING, p->eom, sizeof p->eom, false, false},
+        {"laplacian", Type::STRING, p->laplacian, sizeof p->laplacian, false, false},
     };
 }
 
@@ -83,6 +85,8 @@ void parameters_set_defaults(parameters *p)
     p->device = 0;
     p->checkpoint_path[0] = 0;
     p->restart_path[0] = 0;
+    snprintf(p->eom, sizeof p->eom, "field_rescaled");
+    snprintf(p->laplacian, sizeof p->laplacian, "stencil");
 }
 
 int read_parameter_file(const char *fname, parameters *p)

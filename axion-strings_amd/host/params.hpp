@@ -59,6 +59,8 @@ struct parameters {
     int device;             // HIP device ordinal
     char checkpoint_path[AXS_MAX_LEN];  // "" = none; full state written at the end
     char restart_path[AXS_MAX_LEN];     // "" = none; start from this checkpoint
+    char eom[24];           // "field_rescaled" (s_run.cpp:215) or "hamiltonian" (:216)
+    char laplacian[16];     // "stencil" (s_integrate.cpp:322) or "spectral" (:326-328)
 };
 
 // Gadget2-style reader with the reference's rules (read_parameters.cpp:

@@ -260,6 +260,17 @@ int run_standard(const parameters &p, Out &o)
         fprintf(stderr, "axs: init_mode must be thermal or kinv\n");
         return 1;
     }
+    if (strcmp(p.eom, "field_rescaled") != 0 && strcmp(p.eom, "hamiltonian") != 0) {
+        fprintf(stderr, "axs: eom must be field_rescaled or hamiltonian\n");
+        return 1;
+    }
+    ap.eom = (strcmp(p.eom, "hamiltonian") == 0) ? AXS_EOM_HAMILTONIAN : AXS_EOM_FIELD_RESCALED;
+    if (strcmp(p.laplacian, "stencil") != 0 && strcmp(p.laplacian, "spectral") != 0) {
+        fprintf(stderr, "axs: laplacian must be stencil or spectral\n");
+        return 1;
+    }
+    ap.laplacian =
+        (strcmp(p.laplacian, "spectral") == 0) ? AXS_LAPLACIAN_SPECTRAL : AXS_LAPLACIAN_STENCIL;
     const bool log_bins = strcmp(p.pk_binning, "log") == 0;
 
     axs_ctx *ctx = nullptr;
@@ -503,6 +514,8 @@ extern "C" int axs_host_params_json(const char *path, char *buf, int cap)
     json_num(s, "device", p.device);
     json_str(s, "checkpoint_path", p.checkpoint_path);
     json_str(s, "restart_path", p.restart_path);
+    json_str(s, "eom", p.eom);
+    json_str(s, "laplacian", p.laplacian);
     s.resize(s.size() - 2);
     s += "}";
     if ((int)s.size() + 1 > cap) return 2;

@@ -151,6 +151,32 @@ def ensemble_leg(np, axs, name, T0, steps_override=0):
     return out
 
 
+# the reference's alternative step paths (FP64): the Hamiltonian form on the
+# march kernel, the field-rescaled step with the spectral Laplacian (FFT-bound)
+FORMS = {
+    "hamiltonian_fp64": (dict(eom="hamiltonian"), 1024, 50),
+    "spectral_laplacian_fp64": (dict(laplacian="spectral"), 512, 10),
+}
+
+
+def forms_leg(np, axs, T0):
+    out = {}
+    for name, (kw, N, steps) in FORMS.items():
+        lat = axs.Lattice(3, N, precision="fp64", T_initial=T0, **kw)
+        lat.init("kinv", seed=1998)
+        lat.tau = 1.0
+        lat.step(2)
+        t0 = time.perf_counter()
+        lat.step(steps)
+        wall = time.perf_counter() - t0
+        out[name] = {"N": N, "steps": steps, "ms_per_step": 1e3 * wall / steps,
+                     "site_updates_per_s": N ** 3 * steps / wall,
+                     "kernel_ms": lat.last_step_ms, "config": lat.kernel_config()}
+        lat.close()
+        log(f"[bench] {name}: {N}^3, {1e3 * wall / steps:.3f} ms/step")
+    return out
+
+
 def cpu_baseline(np, seconds: float):
     """The oracle's reference-structured FP64 step on a 256^3 sub-sample of
     the workload (same per-site work), all host threads."""
@@ -218,6 +244,8 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (for calibrating HBM "
                          "traffic counters) and exit")
+    ap.add_argument("--skip-forms", action="store_true",
+                    help="skip the Hamiltonian-form and spectral-Laplacian legs")
     ap.add_argument("--skip-ensembles", action="store_true",
                     help="skip the 2D (512-member) and 3D (48-member) ensemble legs")
     ap.add_argument("--ensemble-steps", type=int, default=0,
@@ -288,6 +316,13 @@ def main() -> int:
             except Exception as e:          # reported, the headline line still prints
                 ens[name] = {"error": str(e)}
 
+    forms_out = None
+    if not args.skip_forms:
+        try:
+            forms_out = forms_leg(np, axs, args.T_initial)
+        except Exception as e:              # reported, the headline line still prints
+            forms_out = {"error": str(e)}
+
     cpu = None
     if not args.skip_cpu:
         log("[bench] CPU baseline (oracle)")
@@ -330,6 +365,7 @@ def main() -> int:
         "fp32_speedup_over_fp64": (r32["site_updates_per_s"] / out64["value"]) if out64 else None,
         "accuracy_fp32_vs_fp64": acc,
         "ensembles": ens,
+        "forms": forms_out,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -61,7 +61,21 @@ typedef struct axs_params {
     int   thr;              /* string threshold, percent            (common.hpp:62) */
     int   precision;        /* AXS_FP64 or AXS_FP32                                 */
     int   device;           /* HIP device ordinal                                   */
+    int   eom;              /* AXS_EOM_* (0: the reference's default form)          */
+    int   laplacian;        /* AXS_LAPLACIAN_* (0: the reference's default)         */
 } axs_params;
+
+/* Equation-of-motion form of the step.  The Hamiltonian form and the
+ * spectral Laplacian are the reference's alternative code paths (selected
+ * there by editing s_run.cpp:215-216 / s_integrate.cpp:321-329); both need
+ * AXS_FP64, and the reference has no Hamiltonian step with the spectral
+ * Laplacian.                                                             */
+#define AXS_EOM_FIELD_RESCALED 0   /* vvsl_field_rescaled, s_integrate.cpp:305-348  */
+#define AXS_EOM_HAMILTONIAN    1   /* vvsl_hamiltonian_form, s_integrate.cpp:358-395;
+                                      the velocity arrays hold pi = phidot * a^2   */
+#define AXS_LAPLACIAN_STENCIL  0   /* sparse stencil matrix, s_integrate.cpp:322-324 */
+#define AXS_LAPLACIAN_SPECTRAL 1   /* compute_laplacian_fft, s_integrate.cpp:179-296,
+                                      used at :326-328                             */
 
 typedef struct axs_ctx axs_ctx;
 
@@ -225,6 +239,14 @@ int axs_last_step_ms(const axs_ctx *ctx, double *ms_per_step);
  * once for `depth` updates) and the path's algorithmic bytes per
  * site-update (read+write of phi and its velocity: FP64 64, FP32 32).     */
 int axs_bytes_per_site(const axs_ctx *ctx, double *scheme_bytes, double *alg_bytes);
+
+/* The Laplacian of one member's current field, per component, into host
+ * arrays of length N^ndims (either may be NULL): mode AXS_LAPLACIAN_STENCIL
+ * gives (1/dx^2) * (stencil matrix) * phi as s_integrate.cpp:322-324 builds
+ * it, AXS_LAPLACIAN_SPECTRAL gives compute_laplacian_fft
+ * (s_integrate.cpp:179-296) -- the reference's two operators side by side
+ * for validation.                                                         */
+int axs_member_laplacian(axs_ctx *ctx, int member, int mode, double *lap1, double *lap2);
 
 const char *axs_strerror(int status);
 

@@ -281,6 +281,176 @@ void orc_vvsl_run(const orc_params *p, const orc_matrix *A, int64_t nsteps,
 }
 
 /* ---------------------------------------------------------------------- */
+/* compute_laplacian_fft, s_integrate.cpp:179-296: the field times
+ * (-1)^(i+j[+k]) (shift2D/3D, :153-177), forward complex FFT, each mode
+ * times -k^2 with k[i] = (-N/2 + i) 2 pi / L on every axis (:192, :242),
+ * backward FFT, the shift again, real part divided by the length.        */
+
+int orc_laplacian_fft(int ndims, int N, float dx, const double *field, double *ret)
+{
+    if (ndims != 2 && ndims != 3) return -1;
+    int64_t length = orc_length(ndims, N);
+    float L = N * dx;                                              /* :183 */
+    double *k = (double *)malloc((size_t)N * sizeof(double));
+    double *data = (double *)malloc((size_t)length * 2 * sizeof(double));
+    if (!k || !data) {
+        free(k);
+        free(data);
+        return -1;
+    }
+    /* (- N / 2.0f + i) * 2.0f * M_PI / L: float, float, then double */
+    for (int i = 0; i < N; i++) k[i] = (double)((-N / 2.0f + i) * 2.0f) * M_PI / (double)L;
+    #pragma omp parallel for schedule(static)
+    for (int64_t m = 0; m < length; m++) {                         /* :205-208 */
+        int64_t c0 = m % N, c1 = (m / N) % N, c2 = (ndims == 3) ? m / ((int64_t)N * N) : 0;
+        double sgn = ((c0 + c1 + c2) & 1) ? -1.0 : 1.0;            /* pow(-1.0f, i+j+k) */
+        data[2 * m] = field[m] * sgn;
+        data[2 * m + 1] = 0.0 * sgn;
+    }
+    int rc = orc_fft(ndims, N, data, -1);                          /* FFTW_FORWARD */
+    #pragma omp parallel for schedule(static)
+    for (int64_t m = 0; m < length; m++) {                         /* :214-220 */
+        double k_sq;
+        if (ndims == 2) {
+            int64_t i = m % N, j = m / N;                          /* coordinate2 */
+            k_sq = k[i] * k[i] + k[j] * k[j];
+        } else {
+            int64_t i = m / ((int64_t)N * N), j = (m / N) % N, kk = m % N;   /* coordinate3 */
+            k_sq = k[i] * k[i] + k[j] * k[j] + k[kk] * k[kk];
+        }
+        data[2 * m] *= -k_sq;
+        data[2 * m + 1] *= -k_sq;
+    }
+    if (rc == 0) rc = orc_fft(ndims, N, data, +1);                 /* FFTW_BACKWARD */
+    #pragma omp parallel for schedule(static)
+    for (int64_t m = 0; m < length; m++) {                         /* :226-232 */
+        int64_t c0 = m % N, c1 = (m / N) % N, c2 = (ndims == 3) ? m / ((int64_t)N * N) : 0;
+        double sgn = ((c0 + c1 + c2) & 1) ? -1.0 : 1.0;
+        ret[m] = (data[2 * m] * sgn) / (double)length;
+    }
+    free(k);
+    free(data);
+    return rc;
+}
+
+/* ---------------------------------------------------------------------- */
+/* vvsl_hamiltonian_form, s_integrate.cpp:358-395: tau advances first, the
+ * drift uses dt / tau^2 and pi = phidot * a^2, the Laplacian carries
+ * alpha = tau^2 / dx^2 and the potential -tau^2 lambda phi (|phi|^2 - 1 +
+ * T0^2 / (3 tau^2)).                                                      */
+
+void orc_vvsl_hamiltonian_step(const orc_params *p, const orc_matrix *A,
+                               double *phi1, double *phi2, double *pi1, double *pi2,
+                               double *ker1_curr, double *ker2_curr,
+                               double *ker1_next, double *ker2_next, double *tau)
+{
+    int64_t length = A->L;
+    float dt = p->time_step;                                       /* :361 */
+    float dx = p->space_step;                                      /* :362 */
+    double ddt = (double)dt;
+
+    *tau = *tau + ddt;                                             /* :364 */
+    double t = *tau;
+    double t2 = t * t;
+
+    /* :368-369 -- phi += dt / pow_2(tau) * (pi + 0.5f * dt * ker) */
+    double cd = ddt / t2;
+    double hdt = (double)(0.5f * dt);
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < length; i++) {
+        phi1[i] = phi1[i] + cd * (pi1[i] + hdt * ker1_curr[i]);
+        phi2[i] = phi2[i] + cd * (pi2[i] + hdt * ker2_curr[i]);
+    }
+
+    /* :374-376 -- alpha = pow_2(tau) / pow_2(dx) */
+    double alpha = t2 / ((double)dx * (double)dx);
+    orc_spmv(A, alpha, phi1, ker1_next);
+    orc_spmv(A, alpha, phi2, ker2_next);
+
+    /* :379-380 -- ker += -pow_2(tau) * lambda * phi *
+     *            (phi1^2 + phi2^2 - 1.0f + pow_2(T_initial) / (3.0f * pow_2(tau))) */
+    double ch = (-t2) * (double)p->lambda;
+    double T0 = (double)p->T_initial;
+    double X = (T0 * T0) / ((double)3.0f * t2);
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < length; i++) {
+        double s = phi1[i] * phi1[i] + phi2[i] * phi2[i] - 1.0 + X;
+        ker1_next[i] = ker1_next[i] + ch * phi1[i] * s;
+        ker2_next[i] = ker2_next[i] + ch * phi2[i] * s;
+    }
+
+    /* :384-385, :389-392 -- kick and copy as in the field-rescaled form */
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < length; i++) {
+        pi1[i] = pi1[i] + 0.5 * (ker1_curr[i] + ker1_next[i]) * ddt;
+        pi2[i] = pi2[i] + 0.5 * (ker2_curr[i] + ker2_next[i]) * ddt;
+        ker1_curr[i] = ker1_next[i];
+        ker2_curr[i] = ker2_next[i];
+    }
+}
+
+/* vvsl_field_rescaled with compute_laplacian_fft in place of the sparse
+ * matrix (the #else branch, s_integrate.cpp:326-328); everything else as
+ * orc_vvsl_step.                                                          */
+int orc_vvsl_spectral_step(const orc_params *p, int64_t length,
+                           double *phi1, double *phi2, double *phidot1, double *phidot2,
+                           double *ker1_curr, double *ker2_curr,
+                           double *ker1_next, double *ker2_next, double *tau)
+{
+    float dt = p->time_step;
+    double ddt = (double)dt;
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < length; i++) {
+        phi1[i] = phi1[i] + ddt * (phidot1[i] + 0.5 * ker1_curr[i] * ddt);
+        phi2[i] = phi2[i] + ddt * (phidot2[i] + 0.5 * ker2_curr[i] * ddt);
+    }
+    *tau = *tau + ddt;
+    double t = *tau;
+    int rc = orc_laplacian_fft(p->ndims, p->N, p->space_step, phi1, ker1_next);
+    if (rc == 0) rc = orc_laplacian_fft(p->ndims, p->N, p->space_step, phi2, ker2_next);
+    if (rc != 0) return rc;
+    double lam = (double)p->lambda;
+    double T0 = (double)p->T_initial;
+    double c = (double)(-1.0f) / (t * t);
+    double tau2 = t * t;
+    double T03 = (T0 * T0) / (double)3.0f;
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < length; i++) {
+        double s = phi1[i] * phi1[i] + phi2[i] * phi2[i] - tau2 + T03;
+        ker1_next[i] = ker1_next[i] + c * lam * phi1[i] * s;
+        ker2_next[i] = ker2_next[i] + c * lam * phi2[i] * s;
+        phidot1[i] = phidot1[i] + 0.5 * (ker1_curr[i] + ker1_next[i]) * ddt;
+        phidot2[i] = phidot2[i] + 0.5 * (ker2_curr[i] + ker2_next[i]) * ddt;
+        ker1_curr[i] = ker1_next[i];
+        ker2_curr[i] = ker2_next[i];
+    }
+    return 0;
+}
+
+int orc_vvsl_run_form(const orc_params *p, const orc_matrix *A, int eom, int laplacian,
+                      int64_t nsteps, double *phi1, double *phi2, double *v1, double *v2,
+                      double *ker1_curr, double *ker2_curr, double *ker1_next,
+                      double *ker2_next, double *tau)
+{
+    int64_t length = orc_length(p->ndims, p->N);
+    if (eom == 1 && laplacian != 0) return -1;     /* the reference has no such pair */
+    for (int64_t s = 0; s < nsteps; s++) {
+        if (eom == 1) {
+            orc_vvsl_hamiltonian_step(p, A, phi1, phi2, v1, v2, ker1_curr, ker2_curr,
+                                      ker1_next, ker2_next, tau);
+        } else if (laplacian == 1) {
+            int rc = orc_vvsl_spectral_step(p, length, phi1, phi2, v1, v2, ker1_curr,
+                                            ker2_curr, ker1_next, ker2_next, tau);
+            if (rc) return rc;
+        } else {
+            orc_vvsl_step(p, A, phi1, phi2, v1, v2, ker1_curr, ker2_curr, ker1_next,
+                          ker2_next, tau);
+        }
+    }
+    return 0;
+}
+
+/* ---------------------------------------------------------------------- */
 /* strings                                                                 */
 
 void orc_axion(int64_t L, const double *phi1, const double *phi2, double *axion)

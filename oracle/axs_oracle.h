@@ -94,6 +94,28 @@ void orc_energy(const orc_params *p, double tau,
  * sign = -1 forward (FFTW_FORWARD), +1 backward; unnormalised.          */
 int orc_fft(int ndims, int N, double *data, int sign);
 
+/* compute_laplacian_fft, s_integrate.cpp:179-296 (ret = spectral
+ * Laplacian of field, continuum -k^2 on every mode)                      */
+int orc_laplacian_fft(int ndims, int N, float dx, const double *field, double *ret);
+
+/* vvsl_hamiltonian_form, s_integrate.cpp:358-395 (v = pi = phidot a^2)   */
+void orc_vvsl_hamiltonian_step(const orc_params *p, const orc_matrix *A,
+                               double *phi1, double *phi2, double *pi1, double *pi2,
+                               double *ker1_curr, double *ker2_curr,
+                               double *ker1_next, double *ker2_next, double *tau);
+/* vvsl_field_rescaled with compute_laplacian_fft (s_integrate.cpp:326-328) */
+int orc_vvsl_spectral_step(const orc_params *p, int64_t length,
+                           double *phi1, double *phi2, double *phidot1, double *phidot2,
+                           double *ker1_curr, double *ker2_curr,
+                           double *ker1_next, double *ker2_next, double *tau);
+/* nsteps of the form: eom 0 field-rescaled / 1 Hamiltonian, laplacian 0
+ * stencil matrix / 1 spectral (field-rescaled only); A may be NULL for the
+ * spectral form                                                          */
+int orc_vvsl_run_form(const orc_params *p, const orc_matrix *A, int eom, int laplacian,
+                      int64_t nsteps, double *phi1, double *phi2, double *v1, double *v2,
+                      double *ker1_curr, double *ker2_curr, double *ker1_next,
+                      double *ker2_next, double *tau);
+
 /* output_powerspec, src/common/pk_estimation.cpp:37-214.
  * data_imag may be NULL.  Outputs have N/2 entries.                     */
 void orc_powerspec(int ndims, int N, float dx, const double *data_real,

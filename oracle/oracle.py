@@ -56,6 +56,9 @@ def lib() -> ctypes.CDLL:
         "orc_background": ([c_int64] + [_DP] * 5, None),
         "orc_energy": ([POINTER(OrcParams), c_double] + [_DP] * 5, None),
         "orc_fft": ([c_int, c_int, _DP, c_int], c_int),
+        "orc_laplacian_fft": ([c_int, c_int, c_float, _DP, _DP], c_int),
+        "orc_vvsl_run_form": ([POINTER(OrcParams), c_void_p, c_int, c_int, c_int64] + [_DP] * 9,
+                              c_int),
         "orc_powerspec": ([c_int, c_int, c_float, _DP, _DP, _DP, _DP, _IP], None),
         "orc_powerspec_log": ([c_int, c_int, c_float, _DP, c_int, _DP, _DP, _IP], None),
     }
@@ -137,6 +140,30 @@ class State:
                            _dp(self.ker2), _dp(self._k1n), _dp(self._k2n), ctypes.byref(t))
         self.tau = t.value
         return self
+
+    def run_form(self, p: OrcParams, A, nsteps: int, eom: int = 0, laplacian: int = 0) -> "State":
+        """eom 0: vvsl_field_rescaled, 1: vvsl_hamiltonian_form (phidot
+        holds pi); laplacian 1: compute_laplacian_fft in the field-rescaled
+        step (A unused, may be None)."""
+        t = c_double(self.tau)
+        rc = lib().orc_vvsl_run_form(ctypes.byref(p), A.h if A is not None else None, eom,
+                                     laplacian, int(nsteps), _dp(self.phi1), _dp(self.phi2),
+                                     _dp(self.phidot1), _dp(self.phidot2), _dp(self.ker1),
+                                     _dp(self.ker2), _dp(self._k1n), _dp(self._k2n),
+                                     ctypes.byref(t))
+        if rc != 0:
+            raise ValueError(f"orc_vvsl_run_form failed ({rc})")
+        self.tau = t.value
+        return self
+
+
+def laplacian_fft(field, ndims: int, N: int, dx: float = 0.5) -> np.ndarray:
+    """compute_laplacian_fft (s_integrate.cpp:179-296)."""
+    f = _f64(field)
+    out = np.empty_like(f)
+    if lib().orc_laplacian_fft(ndims, N, dx, _dp(f), _dp(out)) != 0:
+        raise ValueError("orc_laplacian_fft failed")
+    return out
 
 
 def axion(phi1, phi2) -> np.ndarray:

@@ -63,9 +63,15 @@ def _run_driver(tmp_path, nd, N, steps, init="thermal", T0=4.0, extra=""):
     return out
 
 
-@pytest.mark.parametrize("nd,N,steps,init", [(2, 128, 120, "thermal"), (3, 32, 60, "kinv")])
-def test_driver_outputs_match_oracle(tmp_path, nd, N, steps, init):
-    out = _run_driver(tmp_path, nd, N, steps, init=init)
+@pytest.mark.parametrize("nd,N,steps,init,eom,lap", [
+    (2, 128, 120, "thermal", "field_rescaled", "stencil"), (3, 32, 60, "kinv", "field_rescaled", "stencil"),
+    (2, 64, 50, "thermal", "hamiltonian", "stencil"), (2, 64, 40, "kinv", "field_rescaled", "spectral")])
+def test_driver_outputs_match_oracle(tmp_path, nd, N, steps, init, eom, lap):
+    # the Hamiltonian form and the spectral Laplacian are the reference's
+    # alternative step paths, selected here by the optional tags
+    out = _run_driver(tmp_path, nd, N, steps, init=init, extra=f"eom {eom}\nlaplacian {lap}")
+    spectral = lap == "spectral"
+    eom_i, lap_i = int(eom == "hamiltonian"), int(spectral)
     L = N ** nd
     p1, p2, v1, v2 = axs.host_init(init, nd, N, space_step=0.5, lam=1.0, T_initial=4.0, seed=1998)
     st = O.State(p1, p2, v1, v2)
@@ -87,7 +93,7 @@ def test_driver_outputs_match_oracle(tmp_path, nd, N, steps, init):
             ks, pk, cnt = O.powerspec(st.phi2, nd, N, 0.5)
             n, c = O.cores(O.axion(st.phi1, st.phi2), nd, N)
             snaps.append((st.tau, st.phi1.copy(), st.phi2.copy(), c, ks, pk, cnt))
-        st.run(prm, A, 1)
+        st.run_form(prm, A, 1, eom=eom_i, laplacian=lap_i)
 
     # time series: header and values (printed with %f)
     lines = open(out / "time-series.csv").read().splitlines()
@@ -110,8 +116,12 @@ def test_driver_outputs_match_oracle(tmp_path, nd, N, steps, init):
         raw1 = np.fromfile(out / f"snapshot{s}-phi1", dtype=np.float64)
         raw2 = np.fromfile(out / f"snapshot{s}-phi2", dtype=np.float64)
         nsave = L if nd == 2 else N * N                    # 3D: first N^2 (fileio.cpp:31-46)
-        np.testing.assert_array_equal(raw1, f1[:nsave])
-        np.testing.assert_array_equal(raw2, f2[:nsave])
+        if spectral:        # FFT rounding: rocFFT against the oracle's transform
+            assert np.abs(raw1 - f1[:nsave]).max() < 1e-11 * np.abs(f1).max()
+            assert np.abs(raw2 - f2[:nsave]).max() < 1e-11 * np.abs(f2).max()
+        else:
+            np.testing.assert_array_equal(raw1, f1[:nsave])
+            np.testing.assert_array_equal(raw2, f2[:nsave])
         text = open(out / f"snapshot{s}-strings.csv").read()
         want = "".join(", ".join(str(int(v)) for v in row) + "\n" for row in c)
         assert text == want
