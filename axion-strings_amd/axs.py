@@ -36,6 +36,9 @@ AXS_BIN_LINEAR = 0
 AXS_BIN_LOG = 1
 AXS_INIT_KINV = 0
 AXS_INIT_THERMAL = 1
+AXS_SNAP_PHI = 1
+AXS_SNAP_VEL = 2
+AXS_SNAP_KER = 4
 AXS_EOM_FIELD_RESCALED = 0
 AXS_EOM_HAMILTONIAN = 1
 AXS_LAPLACIAN_STENCIL = 0
@@ -53,6 +56,7 @@ EXPORTS = (
     "axs_member_energy", "axs_energy", "axs_member_powerspec", "axs_powerspec",
     "axs_last_step_ms", "axs_bytes_per_site", "axs_strerror", "axs_last_error",
     "axs_set_kernel_config", "axs_get_kernel_config", "axs_autotune", "axs_member_laplacian",
+    "axs_snapshot_begin", "axs_snapshot_end",
 )
 
 AXS_KERNEL_AUTO = 0
@@ -141,6 +145,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "axs_last_step_ms": ([c_void_p, _DP], c_int),
         "axs_bytes_per_site": ([c_void_p, _DP, _DP], c_int),
         "axs_member_laplacian": ([c_void_p, c_int, c_int, _DP, _DP], c_int),
+        "axs_snapshot_begin": ([c_void_p, c_int, c_int, c_int64], c_int),
+        "axs_snapshot_end": ([c_void_p] + [_DP] * 7, c_int),
         "axs_set_kernel_config": ([c_void_p, POINTER(AxsKernelConfig)], c_int),
         "axs_get_kernel_config": ([c_void_p, POINTER(AxsKernelConfig)], c_int),
         "axs_autotune": ([c_void_p, c_int, POINTER(AxsKernelConfig), _DP], c_int),
@@ -338,6 +344,28 @@ class Lattice:
         _check(self.lib.axs_member_background(self._ctx, member, _dp(out)),
                "axs_member_background")
         return out
+
+    def snapshot_begin(self, member: int = 0, fields=("phi1", "phi2", "phidot1", "phidot2",
+                                                      "ker1", "ker2"), count: int = 0):
+        """start an asynchronous copy of the current state (see
+        axs_snapshot_begin); finish it with snapshot_end()"""
+        bits = ((AXS_SNAP_PHI if {"phi1", "phi2"} & set(fields) else 0) |
+                (AXS_SNAP_VEL if {"phidot1", "phidot2"} & set(fields) else 0) |
+                (AXS_SNAP_KER if {"ker1", "ker2"} & set(fields) else 0))
+        _check(self.lib.axs_snapshot_begin(self._ctx, member, bits, count), "axs_snapshot_begin")
+        self._snap = (member, tuple(fields), count or self.Ns[member] ** self.ndims)
+
+    def snapshot_end(self):
+        """(dict of the requested arrays, tau) of the snapshot in flight"""
+        # without a snapshot begun, the library reports the error
+        member, fields, n = getattr(self, "_snap", None) or (0, (), 0)
+        names = ("phi1", "phi2", "phidot1", "phidot2", "ker1", "ker2")
+        arrs = {k: np.empty(n) for k in fields}
+        tau = c_double()
+        _check(self.lib.axs_snapshot_end(self._ctx, *[_dp(arrs.get(k)) for k in names],
+                                         ctypes.byref(tau)), "axs_snapshot_end")
+        self._snap = None
+        return arrs, tau.value
 
     def laplacian(self, mode: str = "stencil", member: int = 0):
         """(lap1, lap2) of the current field: the stencil operator

@@ -234,6 +234,7 @@ void destroy_inverse_plans(Context *c);
 Context::~Context()
 {
     if (stream) (void)hipStreamSynchronize(stream);
+    destroy_snapshot(this);
     destroy_plans(this);
     destroy_inverse_plans(this);
     destroy_spec_plans(this);
@@ -862,6 +863,33 @@ int axs_member_laplacian(axs_ctx *ctx, int member, int mode, double *lap1, doubl
         return AXS_ERR_INVALID_ARG;
     }
     return run_laplacian(&ctx->impl, ctx->impl.members[member], mode, lap1, lap2);
+}
+
+int axs_snapshot_begin(axs_ctx *ctx, int member, int fields, int64_t count)
+{
+    AXS_TRY(check_member(ctx, member));
+    Member &m = ctx->impl.members[member];
+    if (fields <= 0 || (fields & ~(AXS_SNAP_PHI | AXS_SNAP_VEL | AXS_SNAP_KER)) != 0) {
+        set_error("fields must be a non-empty set of AXS_SNAP_* bits (got %d)", fields);
+        return AXS_ERR_INVALID_ARG;
+    }
+    if (count < 0 || count > m.L) {
+        set_error("count %lld outside [0, %lld]", (long long)count, (long long)m.L);
+        return AXS_ERR_INVALID_ARG;
+    }
+    return snapshot_begin(&ctx->impl, m, fields, count ? count : m.L);
+}
+
+int axs_snapshot_end(axs_ctx *ctx, double *phi1, double *phi2, double *phidot1,
+                     double *phidot2, double *ker1, double *ker2, double *tau)
+{
+    if (!ctx) {
+        set_error("null context");
+        return AXS_ERR_INVALID_ARG;
+    }
+    AXS_HIP_TRY(hipSetDevice(ctx->impl.device));      // may be another host thread
+    double *out[6] = {phi1, phi2, phidot1, phidot2, ker1, ker2};
+    return snapshot_end(&ctx->impl, out, tau);
 }
 
 int axs_bytes_per_site(const axs_ctx *ctx, double *scheme_bytes, double *alg_bytes)

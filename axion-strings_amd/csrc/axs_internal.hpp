@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -146,6 +147,19 @@ struct Context {
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
 
+    // asynchronous snapshot (axs_snapshot.hip): device staging copy made on
+    // `stream`, device-to-host copy on `copy_stream`
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t snap_staged = nullptr, snap_done = nullptr;
+    void *snap_dev = nullptr;
+    size_t snap_dev_bytes = 0;
+    void *snap_host = nullptr;
+    size_t snap_host_bytes = 0;
+    std::atomic<int> snap_pending{0};
+    int snap_fields = 0;
+    int64_t snap_count = 0;
+    double snap_tau = 0.0;
+
     std::vector<FftPlan *> fft_plans;
     std::vector<void *> inv_plans;   // device-init inverse transforms (axs_init.hip)
     std::vector<void *> spec_plans;  // spectral-Laplacian transforms (axs_spectral.hip)
@@ -196,6 +210,9 @@ int run_spectral_step(Context *c, Member &m, int src, int dst, const StepScalars
 // Laplacian of the current field (AXS_LAPLACIAN_*) into host arrays
 int run_laplacian(Context *c, Member &m, int mode, double *h1, double *h2);
 void destroy_spec_plans(Context *c);
+void destroy_snapshot(Context *c);
+int snapshot_begin(Context *c, Member &m, int fields, int64_t count);
+int snapshot_end(Context *c, double *out[6], double *tau);
 // march geometry chosen for a member (rows per work-item, planes per chunk)
 int march_rows(const Context *c);
 int march_chunk(const Context *c, int N);

@@ -15,8 +15,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <atomic>
 #include <chrono>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "axs_gpu.h"
@@ -90,16 +92,38 @@ struct Out {
 
 std::string path_in(const Out &o, const char *name) { return o.dir + "/" + name; }
 
-bool save_raw(Out &o, const char *name, const double *data, size_t count)
+bool write_raw_file(const std::string &path, const double *data, size_t count)
 {
-    std::string path = path_in(o, name);
-    fprintf(o.main, "  Saving data... at %s\n", path.c_str());
     FILE *fp = fopen(path.c_str(), "w");
     if (!fp) return false;
     size_t w = fwrite(data, sizeof(double), count, fp);
-    fclose(fp);
-    return w == count;
+    return fclose(fp) == 0 && w == count;
 }
+
+// Field snapshots leave the device asynchronously (axs_snapshot_begin) and
+// are written by this thread while the main thread keeps stepping; at most
+// one is in flight.  Joined before the next snapshot and before the
+// context goes away.
+struct SnapshotWriter {
+    std::thread t;
+    std::atomic<bool> failed{false};
+    bool join()
+    {
+        if (t.joinable()) t.join();
+        return !failed.load();
+    }
+    void start(axs_ctx *ctx, size_t count, std::string path1, std::string path2)
+    {
+        t = std::thread([this, ctx, count, path1, path2]() {
+            std::vector<double> f1(count), f2(count);
+            if (axs_snapshot_end(ctx, f1.data(), f2.data(), nullptr, nullptr, nullptr, nullptr,
+                                 nullptr) != AXS_OK ||
+                !write_raw_file(path1, f1.data(), count) || !write_raw_file(path2, f2.data(), count))
+                failed.store(true);
+        });
+    }
+    ~SnapshotWriter() { join(); }
+};
 
 bool save_strings(Out &o, const char *name, const int32_t *c, int64_t n, int nd)
 {
@@ -279,6 +303,7 @@ int run_standard(const parameters &p, Out &o)
         axs_ctx *c;
         ~Guard() { axs_destroy(c); }
     } guard{ctx};
+    SnapshotWriter writer;      // destroyed (joined) before the context
 
     // initial state: checkpoint, device generator or the host generator
     int first_step = 0;
@@ -386,15 +411,20 @@ int run_standard(const parameters &p, Out &o)
             fprintf(o.snap, "\n");
             char name[64];
             if (p.save_fields) {
-                std::vector<double> f1((size_t)length), f2((size_t)length);
-                CHECK(axs_get_state(ctx, f1.data(), f2.data(), nullptr, nullptr, nullptr, nullptr,
-                                    nullptr));
+                if (!writer.join()) {
+                    fprintf(stderr, "axs: writing a field snapshot failed\n");
+                    return 1;
+                }
                 // 3D: the reference writes only the first N^2 values (fileio.cpp:31-46)
                 size_t cnt = (nd == 2 || p.save_full_fields) ? (size_t)length : (size_t)N * N;
+                CHECK(axs_snapshot_begin(ctx, 0, AXS_SNAP_PHI, (int64_t)cnt));
                 snprintf(name, sizeof name, "snapshot%d-phi1", n_snapshots_written);
-                if (!save_raw(o, name, f1.data(), cnt)) return 1;
+                std::string path1 = path_in(o, name);
                 snprintf(name, sizeof name, "snapshot%d-phi2", n_snapshots_written);
-                if (!save_raw(o, name, f2.data(), cnt)) return 1;
+                std::string path2 = path_in(o, name);
+                fprintf(o.main, "  Saving data... at %s\n", path1.c_str());
+                fprintf(o.main, "  Saving data... at %s\n", path2.c_str());
+                writer.start(ctx, cnt, path1, path2);
             }
             if (p.save_strings) {
                 int64_t n = 0;
@@ -426,6 +456,10 @@ int run_standard(const parameters &p, Out &o)
         CHECK(axs_get_tau(ctx, &ph.tau));
     }
 
+    if (!writer.join()) {
+        fprintf(stderr, "axs: writing a field snapshot failed\n");
+        return 1;
+    }
     if (p.checkpoint_path[0]) {
         std::vector<double> a[6];
         for (auto &v : a) v.assign((size_t)length, 0.0);

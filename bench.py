@@ -177,6 +177,39 @@ def forms_leg(np, axs, T0):
     return out
 
 
+def snapshot_leg(np, axs, N, T0, steps=200):
+    """Field snapshot of the 1024^3 FP32 run (phi1, phi2 of every site,
+    8.6 GB each as float64 on the host): the synchronous read-back against
+    the asynchronous one (device staging copy, transfer on a second stream,
+    unpack on a host thread) overlapped with `steps` steps."""
+    import threading
+    lat = axs.Lattice(3, N, precision="fp32", T_initial=T0)
+    lat.init("kinv", seed=1998)
+    lat.tau = 1.0
+    lat.step(8)
+    t0 = time.perf_counter()
+    lat.step(steps)
+    plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lat.get_state(fields=("phi1", "phi2"))
+    sync_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lat.snapshot_begin(fields=("phi1", "phi2"))
+    box = {}
+    th = threading.Thread(target=lambda: box.update(r=lat.snapshot_end()))
+    th.start()
+    lat.step(steps)
+    steps_done = time.perf_counter() - t0
+    th.join()
+    total = time.perf_counter() - t0
+    lat.close()
+    del box
+    return {"N": N, "fields": "phi1,phi2", "steps": steps,
+            "steps_alone_s": plain, "sync_read_back_s": sync_s,
+            "async_steps_s": steps_done, "async_total_s": total,
+            "hidden_fraction": max(0.0, min(1.0, (plain + sync_s - total) / sync_s))}
+
+
 def cpu_baseline(np, seconds: float):
     """The oracle's reference-structured FP64 step on a 256^3 sub-sample of
     the workload (same per-site work), all host threads."""
@@ -244,6 +277,8 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (for calibrating HBM "
                          "traffic counters) and exit")
+    ap.add_argument("--skip-snapshot", action="store_true",
+                    help="skip the asynchronous-snapshot overlap leg")
     ap.add_argument("--skip-forms", action="store_true",
                     help="skip the Hamiltonian-form and spectral-Laplacian legs")
     ap.add_argument("--skip-ensembles", action="store_true",
@@ -323,6 +358,15 @@ def main() -> int:
         except Exception as e:              # reported, the headline line still prints
             forms_out = {"error": str(e)}
 
+    snap = None
+    if not args.skip_snapshot:
+        try:
+            snap = snapshot_leg(np, axs, N, args.T_initial)
+            log(f"[bench] snapshot: sync {snap['sync_read_back_s']:.2f} s, async total "
+                f"{snap['async_total_s']:.2f} s for {snap['steps']} steps + snapshot")
+        except Exception as e:
+            snap = {"error": str(e)}
+
     cpu = None
     if not args.skip_cpu:
         log("[bench] CPU baseline (oracle)")
@@ -366,6 +410,7 @@ def main() -> int:
         "accuracy_fp32_vs_fp64": acc,
         "ensembles": ens,
         "forms": forms_out,
+        "snapshot": snap,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

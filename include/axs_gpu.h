@@ -248,6 +248,25 @@ int axs_bytes_per_site(const axs_ctx *ctx, double *scheme_bytes, double *alg_byt
  * for validation.                                                         */
 int axs_member_laplacian(axs_ctx *ctx, int member, int mode, double *lap1, double *lap2);
 
+/* ---- asynchronous snapshots --------------------------------------------- */
+
+/* The reference's snapshots copy the fields to disk synchronously
+ * (fileio.cpp:15-46, called at s_run.cpp:166-199).  axs_snapshot_begin
+ * copies the requested fields of the first `count` sites (0 = all) of one
+ * member's current state into device staging memory on the context's
+ * stream and queues the device-to-host transfer on a second stream, then
+ * returns: stepping can continue while the data travels.  axs_snapshot_end
+ * waits for the transfer and fills the host arrays (length count; any may
+ * be NULL) and *tau (the snapshot's tau); it may be called from another
+ * host thread while axs_step runs on the context.  One snapshot may be in
+ * flight per context (AXS_ERR_STATE otherwise).                          */
+#define AXS_SNAP_PHI 1   /* phi1, phi2                                     */
+#define AXS_SNAP_VEL 2   /* phidot1, phidot2 (pi in the Hamiltonian form)  */
+#define AXS_SNAP_KER 4   /* ker1, ker2                                     */
+int axs_snapshot_begin(axs_ctx *ctx, int member, int fields, int64_t count);
+int axs_snapshot_end(axs_ctx *ctx, double *phi1, double *phi2, double *phidot1,
+                     double *phidot2, double *ker1, double *ker2, double *tau);
+
 const char *axs_strerror(int status);
 
 /* Message of the last failure on this thread (empty if none). */
