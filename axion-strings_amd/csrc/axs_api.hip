@@ -407,15 +407,15 @@ static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ct
         set_error("stream synchronize failed after allocation");
         return fail(AXS_ERR_HIP);
     }
-    // FP32 3D standard stencil: several steps per pass with 16-wave
-    // work-groups unless configured otherwise; four steps from side 256 on,
-    // three below (the fastest shapes measured at 128^3, 512^3, 1024^3)
+    // FP32 standard stencil: several steps per pass unless configured
+    // otherwise (the fastest shapes measured): 3D four steps from side 256
+    // on, three below, 16-wave work-groups; 2D six steps
     int min_side = 1 << 30;
     for (const Member &m : c->members) min_side = std::min(min_side, m.N);
-    c->tune.depth = (min_side >= 256) ? 4 : 3;
+    c->tune.depth = (c->nd == 2) ? 6 : ((min_side >= 256) ? 4 : 3);
     if (multi_step_depth(c) == 1)
         c->tune.depth = 1;
-    else
+    else if (c->nd == 3)
         c->tune.flags |= kTuneWG1024;
     (void)rc;
     *out = h;
@@ -640,7 +640,7 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
         double tau_next = c->tau + dt;                    // s_integrate.cpp:317
         StepScalars s = make_scalars(c->p, tau_next);
         if (depth > 1 && nsteps - n >= depth) {
-            StepScalars ss[4];
+            StepScalars ss[kMaxDepth];
             double t = c->tau;
             for (int l = 0; l < depth; ++l) {
                 t = t + dt;                               // the same sequence of sums
@@ -777,7 +777,17 @@ int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
     Context *c = &ctx->impl;
     const int depth = cfg->depth ? cfg->depth : 1;
     bool rows_ok;
-    if (depth >= 2) {
+    if (depth >= 2 && c->nd == 2) {
+        // several steps per pass, 2D: FP32 5-point stencil, one strip per wave
+        rows_ok = cfg->rows == 0 && (cfg->flags & (AXS_MARCH_WG512 | AXS_MARCH_WG1024)) == 0;
+        bool fits = cfg->kernel != AXS_KERNEL_SIMPLE && march2d_depth_built(depth);
+        for (const Member &m : c->members) fits = fits && march2d_supported(c, m.N);
+        if (!fits) {
+            set_error("depth %d in 2D needs the FP32 form, stencil 0, sides >= 64, a "
+                      "non-simple kernel and depth 2, 3, 4, 6 or 8", depth);
+            return AXS_ERR_UNSUPPORTED;
+        }
+    } else if (depth >= 2) {
         // several steps per pass: FP32 3D standard stencil on every member
         const int W = multi_waves(cfg->flags);
         rows_ok = !((cfg->flags & AXS_MARCH_WG512) && (cfg->flags & AXS_MARCH_WG1024)) &&
@@ -795,7 +805,7 @@ int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
                   (cfg->rows == 2 && c->prec == AXS_FP64);
     }
     if (cfg->kernel < AXS_KERNEL_AUTO || cfg->kernel > AXS_KERNEL_MARCH || !rows_ok ||
-        cfg->chunk < 0 || depth < 1 || depth > 4 || (cfg->flags & ~63) != 0) {
+        cfg->chunk < 0 || depth < 1 || depth > (c->nd == 2 ? 8 : 4) || (cfg->flags & ~63) != 0) {
         set_error("unsupported kernel configuration (kernel %d rows %d chunk %d depth %d "
                   "flags %d)", cfg->kernel, cfg->rows, cfg->chunk, cfg->depth, cfg->flags);
         return AXS_ERR_UNSUPPORTED;

@@ -69,6 +69,15 @@ int stencil_coefficients(int stencil, int ndims, double *c);
 StepScalars make_scalars(const axs_params &p, double tau_after);
 StepScalarsF to_float(const StepScalars &s);
 
+// most time steps one multi-step launch takes (2D strips: 8, 3D: 4)
+constexpr int kMaxDepth = 8;
+
+// scalars of the D steps of one multi-step launch (tau+dt, tau+2dt, ...)
+template <int D>
+struct TBScalars {
+    StepScalarsF s[D];
+};
+
 // ---------------------------------------------------------------------------
 // one lattice of the context
 
@@ -205,6 +214,11 @@ int multi_waves(int flags);
 int launch_multi_steps(Context *c, const StepScalars *s, int depth);
 // work-item tables of the multi-step kernel for an ensemble
 int prepare_multi_batch(Context *c);
+// 2D multi-step kernel (axs_march2d.hip)
+bool march2d_supported(const Context *c, int N);
+bool march2d_depth_built(int D);
+int prepare_march2d_items(Context *c);
+int launch_march2d(Context *c, int src, const StepScalars *s, int depth);
 // field-rescaled step with the spectral Laplacian, one member
 int run_spectral_step(Context *c, Member &m, int src, int dst, const StepScalars &s);
 // Laplacian of the current field (AXS_LAPLACIAN_*) into host arrays

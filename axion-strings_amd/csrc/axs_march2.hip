@@ -33,11 +33,6 @@ struct TBGeom {
     int N, CI, nTj, nTk, remap;
 };
 
-template <int D>
-struct TBScalars {
-    StepScalarsF s[D];     // steps to tau+dt, tau+2dt, ...
-};
-
 template <int D, int W, int R>
 using TBLds = float[D][2][W * R * kTBW];   // x_l of the plane level l works on
 
@@ -396,12 +391,13 @@ int multi_step_depth(const Context *c)
 {
     if (c->tune.depth < 2 || c->tune.kernel == AXS_KERNEL_SIMPLE) return 1;
     for (const Member &m : c->members)
-        if (!march2_supported(c, m.N)) return 1;
+        if (!(c->nd == 2 ? march2d_supported(c, m.N) : march2_supported(c, m.N))) return 1;
     return c->tune.depth;
 }
 
 int prepare_multi_batch(Context *c)
 {
+    if (c->nd == 2) return prepare_march2d_items(c);
     if (c->members.size() < 2) return AXS_OK;
     const int D = multi_step_depth(c);
     if (D < 2) return AXS_OK;
@@ -444,7 +440,9 @@ int prepare_multi_batch(Context *c)
 int launch_multi_steps(Context *c, const StepScalars *s, int depth)
 {
     const int src = c->cur, dst = c->cur ^ 1;
-    if (depth == 4)
+    if (c->nd == 2)
+        AXS_TRY(launch_march2d(c, src, s, depth));
+    else if (depth == 4)
         AXS_TRY(launch_depth<4>(c, src, dst, s));
     else if (depth == 3)
         AXS_TRY(launch_depth<3>(c, src, dst, s));

@@ -174,7 +174,9 @@ batch_site_kernel(const BatchMember *__restrict__ mem, int n_members, StepScalar
 // shape changed.  Not called during graph capture (it copies to the device).
 int prepare_batch(Context *c)
 {
-    if (c->members.size() < 2) return AXS_OK;
+    // the 2D multi-step kernel runs from a work list even for one lattice
+    if (c->members.size() < 2)
+        return (c->nd == 2 && multi_step_depth(c) > 1) ? prepare_march2d_items(c) : AXS_OK;
     // the spectral step runs member by member (rocFFT per lattice)
     if (c->p.laplacian != AXS_LAPLACIAN_STENCIL) return AXS_OK;
     // the Hamiltonian form's ensembles use the site kernel

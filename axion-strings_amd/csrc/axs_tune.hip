@@ -89,9 +89,9 @@ int time_trial(Context *c, int trials, double *ms)
     const int cur0 = c->cur;
     const double dt = (double)c->p.time_step;
     const int depth = multi_step_depth(c);
-    StepScalars ss[4];
+    StepScalars ss[kMaxDepth];
     double t = c->tau;
-    for (int l = 0; l < 4; ++l) {
+    for (int l = 0; l < kMaxDepth; ++l) {
         t = t + dt;
         ss[l] = make_scalars(c->p, t);
     }
@@ -167,11 +167,32 @@ int run_autotune(Context *c, int trials, double *best_ms)
     int minN = 1 << 30;
     for (const Member &m : c->members) minN = std::min(minN, m.N);
     const bool march_ok = c->nd == 3 && c->p.stencil_setting == 0;
+    bool strips_ok = c->nd == 2 && c->prec == AXS_FP32;
+    for (const Member &m : c->members) strips_ok = strips_ok && march2d_supported(c, m.N);
     for (int graph = 0; graph < 2; ++graph) {
         Tune t;
         t.kernel = AXS_KERNEL_SIMPLE;
         t.flags = graph ? kTuneGraph : 0;
         cands.push_back(t);
+        if (strips_ok) {
+            // 2D: several steps per pass, one strip per wave
+            const int depths[5] = {2, 3, 4, 6, 8};
+            const int cj_opts[4] = {0, 32, 64, 128};
+            for (int d : depths)
+                for (int pf = 0; pf < 2; ++pf)
+                    for (int ci = 0; ci < 4; ++ci) {
+                        bool ok = true;
+                        for (const Member &m : c->members)
+                            ok = ok && (cj_opts[ci] == 0 || m.N % cj_opts[ci] == 0);
+                        if (!ok) continue;
+                        Tune u;
+                        u.kernel = AXS_KERNEL_MARCH;
+                        u.depth = d;
+                        u.ci32 = u.ci64 = cj_opts[ci];
+                        u.flags = (pf ? kTunePrefetch : 0) | kTuneNT | (graph ? kTuneGraph : 0);
+                        cands.push_back(u);
+                    }
+        }
         if (!march_ok) continue;
         const int rows_opts32[2] = {4, 8}, rows_opts64[2] = {2, 4};
         const int flag_opts[3] = {kTunePrefetch | kTuneNT, kTuneNT, kTunePrefetch};

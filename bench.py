@@ -107,25 +107,31 @@ def accuracy(np, lat32, lat64):
 
 
 ENSEMBLES = {
-    # 512 2D lattices: 128 each of 128^2, 256^2, 512^2, 1024^2; 2000 steps
-    "ensemble_2d": (2, [(128, 128), (256, 128), (512, 128), (1024, 128)], 2000),
-    # 48 3D lattices: 16 each of 128^3, 192^3, 256^3; 1500 steps
-    "ensemble_3d": (3, [(128, 16), (192, 16), (256, 16)], 1500),
+    # 512 2D lattices: 128 each of 128^2, 256^2, 512^2, 1024^2; 2000 steps,
+    # strings and spectra of every realization every 500 steps
+    "ensemble_2d": (2, [(128, 128), (256, 128), (512, 128), (1024, 128)], 2000, 500),
+    # 48 3D lattices: 16 each of 128^3, 192^3, 256^3; 1500 steps, sampled
+    # every 250 steps
+    "ensemble_3d": (3, [(128, 16), (192, 16), (256, 16)], 1500, 250),
 }
 
 
 def ensemble_leg(np, axs, name, T0, steps_override=0):
-    """One launch per step for every member (persistent work-list kernel for
-    3D, flat batched kernel for 2D), launch shape from the autotuner; the
-    FP32 form timed, then per-realization string counts against the FP64
-    form after the same steps."""
-    nd, groups, steps = ENSEMBLES[name]
+    """All members advance together (one persistent launch per pass of
+    several steps: the 3D and 2D multi-step kernels' work lists), launch
+    shape from the autotuner; in-loop sampling of every realization (string
+    count, 64-bin log power spectrum) at the stated cadence.  The FP32 form
+    is timed, then per-realization string counts and spectra are compared
+    with the FP64 form after the same steps."""
+    nd, groups, steps, every = ENSEMBLES[name]
     if steps_override:
         steps = steps_override
+        every = max(1, steps // 4)
     Ns = [N for N, cnt in groups for _ in range(cnt)]
     sites = sum(N ** nd for N in Ns)
-    out = {"members": len(Ns), "sides": [N for N, _ in groups], "sites": sites, "steps": steps}
-    counts = {}
+    out = {"members": len(Ns), "sides": [N for N, _ in groups], "sites": sites, "steps": steps,
+           "sample_every": every}
+    counts, spectra = {}, {}
     for prec in ("fp32", "fp64"):
         lat = axs.Lattice(nd, members=Ns, precision=prec, T_initial=T0)
         for r in range(len(Ns)):
@@ -133,14 +139,28 @@ def ensemble_leg(np, axs, name, T0, steps_override=0):
         lat.tau = 1.0
         cfg, tuned_ms = lat.autotune(trials=3)
         log(f"[bench] {name} {prec}: {len(Ns)} members, tuned {cfg} ({tuned_ms:.3f} ms)")
-        t0 = time.perf_counter()
-        lat.step(steps)
-        wall = time.perf_counter() - t0
-        counts[prec] = lat.string_counts()[0]
-        out[prec] = {"site_updates_per_s": sites * steps / wall, "ms_per_step": 1e3 * wall / steps,
-                     "kernel_ms": lat.last_step_ms, "config": cfg}
+        step_s = sample_s = 0.0
+        kern_ms_sum = 0.0
+        done = nsamples = 0
+        while done < steps:
+            k = min(every, steps - done)
+            t0 = time.perf_counter()
+            lat.step(k)
+            step_s += time.perf_counter() - t0
+            kern_ms_sum += lat.last_step_ms * k
+            done += k
+            t0 = time.perf_counter()
+            c = lat.string_counts()[0]
+            pks = [lat.powerspec(member=r, binning="log", nbins=64)[1] for r in range(len(Ns))]
+            sample_s += time.perf_counter() - t0
+            nsamples += 1
+        counts[prec], spectra[prec] = c, pks
+        out[prec] = {"site_updates_per_s": sites * steps / step_s, "ms_per_step": 1e3 * step_s / steps,
+                     "kernel_ms": kern_ms_sum / steps, "config": cfg, "samples": nsamples,
+                     "sample_s": sample_s}
         lat.close()
-        log(f"[bench] {name} {prec}: {1e3 * wall / steps:.3f} ms/step")
+        log(f"[bench] {name} {prec}: {1e3 * step_s / steps:.3f} ms/step, "
+            f"{nsamples} in-loop samples in {sample_s:.2f} s")
     c32 = counts["fp32"].astype(np.float64)
     c64 = counts["fp64"].astype(np.float64)
     rel = np.abs(c32 - c64) / np.maximum(c64, 1.0)
@@ -148,6 +168,12 @@ def ensemble_leg(np, axs, name, T0, steps_override=0):
     out["string_count_rel_diff_mean"] = float(rel.mean())
     out["string_count_total_fp32"] = int(c32.sum())
     out["string_count_total_fp64"] = int(c64.sum())
+    pk_err = 0.0
+    for a, b in zip(spectra["fp32"], spectra["fp64"]):
+        sel = b != 0
+        if sel.any():
+            pk_err = max(pk_err, float(np.max(np.abs(a[sel] - b[sel]) / np.abs(b[sel]))))
+    out["pk_log64_max_rel_err"] = pk_err
     return out
 
 

@@ -151,3 +151,30 @@ def test_multi_step_kernel_on_ensemble_matches_single(depth, flags, rows):
         a, b = lat.get_state(member=r), s.get_state()
         for k in a:
             np.testing.assert_array_equal(a[k], b[k], err_msg=f"member {r} {k}")
+
+
+@pytest.mark.parametrize("depth,flags", [(4, 3), (8, 11), (3, 2)])
+def test_2d_multi_step_ensemble_matches_single(depth, flags):
+    # the ensemble form of the 2D multi-step kernel: one persistent launch
+    # per D steps for all members, each member equal to its lone run
+    Ns = [128, 256, 64, 100]
+    lat = axs.Lattice(2, members=Ns, precision="fp32")
+    lat.set_kernel("march", depth=depth, flags=flags)
+    singles = []
+    for r, N in enumerate(Ns):
+        p1, p2 = kinv_field(2, N, seed=7000 + r)
+        z = np.zeros(N ** 2)
+        lat.set_state(p1, p2, z, z, member=r)
+        s = axs.Lattice(2, N, precision="fp32")
+        s.set_kernel("simple")
+        s.set_state(p1, p2, z, z, tau=1.0)
+        singles.append(s)
+    lat.tau = 1.0
+    lat.step(13)
+    npl, _ = lat.string_counts()
+    for r, s in enumerate(singles):
+        s.step(13)
+        a, b = lat.get_state(member=r), s.get_state()
+        for k in a:
+            np.testing.assert_array_equal(a[k], b[k], err_msg=f"member {r} {k}")
+        assert npl[r] == s.strings(coords=False)[0]
