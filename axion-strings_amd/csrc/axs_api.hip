@@ -407,16 +407,20 @@ static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ct
         set_error("stream synchronize failed after allocation");
         return fail(AXS_ERR_HIP);
     }
-    // FP32 standard stencil: several steps per pass unless configured
-    // otherwise (the fastest shapes measured): 3D four steps from side 256
-    // on, three below, 16-wave work-groups; 2D six steps
+    // standard stencil: several steps per pass unless configured otherwise
+    // (the fastest shapes measured): FP32 3D four steps from side 256 on,
+    // three below, 16-wave work-groups; FP64 3D three steps, 8-wave
+    // work-groups, two rows per wave; 2D six steps in both forms
     int min_side = 1 << 30;
     for (const Member &m : c->members) min_side = std::min(min_side, m.N);
-    c->tune.depth = (c->nd == 2) ? 6 : ((min_side >= 256) ? 4 : 3);
+    if (c->nd == 2)
+        c->tune.depth = 6;
+    else
+        c->tune.depth = (c->prec == AXS_FP32) ? ((min_side >= 256) ? 4 : 3) : 3;
     if (multi_step_depth(c) == 1)
         c->tune.depth = 1;
     else if (c->nd == 3)
-        c->tune.flags |= kTuneWG1024;
+        c->tune.flags |= (c->prec == AXS_FP32) ? kTuneWG1024 : kTuneWG512;
     (void)rc;
     *out = h;
     return AXS_OK;
@@ -778,26 +782,26 @@ int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
     const int depth = cfg->depth ? cfg->depth : 1;
     bool rows_ok;
     if (depth >= 2 && c->nd == 2) {
-        // several steps per pass, 2D: FP32 5-point stencil, one strip per wave
+        // several steps per pass, 2D: 5-point stencil, one strip per wave
         rows_ok = cfg->rows == 0 && (cfg->flags & (AXS_MARCH_WG512 | AXS_MARCH_WG1024)) == 0;
         bool fits = cfg->kernel != AXS_KERNEL_SIMPLE && march2d_depth_built(depth);
         for (const Member &m : c->members) fits = fits && march2d_supported(c, m.N);
         if (!fits) {
-            set_error("depth %d in 2D needs the FP32 form, stencil 0, sides >= 64, a "
-                      "non-simple kernel and depth 2, 3, 4, 6 or 8", depth);
+            set_error("depth %d in 2D needs stencil 0, the stencil Laplacian, sides >= 64, "
+                      "a non-simple kernel and depth 2, 3, 4, 6 or 8", depth);
             return AXS_ERR_UNSUPPORTED;
         }
     } else if (depth >= 2) {
         // several steps per pass: FP32 3D standard stencil on every member
         const int W = multi_waves(cfg->flags);
+        const int R = cfg->rows ? cfg->rows : (W == 4 ? 4 : (W == 8 ? 3 : 2));
         rows_ok = !((cfg->flags & AXS_MARCH_WG512) && (cfg->flags & AXS_MARCH_WG1024)) &&
-                  depth <= 4 && (cfg->rows == 0 || multi_shape_built(depth, W, cfg->rows)) &&
-                  (W != 4 || depth <= 3);
+                  multi_shape_built(depth, W, R, c->prec);
         bool fits = cfg->kernel != AXS_KERNEL_SIMPLE && depth <= 4;
         for (const Member &m : c->members) fits = fits && march2_supported(c, m.N);
         if (!fits) {
-            set_error("depth %d needs the FP32 form, 3D, stencil 0, sides >= 64, a "
-                      "non-simple kernel and depth <= 4", depth);
+            set_error("depth %d needs 3D, stencil 0, sides >= 64, a non-simple kernel and "
+                      "depth <= 4", depth);
             return AXS_ERR_UNSUPPORTED;
         }
     } else {

@@ -207,7 +207,7 @@ bool march2_supported(const Context *c, int N);
 // time steps per launch the configuration gives every member (1 to 4)
 int multi_step_depth(const Context *c);
 // the multi-step kernel is built for depth D, W waves, R rows per wave
-bool multi_shape_built(int D, int W, int R);
+bool multi_shape_built(int D, int W, int R, int prec);
 int multi_waves(int flags);
 // `depth` steps (scalars s[0..depth-1] of tau+dt, tau+2dt, ...) in one pass
 // per member

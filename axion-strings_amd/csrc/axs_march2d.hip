@@ -1,5 +1,6 @@
-// axs_march2d.hip -- FP32 2D step with temporal blocking: D time steps per
-// pass over HBM (5-point stencil), for single lattices and ensembles.
+// axs_march2d.hip -- 2D step with temporal blocking: D time steps per pass
+// over HBM (5-point stencil), for single lattices and ensembles, in the FP32
+// form and the bit-exact FP64 form (field-rescaled or Hamiltonian).
 //
 // Layout m = i + N*j (i contiguous).  One wave owns a strip of 64
 // consecutive columns i and marches along j over a chunk of CJ rows; level 0
@@ -7,20 +8,22 @@
 // computed on the strip minus an l-column rim, so the wave writes 64 - 2D
 // columns.  Per row j:
 //
-//   load (phi, u) of row j+D, drift x1 = phi + dt u           (level 0 -> x1)
-//   for l = 1..D, at row q = j+D-l:
-//      ker = F(x_l, tau_l);  u_l = u_{l-1} + dt ker;  x_{l+1} = x_l + dt u_l
-//   write (x_D, u_D) of row j
+//   load the state of row j+D and drift it                (level 0 -> x_1)
+//   for l = 1..D, at row q = j+D-l: one step of the form (axs_forms.hpp)
+//   write the state of row j
 //
-// Each level keeps x_l of rows q-1 and q and u_{l-1} of row q in registers;
-// row q+1 is the value the level below has just produced (the j
-// neighbours), and the i neighbours are the neighbouring lanes, read with
-// cross-lane shuffles -- no LDS, no barriers, every wave independent.  The
-// Laplacian sums in the one-step kernels' order (centre, +i, -i, +j, -j)
-// with the same expressions, so a pass is bit-identical to D one-step
-// launches.  Work items (member, strip, chunk) are dealt to waves from a
-// persistent list; strips at the lattice edge overhang it (columns wrap
-// periodically, stores are masked).
+// Each level keeps x_l of rows q-1 and q and the site's own data (FP32: u;
+// FP64: phidot, ker) of row q in registers; row q+1 is the value the level
+// below has just produced (the j neighbours), and the i neighbours are the
+// neighbouring lanes, read with cross-lane shuffles -- no LDS, no barriers,
+// every wave independent.  The Laplacian sums in the one-step kernels'
+// order (centre, +i, -i, +j, -j) with the same expressions, so a pass is
+// bit-identical to D one-step launches.  Work items (member, strip, chunk)
+// are dealt to waves from a persistent list; strips at the lattice edge
+// overhang it (columns wrap periodically, stores are masked).
+#include <type_traits>
+
+#include "axs_forms.hpp"
 #include "axs_internal.hpp"
 #include "axs_lattice.hpp"
 
@@ -31,18 +34,21 @@ constexpr int kWaves2D = 4;       // waves per work-group
 
 bool march2d_supported(const Context *c, int N)
 {
-    return c->nd == 2 && c->prec == AXS_FP32 && c->p.stencil_setting == 0 &&
-           c->p.eom == AXS_EOM_FIELD_RESCALED && c->p.laplacian == AXS_LAPLACIAN_STENCIL &&
+    return c->nd == 2 && c->p.stencil_setting == 0 && c->p.laplacian == AXS_LAPLACIAN_STENCIL &&
            N >= kW2;
 }
 
 bool march2d_depth_built(int D) { return D == 2 || D == 3 || D == 4 || D == 6 || D == 8; }
 
-template <int D, bool PF>
+template <class Tr, int D, bool PF>
 __device__ __forceinline__ void march2d_strip(const int N, const int CJ, const int ti,
-                                              const int ch, const fvec4 *__restrict__ in,
-                                              fvec4 *__restrict__ out, const TBScalars<D> &sc)
+                                              const int ch, const typename Tr::In &in,
+                                              const typename Tr::Out &out,
+                                              const LevelScalars<typename Tr::Sc, D> &sc)
 {
+    using T = typename Tr::T;
+    using Own = typename Tr::Own;
+    using Raw = typename Tr::Raw;
     constexpr int TI = kW2 - 2 * D;      // columns written
     const int lane = threadIdx.x & 63;
     const int i0 = ti * TI, j0 = ch * CJ;
@@ -55,41 +61,43 @@ __device__ __forceinline__ void march2d_strip(const int N, const int CJ, const i
         return (uint32_t)w * (uint32_t)N + (uint32_t)ii;
     };
 
-    // per level l = 1..D (index l-1): x_l of rows q-1 (xm), q (x0), u_{l-1} of row q
-    float xm[D][2], x0[D][2], uq[D][2];
+    // per level l = 1..D (index l-1): x_l of rows q-1 (xm), q (x0), own data of row q
+    T xm[D][2], x0[D][2];
+    Own oq[D];
 #pragma unroll
-    for (int l = 0; l < D; ++l)
+    for (int l = 0; l < D; ++l) {
 #pragma unroll
-        for (int q = 0; q < 2; ++q) xm[l][q] = x0[l][q] = uq[l][q] = 0.f;
-    fvec4 raw, pf;
-    if (PF) pf = in[row(j0 - D)];
+        for (int q = 0; q < 2; ++q) xm[l][q] = x0[l][q] = (T)0;
+        oq[l] = Own{};
+    }
+    Raw raw, pf;
+    if (PF) Tr::fetch(in, row(j0 - D), pf);
 
     for (int it = -2 * D; it < CJ; ++it) {
         const int j = j0 + it;
         if (PF) {
             raw = pf;
-            if (it + 1 < CJ) pf = in[row(j + D + 1)];
+            if (it + 1 < CJ) Tr::fetch(in, row(j + D + 1), pf);
         } else {
-            raw = in[row(j + D)];
+            Tr::fetch(in, row(j + D), raw);
         }
-        const StepScalarsF &s0 = sc.s[0];
-        float fx[2], fu[2];                 // x_l and u_{l-1} of row q+1, handed up
-        fx[0] = raw.x + s0.dt * raw.z;
-        fx[1] = raw.y + s0.dt * raw.w;
-        fu[0] = raw.z;
-        fu[1] = raw.w;
+        T fx[2];                           // x_l of row q+1, handed up
+        Own fo;                            // own data of row q+1
+        Tr::drift0(raw, sc.s[0], fx[0], fx[1], fo);
 #pragma unroll
         for (int l = 1; l <= D; ++l) {
-            const StepScalarsF &s = sc.s[l - 1];
+            const typename Tr::Sc &s = sc.s[l - 1];
+            const typename Tr::Sc &sn = sc.s[l < D ? l : D - 1];
             const bool active = it >= 2 * l - 2 * D;   // x_l of rows q-1, q, q+1 exist
-            float nx[2] = {0.f, 0.f}, nu[2] = {0.f, 0.f};
+            T nx[2] = {(T)0, (T)0};
+            Own no{};
             if (active) {
-                const float x = x0[l - 1][0], y = x0[l - 1][1];
+                const T x = x0[l - 1][0], y = x0[l - 1][1];
                 // i neighbours from the adjacent lanes (all lanes take part)
-                const float xr = __shfl_down(x, 1), yr = __shfl_down(y, 1);
-                const float xl = __shfl_up(x, 1), yl = __shfl_up(y, 1);
+                const T xr = __shfl_down(x, 1), yr = __shfl_down(y, 1);
+                const T xl = __shfl_up(x, 1), yl = __shfl_up(y, 1);
                 if (lane >= l && lane < kW2 - l) {
-                    float a = s.c[0] * x, c = s.c[0] * y;
+                    T a = s.c[0] * x, c = s.c[0] * y;
                     a = a + xr;
                     c = c + yr;                 // i+1
                     a = a + xl;
@@ -98,26 +106,15 @@ __device__ __forceinline__ void march2d_strip(const int N, const int CJ, const i
                     c = c + fx[1];              // j+1
                     a = a + xm[l - 1][0];
                     c = c + xm[l - 1][1];       // j-1
-                    const float S = x * x + y * y - s.tau2 + s.T03;
-                    const float k1 = s.alpha * a + s.cl * x * S;
-                    const float k2 = s.alpha * c + s.cl * y * S;
-                    const float ua = uq[l - 1][0] + s.dt * k1;
-                    const float ub = uq[l - 1][1] + s.dt * k2;
+                    Own on;
+                    T ex, ey;
+                    Tr::level(x, y, a, c, oq[l - 1], s, sn, on, ex, ey);
                     if (l == D) {
-                        if (it >= 0 && colout) {
-                            fvec4 v;
-                            v.x = x;
-                            v.y = y;
-                            v.z = ua;
-                            v.w = ub;
-                            store_nt(out + row(j), v);
-                        }
+                        if (it >= 0 && colout) Tr::store(out, row(j), x, y, on);
                     } else {
-                        const StepScalarsF &sn = sc.s[l < D ? l : D - 1];
-                        nx[0] = x + sn.dt * ua;
-                        nx[1] = y + sn.dt * ub;
-                        nu[0] = ua;
-                        nu[1] = ub;
+                        nx[0] = ex;
+                        nx[1] = ey;
+                        no = on;
                     }
                 }
             }
@@ -125,19 +122,36 @@ __device__ __forceinline__ void march2d_strip(const int N, const int CJ, const i
             for (int q = 0; q < 2; ++q) {
                 xm[l - 1][q] = x0[l - 1][q];
                 x0[l - 1][q] = fx[q];
-                uq[l - 1][q] = fu[q];
                 fx[q] = nx[q];
-                fu[q] = nu[q];
             }
+            oq[l - 1] = fo;
+            fo = no;
         }
     }
 }
 
+template <class Tr>
+__device__ __forceinline__ typename Tr::In strip_in(void *const *p)
+{
+    if constexpr (std::is_same<Tr, F32Form>::value)
+        return {(const fvec4 *)p[PHI]};
+    else
+        return {(const dvec2 *)p[PHI], (const dvec2 *)p[VEL], (const dvec2 *)p[KER]};
+}
+template <class Tr>
+__device__ __forceinline__ typename Tr::Out strip_out(void *const *p)
+{
+    if constexpr (std::is_same<Tr, F32Form>::value)
+        return {(fvec4 *)p[PHI]};
+    else
+        return {(dvec2 *)p[PHI], (dvec2 *)p[VEL], (dvec2 *)p[KER]};
+}
+
 // one work item per wave: (member, strip, chunk) from the member table
-template <int D, bool PF>
+template <class Tr, int D, bool PF>
 __global__ void __launch_bounds__(64 * kWaves2D)
 march2d_kernel(const BatchMember *__restrict__ mem, int n_members, uint32_t n_items,
-               TBScalars<D> sc)
+               LevelScalars<typename Tr::Sc, D> sc)
 {
     const uint32_t wave = blockIdx.x * kWaves2D + (threadIdx.x >> 6);
     const uint32_t stride = gridDim.x * kWaves2D;
@@ -154,19 +168,18 @@ march2d_kernel(const BatchMember *__restrict__ mem, int n_members, uint32_t n_it
         const uint32_t t = item - m.item_begin;
         const int ti = (int)(t % (uint32_t)m.nTj);
         const int ch = (int)(t / (uint32_t)m.nTj);
-        march2d_strip<D, PF>(m.N, m.CI, ti, ch, (const fvec4 *)m.in[PHI], (fvec4 *)m.out[PHI],
-                             sc);
+        march2d_strip<Tr, D, PF>(m.N, m.CI, ti, ch, strip_in<Tr>(m.in), strip_out<Tr>(m.out), sc);
     }
 }
 
-// rows per chunk: the configured count if it divides N, else 64 (measured
-// best for the 512-lattice ensemble of sides 128..1024: fewer halo rows per
-// chunk than 16 or 32)
+// rows per chunk: the configured count if it divides N, else 128 (measured
+// best for the 512-lattice ensemble of sides 128..1024 in both forms: fewer
+// halo rows per chunk than 32 or 64)
 static int pick_cj(const Context *c, int N)
 {
     int cj = c->tune.ci32;
     if (cj > 0 && N % cj == 0) return cj;
-    cj = 64;
+    cj = 128;
     while (N % cj != 0) --cj;
     return cj;
 }
@@ -190,8 +203,10 @@ int prepare_march2d_items(Context *c)
         b.CI = pick_cj(c, m.N);            // rows per chunk
         items += (uint32_t)b.nTj * (uint32_t)(m.N / b.CI);
         for (int h = 0; h < 2; ++h) {
-            b.in[PHI] = m.state[h][PHI];
-            b.out[PHI] = m.state[h ^ 1][PHI];
+            for (int a = 0; a < 3; ++a) {
+                b.in[a] = m.state[h][a];
+                b.out[a] = m.state[h ^ 1][a];
+            }
             tab[h].push_back(b);
         }
     }
@@ -212,12 +227,21 @@ int prepare_march2d_items(Context *c)
     return AXS_OK;
 }
 
-template <int D>
+template <class Sc>
+static Sc strip_scalars(const StepScalars &s)
+{
+    if constexpr (std::is_same<Sc, StepScalarsF>::value)
+        return to_float(s);
+    else
+        return s;
+}
+
+template <class Tr, int D>
 static int launch_d(Context *c, int src, const StepScalars *s, bool pf)
 {
-    TBScalars<D> sc;
-    for (int l = 0; l < D; ++l) sc.s[l] = to_float(s[l]);
-    auto kern = pf ? march2d_kernel<D, true> : march2d_kernel<D, false>;
+    LevelScalars<typename Tr::Sc, D> sc;
+    for (int l = 0; l < D; ++l) sc.s[l] = strip_scalars<typename Tr::Sc>(s[l]);
+    auto kern = pf ? march2d_kernel<Tr, D, true> : march2d_kernel<Tr, D, false>;
     int per_cu = 0;
     AXS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * kWaves2D, 0));
     const uint32_t cap = (uint32_t)c->num_cu * (uint32_t)(per_cu > 0 ? per_cu : 1);
@@ -228,6 +252,21 @@ static int launch_d(Context *c, int src, const StepScalars *s, bool pf)
     return AXS_OK;
 }
 
+template <class Tr>
+static int launch_form2d(Context *c, int src, const StepScalars *s, int depth, bool pf)
+{
+    switch (depth) {
+    case 2: return launch_d<Tr, 2>(c, src, s, pf);
+    case 3: return launch_d<Tr, 3>(c, src, s, pf);
+    case 4: return launch_d<Tr, 4>(c, src, s, pf);
+    case 6: return launch_d<Tr, 6>(c, src, s, pf);
+    case 8: return launch_d<Tr, 8>(c, src, s, pf);
+    default:
+        set_error("2D depth %d is not built", depth);
+        return AXS_ERR_UNSUPPORTED;
+    }
+}
+
 int launch_march2d(Context *c, int src, const StepScalars *s, int depth)
 {
     if (!c->mbatch_tab[src]) {
@@ -235,16 +274,9 @@ int launch_march2d(Context *c, int src, const StepScalars *s, int depth)
         return AXS_ERR_STATE;
     }
     const bool pf = (c->tune.flags & kTunePrefetch) != 0;
-    switch (depth) {
-    case 2: return launch_d<2>(c, src, s, pf);
-    case 3: return launch_d<3>(c, src, s, pf);
-    case 4: return launch_d<4>(c, src, s, pf);
-    case 6: return launch_d<6>(c, src, s, pf);
-    case 8: return launch_d<8>(c, src, s, pf);
-    default:
-        set_error("2D depth %d is not built", depth);
-        return AXS_ERR_UNSUPPORTED;
-    }
+    if (c->prec == AXS_FP32) return launch_form2d<F32Form>(c, src, s, depth, pf);
+    if (c->p.eom == AXS_EOM_HAMILTONIAN) return launch_form2d<F64FormT<true>>(c, src, s, depth, pf);
+    return launch_form2d<F64FormT<false>>(c, src, s, depth, pf);
 }
 
 }  // namespace axs

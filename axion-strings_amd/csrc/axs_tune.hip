@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <mutex>
 #include <string>
@@ -167,7 +168,7 @@ int run_autotune(Context *c, int trials, double *best_ms)
     int minN = 1 << 30;
     for (const Member &m : c->members) minN = std::min(minN, m.N);
     const bool march_ok = c->nd == 3 && c->p.stencil_setting == 0;
-    bool strips_ok = c->nd == 2 && c->prec == AXS_FP32;
+    bool strips_ok = c->nd == 2;
     for (const Member &m : c->members) strips_ok = strips_ok && march2d_supported(c, m.N);
     for (int graph = 0; graph < 2; ++graph) {
         Tune t;
@@ -212,11 +213,16 @@ int run_autotune(Context *c, int trials, double *best_ms)
                     u.flags = flag_opts[fi] | (graph ? kTuneGraph : 0);
                     cands.push_back(u);
                 }
-        if (c->prec != AXS_FP32) continue;
         // several steps per pass: (depth, waves per work-group, rows per wave)
-        const int shapes[][3] = {{2, 4, 4}, {2, 8, 2}, {2, 8, 3}, {3, 4, 4}, {3, 8, 2},
-                                 {3, 8, 3}, {3, 8, 4}, {3, 16, 2}, {4, 8, 3}, {4, 8, 4},
-                                 {4, 16, 2}, {4, 16, 3}};
+        const int shapes32[][3] = {{2, 4, 4}, {2, 8, 2}, {2, 8, 3}, {3, 4, 4}, {3, 8, 2},
+                                   {3, 8, 3}, {3, 8, 4}, {3, 16, 2}, {4, 8, 3}, {4, 8, 4},
+                                   {4, 16, 2}, {4, 16, 3}};
+        const int shapes64[][3] = {{2, 8, 2}, {2, 8, 3}, {3, 8, 2}, {3, 8, 3}};
+        std::vector<std::array<int, 3>> shapes;
+        if (c->prec == AXS_FP32)
+            for (const auto &x : shapes32) shapes.push_back({x[0], x[1], x[2]});
+        else
+            for (const auto &x : shapes64) shapes.push_back({x[0], x[1], x[2]});
         const int chunk_opts2[3] = {0, 16, 32};
         for (const auto &sh : shapes)
             for (int pf = 0; pf < 2; ++pf)

@@ -207,11 +207,13 @@ typedef struct axs_kernel_config {
                      with AXS_MARCH_WG512 2, 3 or 4, with AXS_MARCH_WG1024
                      2 or 3                                                 */
     int chunk;    /* planes marched per work-group (divides N)              */
-    int depth;    /* time steps per pass over the lattice: 1 (0 = 1), or
-                     FP32 form, stencil 0, every side >= 64: 3D 2 to 4 (a
-                     new context starts at 4, or 3 if a side is below 256,
-                     with AXS_MARCH_WG1024), 2D 2, 3, 4, 6 or 8 (starts
-                     at 6; rows 0, no work-group flags).
+    int depth;    /* time steps per pass over the lattice: 1 (0 = 1), or,
+                     with stencil 0, the stencil Laplacian and every side
+                     >= 64: 3D FP32 2 to 4 (a new context starts at 4, or
+                     3 if a side is below 256, with AXS_MARCH_WG1024), 3D
+                     FP64 2 or 3 with AXS_MARCH_WG512 (starts at 3), 2D
+                     2, 3, 4, 6 or 8 (starts at 6; rows 0, no work-group
+                     flags).
                      Step counts that are not a multiple end with one-step
                      launches.                                              */
     int flags;    /* AXS_MARCH_* bits                                       */

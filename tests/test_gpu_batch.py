@@ -178,3 +178,43 @@ def test_2d_multi_step_ensemble_matches_single(depth, flags):
         for k in a:
             np.testing.assert_array_equal(a[k], b[k], err_msg=f"member {r} {k}")
         assert npl[r] == s.strings(coords=False)[0]
+
+
+def test_fp64_multi_step_ensemble_bit_exact():
+    Ns = [64, 128, 96]
+    lat = axs.Lattice(3, members=Ns, precision="fp64")
+    lat.set_kernel("march", depth=2, flags=19)
+    states = []
+    for r, N in enumerate(Ns):
+        p1, p2 = kinv_field(3, N, seed=8000 + r)
+        z = np.zeros(N ** 3)
+        lat.set_state(p1, p2, z, z, member=r)
+        states.append((p1, p2, z, z))
+    lat.tau = 1.0
+    lat.step(7)
+    for r, N in enumerate(Ns):
+        st = O.State(*states[r])
+        st.run(O.params(3, N), O.Matrix(3, N), 7)
+        g = lat.get_state(member=r)
+        for k in ("phi1", "phi2", "phidot1", "phidot2", "ker1", "ker2"):
+            np.testing.assert_array_equal(g[k], getattr(st, k), err_msg=f"member {r} {k}")
+
+
+def test_2d_fp64_multi_step_ensemble_bit_exact():
+    Ns = [128, 64, 256]
+    lat = axs.Lattice(2, members=Ns, precision="fp64")
+    lat.set_kernel("march", depth=4, flags=3)
+    states = []
+    for r, N in enumerate(Ns):
+        p1, p2 = kinv_field(2, N, seed=9000 + r)
+        z = np.zeros(N ** 2)
+        lat.set_state(p1, p2, z, z, member=r)
+        states.append((p1, p2, z, z))
+    lat.tau = 1.0
+    lat.step(11)
+    for r, N in enumerate(Ns):
+        st = O.State(*states[r])
+        st.run(O.params(2, N), O.Matrix(2, N), 11)
+        g = lat.get_state(member=r)
+        for k in ("phi1", "phi2", "phidot1", "phidot2", "ker1", "ker2"):
+            np.testing.assert_array_equal(g[k], getattr(st, k), err_msg=f"member {r} {k}")

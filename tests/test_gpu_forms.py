@@ -145,3 +145,30 @@ def test_energy_rejected_for_hamiltonian_form():
     lat.set_state(z + 1, z, z, z, tau=1.0)
     with pytest.raises(axs.AxsError, match="field-rescaled"):
         lat.energy()
+
+
+@pytest.mark.parametrize("depth,rows,flags,nsteps", [(2, 2, 19, 9), (3, 3, 18, 8)])
+def test_hamiltonian_form_multi_step_bit_exact(depth, rows, flags, nsteps):
+    nd, N = 3, 64
+    state = random_state(nd, N, seed=22, ker=True)
+    st = _oracle(nd, N, state, nsteps, eom=1)
+    lat = axs.Lattice(nd, N, eom="hamiltonian")
+    lat.set_kernel("march", rows=rows, depth=depth, flags=flags)
+    lat.set_state(*state, tau=1.0)
+    lat.step(nsteps)
+    g = lat.get_state()
+    for k in ("phi1", "phi2", "phidot1", "phidot2", "ker1", "ker2"):
+        np.testing.assert_array_equal(g[k], getattr(st, k), err_msg=k)
+
+
+def test_hamiltonian_form_2d_multi_step_bit_exact():
+    nd, N, nsteps = 2, 128, 15
+    state = random_state(nd, N, seed=23, ker=True)
+    st = _oracle(nd, N, state, nsteps, eom=1)
+    lat = axs.Lattice(nd, N, eom="hamiltonian")
+    lat.set_kernel("march", depth=4, flags=3)
+    lat.set_state(*state, tau=1.0)
+    lat.step(nsteps)
+    g = lat.get_state()
+    for k in ("phi1", "phi2", "phidot1", "phidot2", "ker1", "ker2"):
+        np.testing.assert_array_equal(g[k], getattr(st, k), err_msg=k)

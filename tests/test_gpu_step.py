@@ -201,11 +201,11 @@ def test_multi_step_kernel_matches_one_step(N, depth, rows, chunk, flags, nsteps
     (3, 64, "fp64", "march"), (2, 48, "fp32", "auto"), (3, 48, "fp32", "march"),
     (3, 64, "fp32", "simple")])
 def test_two_step_kernel_rejects_other_forms(nd, N, precision, kernel):
-    # a rejected configuration leaves the context's own one in place (an
-    # FP32 3D context below side 256 starts at depth 3)
+    # a rejected configuration leaves the context's own one in place (a 3D
+    # context below side 256 starts at depth 3, FP32 and FP64 alike)
     lat = axs.Lattice(nd, N, precision=precision)
     before = lat.kernel_config()
-    assert before["depth"] == (3 if (nd, N, precision) == (3, 64, "fp32") else 1)
+    assert before["depth"] == (3 if (nd, N) == (3, 64) else 1)
     with pytest.raises(axs.AxsError, match="depth 2"):
         lat.set_kernel(kernel, depth=2)
     assert lat.kernel_config() == before
@@ -251,3 +251,32 @@ def test_2d_multi_step_rejects_unbuilt_shapes(depth, rows, flags):
     with pytest.raises(axs.AxsError):
         lat.set_kernel("march", rows=rows, depth=depth, flags=flags)
     assert lat.kernel_config() == before
+
+
+@pytest.mark.parametrize("N,depth,rows,chunk,flags,nsteps", [
+    (64, 2, 2, 0, 19, 8), (64, 3, 3, 16, 18, 7), (100, 2, 3, 0, 19, 9), (128, 3, 2, 32, 27, 10),
+    (192, 2, 2, 0, 18, 5)])
+def test_fp64_multi_step_bit_exact_vs_oracle(N, depth, rows, chunk, flags, nsteps):
+    # the plain FP64 form, several steps per pass (8 waves per work-group):
+    # the reference's velocity-Verlet bit for bit, ker included
+    state = random_state(3, N, seed=13, ker=True)
+    st = _oracle_run(3, N, state, nsteps)
+    lat = axs.Lattice(3, N)
+    lat.set_kernel("march", rows=rows, chunk=chunk, depth=depth, flags=flags)
+    assert lat.kernel_config()["depth"] == depth
+    assert lat.bytes_per_site()[0] == 96.0 / depth
+    lat.set_state(*state, tau=1.0)
+    lat.step(nsteps)
+    _assert_same(lat, st)
+
+
+@pytest.mark.parametrize("N,depth,chunk,flags,nsteps", [
+    (64, 2, 0, 3, 8), (96, 4, 0, 2, 11), (128, 6, 32, 3, 13), (100, 3, 0, 11, 7)])
+def test_2d_fp64_multi_step_bit_exact_vs_oracle(N, depth, chunk, flags, nsteps):
+    state = random_state(2, N, seed=14, ker=True)
+    st = _oracle_run(2, N, state, nsteps)
+    lat = axs.Lattice(2, N)
+    lat.set_kernel("march", chunk=chunk, depth=depth, flags=flags)
+    lat.set_state(*state, tau=1.0)
+    lat.step(nsteps)
+    _assert_same(lat, st)
