@@ -247,6 +247,7 @@ Context::~Context()
     }
     if (scratch) (void)hipFree(scratch);
     if (listbuf) (void)hipFree(listbuf);
+    if (strbuf) (void)hipFree(strbuf);
     for (int h = 0; h < 2; ++h)
         if (batch_tab[h]) (void)hipFree(batch_tab[h]);
     for (int h = 0; h < 2; ++h)
@@ -268,9 +269,12 @@ int current_fields(Context *c, Member &m, FieldView *v)
         v->vel = m.state[c->cur][VEL];
         return AXS_OK;
     }
-    StepScalars s = make_scalars(c->p, c->tau);
-    AXS_TRY(launch_fp32_canonical(c, m, s, !m.ker_valid));
-    m.ker_valid = true;
+    if (!m.vel_valid) {     // once per state: ker (if needed) and phidot
+        StepScalars s = make_scalars(c->p, c->tau);
+        AXS_TRY(launch_fp32_canonical(c, m, s, !m.ker_valid));
+        m.ker_valid = true;
+        m.vel_valid = true;
+    }
     v->vel = m.aux_vel;
     return AXS_OK;
 }
@@ -514,6 +518,8 @@ int axs_set_member_state(axs_ctx *ctx, int member, const double *phi1, const dou
         m.ker_valid = true;
     }
     m.has_state = true;
+    m.vel_valid = false;
+    m.version++;
     return AXS_OK;
 }
 
@@ -581,7 +587,9 @@ int axs_set_tau(axs_ctx *ctx, double tau)
         return AXS_ERR_INVALID_ARG;
     }
     ctx->impl.tau = tau;
-    // FP32: a stored ker no longer matches the new tau unless no step ran
+    // FP32: a ker recomputed from phi depends on tau; an uploaded one does not
+    for (Member &m : ctx->impl.members)
+        if (!m.ker_valid) m.vel_valid = false;
     return AXS_OK;
 }
 
@@ -696,7 +704,11 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
     float ms = 0.f;
     AXS_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_step_ms = (double)ms / (double)nsteps;
-    for (Member &m : c->members) m.ker_valid = false;
+    for (Member &m : c->members) {
+        m.ker_valid = false;
+        m.vel_valid = false;
+        m.version++;
+    }
     return AXS_OK;
 }
 

@@ -344,6 +344,8 @@ int run_init(Context *c, Member &m, int mode, uint64_t seed)
     AXS_HIP_TRY(hipGetLastError());
     AXS_HIP_TRY(hipStreamSynchronize(c->stream));
     m.has_state = true;
+    m.vel_valid = false;
+    m.version++;
     return AXS_OK;
 }
 

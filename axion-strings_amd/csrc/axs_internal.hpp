@@ -96,6 +96,8 @@ struct Member {
     void *aux_vel = nullptr;
     bool has_state = false;
     bool ker_valid = true;
+    bool vel_valid = false;   // FP32: aux_vel holds phidot of the current state
+    uint64_t version = 0;     // bumped whenever the state changes
 };
 
 struct FftPlan;
@@ -181,6 +183,13 @@ struct Context {
     bool batch_march = false;    // 3D ensemble on the march kernel
     // the same for the multi-step kernel (tiles of its own shape)
     BatchMember *mbatch_tab[2] = {nullptr, nullptr};
+    // string finder results of the last call (ballot words, block counts,
+    // offsets), reused while the member's state is unchanged
+    void *strbuf = nullptr;
+    size_t strbuf_bytes = 0;
+    int str_member = -1;
+    uint64_t str_version = 0;
+    uint64_t str_total = 0;
     uint32_t mbatch_items = 0;
     int mbatch_key = -1;
 

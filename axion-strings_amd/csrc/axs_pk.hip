@@ -11,9 +11,11 @@
 //
 // Binning is done per bin rather than per mode: a bin is a range [lo, hi)
 // of the integer |k|^2, so for every (fi, fj) row the modes that fall in
-// the bin are one contiguous run of fz.  One workgroup owns one bin, walks
-// the rows of the bin's bounding box in a fixed order and sums its runs in
-// compensated FP64 -- deterministic, with no atomics.
+// the bin are one contiguous run of fz.  The rows of each bin's bounding
+// box are cut into chunks of kPkRows; one workgroup sums one (bin, chunk)
+// item in compensated FP64 (a work-item per row, then a fixed-order tree),
+// and a fold kernel merges each bin's items in chunk order -- balanced over
+// the bins (the outer log bins hold most modes), deterministic, no atomics.
 #include <rocfft/rocfft.h>
 
 #include <math.h>
@@ -140,28 +142,43 @@ __device__ __forceinline__ int64_t isqrt_ceil(int64_t v)
     return r;
 }
 
-// One workgroup per bin b: sums over modes with lo2[b] <= |k|^2 < hi2[b].
-// F: half spectrum, fast axis length H = N/2+1.  Rows enumerated over the
-// bounding box |fi|,|fj| <= R (3D) or |fj| <= R (2D), in row-major order.
+constexpr int kPkRows = 256;    // bounding-box rows per work item
+
+// Item t of bin b (item_begin[b] <= t < item_begin[b+1]) sums the modes with
+// lo2[b] <= |k|^2 < hi2[b] on rows [chunk*kPkRows, chunk*kPkRows + kPkRows)
+// of the bin's bounding box |fi|,|fj| <= R (3D) or |fj| <= R (2D), rows in
+// row-major order.  F: half spectrum, fast axis length H = N/2+1.
 template <int ND>
-__global__ void __launch_bounds__(256)
-bin_kernel(int N, const double2 *__restrict__ F, const int64_t *__restrict__ lo2,
-           const int64_t *__restrict__ hi2, double *__restrict__ out_pk,
-           double *__restrict__ out_ks, long long *__restrict__ out_cnt)
+__global__ void __launch_bounds__(kPkRows)
+bin_kernel(int N, const double2 *__restrict__ F, int nbins, const uint32_t *__restrict__ item_begin,
+           const int64_t *__restrict__ lo2, const int64_t *__restrict__ hi2,
+           KahanP *__restrict__ part_pk, KahanP *__restrict__ part_ks,
+           long long *__restrict__ part_cnt)
 {
-    __shared__ KahanP sh_pk[4], sh_ks[4];
-    __shared__ long long sh_n[4];
-    const int b = blockIdx.x;
+    __shared__ KahanP sh_pk[kPkRows / 64], sh_ks[kPkRows / 64];
+    __shared__ long long sh_n[kPkRows / 64];
+    const uint32_t item = blockIdx.x;
+    int lo_b = 0, hi_b = nbins - 1;              // last bin starting at or before item
+    while (lo_b < hi_b) {
+        int mid = (lo_b + hi_b + 1) >> 1;
+        if (item_begin[mid] <= item)
+            lo_b = mid;
+        else
+            hi_b = mid - 1;
+    }
+    const int b = lo_b;
+    const int64_t chunk = item - item_begin[b];
     const int64_t lo = lo2[b], hi = hi2[b];
     const int H = N / 2 + 1;
     KahanP pk{0.0, 0.0}, ks{0.0, 0.0};
     long long cnt = 0;
-    if (hi > lo) {
+    {
         int64_t R = isqrt_ceil(hi);           // |f| < R on every axis
         if (R > N / 2) R = N / 2;
         int64_t side = 2 * R + 1;
         int64_t nrows = (ND == 3) ? side * side : side;
-        for (int64_t r = threadIdx.x; r < nrows; r += blockDim.x) {
+        int64_t r = chunk * kPkRows + threadIdx.x;
+        if (hi > lo && r < nrows) {
             int64_t fi = 0, fj;
             if (ND == 3) {
                 fi = r / side - R;
@@ -169,23 +186,22 @@ bin_kernel(int N, const double2 *__restrict__ F, const int64_t *__restrict__ lo2
             } else {
                 fj = r - R;
             }
-            if (fi <= -N / 2 || fj <= -N / 2) continue;     // |k| >= N/2 there
             int64_t r2 = fi * fi + fj * fj;
-            if (r2 >= hi) continue;
-            int64_t z0 = isqrt_ceil(lo - r2);
-            int64_t z1 = isqrt_ceil(hi - r2);                // exclusive
-            if (z1 > H) z1 = H;
-            if (z0 >= z1) continue;
-            int64_t ii = fi < 0 ? fi + N : fi, jj = fj < 0 ? fj + N : fj;
-            int64_t rowbase = (ND == 3) ? (ii * N + jj) * H : jj * H;
-            for (int64_t z = z0; z < z1; ++z) {
-                double2 f = F[rowbase + z];
-                double w = (z == 0 || z == N / 2) ? 1.0 : 2.0;
-                double p2 = f.x * f.x + f.y * f.y;
-                double km = sqrt((double)(r2 + z * z));
-                kaddp(pk, w * p2);
-                kaddp(ks, w * km);
-                cnt += (long long)w;
+            if (fi > -N / 2 && fj > -N / 2 && r2 < hi) {   // |k| >= N/2 otherwise
+                int64_t z0 = isqrt_ceil(lo - r2);
+                int64_t z1 = isqrt_ceil(hi - r2);        // exclusive
+                if (z1 > H) z1 = H;
+                int64_t ii = fi < 0 ? fi + N : fi, jj = fj < 0 ? fj + N : fj;
+                int64_t rowbase = (ND == 3) ? (ii * N + jj) * H : jj * H;
+                for (int64_t z = z0; z < z1; ++z) {
+                    double2 f = F[rowbase + z];
+                    double w = (z == 0 || z == N / 2) ? 1.0 : 2.0;
+                    double p2 = f.x * f.x + f.y * f.y;
+                    double km = sqrt((double)(r2 + z * z));
+                    kaddp(pk, w * p2);
+                    kaddp(ks, w * km);
+                    cnt += (long long)w;
+                }
             }
         }
     }
@@ -207,15 +223,36 @@ bin_kernel(int N, const double2 *__restrict__ F, const int64_t *__restrict__ lo2
     if (threadIdx.x == 0) {
         KahanP a = sh_pk[0], k = sh_ks[0];
         long long n = sh_n[0];
-        for (int w = 1; w < 4; ++w) {
+        for (int w = 1; w < kPkRows / 64; ++w) {
             a = kmergep(a, sh_pk[w]);
             k = kmergep(k, sh_ks[w]);
             n += sh_n[w];
         }
-        out_pk[b] = a.s + a.c;
-        out_ks[b] = k.s + k.c;
-        out_cnt[b] = n;
+        part_pk[item] = a;
+        part_ks[item] = k;
+        part_cnt[item] = n;
     }
+}
+
+// bin b: its items' partials merged in item order
+__global__ void __launch_bounds__(64)
+bin_fold_kernel(int nbins, const uint32_t *__restrict__ item_begin,
+                const KahanP *__restrict__ part_pk, const KahanP *__restrict__ part_ks,
+                const long long *__restrict__ part_cnt, double *__restrict__ out_pk,
+                double *__restrict__ out_ks, long long *__restrict__ out_cnt)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbins) return;
+    KahanP a{0.0, 0.0}, k{0.0, 0.0};
+    long long n = 0;
+    for (uint32_t t = item_begin[b]; t < item_begin[b + 1]; ++t) {
+        a = kmergep(a, part_pk[t]);
+        k = kmergep(k, part_ks[t]);
+        n += part_cnt[t];
+    }
+    out_pk[b] = a.s + a.c;
+    out_ks[b] = k.s + k.c;
+    out_cnt[b] = n;
 }
 
 int run_powerspec(Context *c, Member &m, int binning, int nbins, double *ks, double *pk,
@@ -277,20 +314,50 @@ int run_powerspec(Context *c, Member &m, int binning, int nbins, double *ks, dou
             if (lo[b] > hi[b]) lo[b] = hi[b];
         }
     }
+    // work items: each bin's bounding-box rows in chunks of kPkRows
     size_t nb = (size_t)nbins;
-    size_t need = nb * (8 + 8 + 8 + 8 + 8);
-    AXS_TRY(ensure_scratch(c, need + 256));
+    std::vector<uint32_t> items(nb + 1, 0);
+    for (int b = 0; b < nbins; ++b) {
+        int64_t rows = 0;
+        if (hi[b] > lo[b]) {
+            int64_t R = (int64_t)ceil(sqrt((double)hi[b]));
+            while (R * R < hi[b]) ++R;
+            while (R > 0 && (R - 1) * (R - 1) >= hi[b]) --R;
+            if (R > N / 2) R = N / 2;
+            int64_t side = 2 * R + 1;
+            rows = (c->nd == 3) ? side * side : side;
+        }
+        items[b + 1] = items[b] + (uint32_t)((rows + kPkRows - 1) / kPkRows);
+    }
+    const uint32_t nitems = items[nb];
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_lo = 0, o_hi = o_lo + align(nb * 8), o_ib = o_hi + align(nb * 8);
+    const size_t o_ppk = o_ib + align((nb + 1) * 4), o_pks = o_ppk + align((size_t)nitems * 16);
+    const size_t o_pcnt = o_pks + align((size_t)nitems * 16);
+    const size_t o_pk = o_pcnt + align((size_t)nitems * 8), o_ks = o_pk + align(nb * 8);
+    const size_t o_cnt = o_ks + align(nb * 8), need = o_cnt + align(nb * 8);
+    AXS_TRY(ensure_scratch(c, need));
     char *sp = (char *)c->scratch;
-    int64_t *dlo = (int64_t *)sp;
-    int64_t *dhi = dlo + nb;
-    double *dpk = (double *)(dhi + nb);
-    double *dks = dpk + nb;
-    long long *dcnt = (long long *)(dks + nb);
+    int64_t *dlo = (int64_t *)(sp + o_lo);
+    int64_t *dhi = (int64_t *)(sp + o_hi);
+    uint32_t *dib = (uint32_t *)(sp + o_ib);
+    KahanP *ppk = (KahanP *)(sp + o_ppk);
+    KahanP *pks = (KahanP *)(sp + o_pks);
+    long long *pcnt = (long long *)(sp + o_pcnt);
+    double *dpk = (double *)(sp + o_pk);
+    double *dks = (double *)(sp + o_ks);
+    long long *dcnt = (long long *)(sp + o_cnt);
     AXS_HIP_TRY(hipMemcpyAsync(dlo, lo.data(), nb * 8, hipMemcpyHostToDevice, c->stream));
     AXS_HIP_TRY(hipMemcpyAsync(dhi, hi.data(), nb * 8, hipMemcpyHostToDevice, c->stream));
-    auto k = (c->nd == 3) ? bin_kernel<3> : bin_kernel<2>;
-    hipLaunchKernelGGL(k, dim3(nbins), dim3(256), 0, c->stream, N, (const double2 *)plan->out,
-                       dlo, dhi, dpk, dks, dcnt);
+    AXS_HIP_TRY(hipMemcpyAsync(dib, items.data(), (nb + 1) * 4, hipMemcpyHostToDevice,
+                               c->stream));
+    if (nitems > 0) {
+        auto k = (c->nd == 3) ? bin_kernel<3> : bin_kernel<2>;
+        hipLaunchKernelGGL(k, dim3(nitems), dim3(kPkRows), 0, c->stream, N,
+                           (const double2 *)plan->out, nbins, dib, dlo, dhi, ppk, pks, pcnt);
+    }
+    hipLaunchKernelGGL(bin_fold_kernel, dim3((nbins + 63) / 64), dim3(64), 0, c->stream, nbins,
+                       dib, ppk, pks, pcnt, dpk, dks, dcnt);
     AXS_HIP_TRY(hipGetLastError());
     std::vector<double> hpk(nb), hks(nb);
     std::vector<long long> hcnt(nb);

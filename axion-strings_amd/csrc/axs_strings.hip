@@ -7,11 +7,15 @@
 // is always in ascending site order (the single-thread order), which also
 // fixes the 2D core count of s_run.cpp:101-109.
 //
-// Pass 1 evaluates the plaquette test per site and stores one 64-bit ballot
-// word per wave (a bitmask of flagged sites) plus a count per block; pass 2
-// scans the block counts; pass 3 re-reads the ballot words and writes the
-// coordinates at their ordered positions.  The angle of each corner is
-// (atan2(phi1, phi2) + pi) / (2 pi) in FP64, as in the reference.
+// Pass 0 computes each site's angle (atan2(phi1, phi2) + pi) / (2 pi) in
+// FP64 once -- the reference's axion array (s_run.cpp:92-94) scaled as in
+// string_finding.cpp:14-16 -- so the FP64 atan2 runs once per site rather
+// than once per plaquette corner.  Pass 1 evaluates the plaquette test per
+// site from those angles and stores one 64-bit ballot word per wave (a
+// bitmask of flagged sites) plus a count per block; pass 2 scans the block
+// counts; pass 3 re-reads the ballot words and writes the coordinates at
+// their ordered positions.  Passes 0-2 are skipped when the same member's
+// unchanged state was searched by the previous call.
 #include "axs_internal.hpp"
 #include "axs_lattice.hpp"
 
@@ -22,12 +26,16 @@ constexpr int kStrIters = 8;                    // sites per thread per block
 constexpr int kStrSpan = kStrBlock * kStrIters; // sites per block
 
 template <class P>
-__device__ __forceinline__ double corner_angle(const P &phi, uint32_t q)
+__global__ void __launch_bounds__(256)
+angle_kernel(uint32_t L, P phi, double *__restrict__ ang)
 {
-    double a1, a2;
-    phi.get(q, a1, a2);
-    double a = atan2(a1, a2);
-    return (a + M_PI) / (2 * M_PI);
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < L) {
+        double a1, a2;
+        phi.get(m, a1, a2);
+        double a = atan2(a1, a2);
+        ang[m] = (a + M_PI) / (2 * M_PI);
+    }
 }
 
 // std::min(|b-a|, 1-|b-a|), string_finding.cpp:22-24
@@ -38,8 +46,9 @@ __device__ __forceinline__ double edge_theta(double a, double b)
     return (e < d) ? e : d;
 }
 
-template <int ND, class P>
-__device__ __forceinline__ bool plaquette_flag(const P &phi, uint32_t m, int N, double accept)
+template <int ND>
+__device__ __forceinline__ bool plaquette_flag(const double *__restrict__ ang, uint32_t m, int N,
+                                               double accept)
 {
     Site<ND> s = decompose<ND>(m, N);
     if (ND == 2) {
@@ -48,10 +57,10 @@ __device__ __forceinline__ bool plaquette_flag(const P &phi, uint32_t m, int N, 
         Site<ND> s2 = decompose<ND>(q2, N);
         uint32_t q3 = shifted<ND>(s2, 1, 1, N);
         uint32_t q4 = shifted<ND>(s, 1, 1, N);
-        double n1 = corner_angle(phi, m);
-        double n2 = corner_angle(phi, q2);
-        double n3 = corner_angle(phi, q3);
-        double n4 = corner_angle(phi, q4);
+        double n1 = ang[m];
+        double n2 = ang[q2];
+        double n3 = ang[q3];
+        double n4 = ang[q4];
         double sum = edge_theta(n1, n2) + edge_theta(n2, n3) + edge_theta(n3, n4);
         return sum > accept;
     } else {
@@ -63,12 +72,12 @@ __device__ __forceinline__ bool plaquette_flag(const P &phi, uint32_t m, int N, 
         Site<ND> sj = decompose<ND>(qj, N);
         uint32_t qjk = shifted<ND>(sj, 2, 1, N);                   // (i,   j+1, k+1)
         uint32_t qk = shifted<ND>(s, 2, 1, N);                     // (i,   j,   k+1)
-        double n0 = corner_angle(phi, m);
-        double ni = corner_angle(phi, qi);
-        double nij = corner_angle(phi, qij);
-        double nj = corner_angle(phi, qj);
-        double njk = corner_angle(phi, qjk);
-        double nk = corner_angle(phi, qk);
+        double n0 = ang[m];
+        double ni = ang[qi];
+        double nij = ang[qij];
+        double nj = ang[qj];
+        double njk = ang[qjk];
+        double nk = ang[qk];
         double sa = edge_theta(n0, ni) + edge_theta(ni, nij) + edge_theta(nij, nj);
         double sb = edge_theta(n0, ni) + edge_theta(ni, njk) + edge_theta(njk, nk);
         double sc = edge_theta(n0, nj) + edge_theta(nj, njk) + edge_theta(njk, nk);
@@ -76,10 +85,10 @@ __device__ __forceinline__ bool plaquette_flag(const P &phi, uint32_t m, int N, 
     }
 }
 
-template <int ND, class P>
+template <int ND>
 __global__ void __launch_bounds__(kStrBlock)
-strings_flag_kernel(int N, uint32_t L, P phi, double accept, uint64_t *__restrict__ words,
-                    uint32_t *__restrict__ bcount)
+strings_flag_kernel(int N, uint32_t L, const double *__restrict__ ang, double accept,
+                    uint64_t *__restrict__ words, uint32_t *__restrict__ bcount)
 {
     __shared__ uint32_t wave_tot[kStrBlock / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -87,7 +96,7 @@ strings_flag_kernel(int N, uint32_t L, P phi, double accept, uint64_t *__restric
     uint32_t base = blockIdx.x * (uint32_t)kStrSpan;
     for (int it = 0; it < kStrIters; ++it) {
         uint32_t m = base + it * kStrBlock + threadIdx.x;
-        bool f = (m < L) && plaquette_flag<ND, P>(phi, m, N, accept);
+        bool f = (m < L) && plaquette_flag<ND>(ang, m, N, accept);
         uint64_t w = __ballot(f);
         uint32_t wi = (base + it * kStrBlock) / 64 + wave;
         if (lane == 0 && (uint64_t)wi * 64 < L) words[wi] = w;
@@ -191,39 +200,57 @@ static double accept_threshold(int thr)
 int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *count,
                 int64_t *num_cores)
 {
-    FieldView fv;
-    AXS_TRY(current_fields(c, m, &fv));
+    if (!m.has_state) {
+        set_error("member has no state: call axs_set_member_state first");
+        return AXS_ERR_STATE;
+    }
+    const int member = (int)(&m - c->members.data());
     uint32_t L = (uint32_t)m.L;
     uint32_t nblocks = (L + kStrSpan - 1) / kStrSpan;
-    uint32_t nwords = (L + 63) / 64;
     size_t words_b = ((size_t)nblocks * kStrSpan / 64) * 8;
     size_t cnt_b = ((size_t)nblocks * 4 + 255) & ~(size_t)255;
     size_t off_b = ((size_t)(nblocks + 1) * 8 + 255) & ~(size_t)255;
     size_t tail_b = 256;
-    (void)nwords;
-    AXS_TRY(ensure_scratch(c, words_b + cnt_b + off_b + tail_b));
-    char *sp = (char *)c->scratch;
+    const bool cached = c->str_member == member && c->str_version == m.version && c->strbuf &&
+                        c->strbuf_bytes >= words_b + cnt_b + off_b + tail_b;
+    if (!cached) {
+        c->str_member = -1;
+        AXS_TRY(ensure_device(c, &c->strbuf, &c->strbuf_bytes, words_b + cnt_b + off_b + tail_b));
+    }
+    char *sp = (char *)c->strbuf;
     uint64_t *words = (uint64_t *)sp;
     uint32_t *bcount = (uint32_t *)(sp + words_b);
     uint64_t *off = (uint64_t *)(sp + words_b + cnt_b);
     unsigned long long *pairs = (unsigned long long *)(sp + words_b + cnt_b + off_b);
 
-    double accept = accept_threshold(c->p.thr);
-    if (c->prec == AXS_FP64) {
-        auto k = (c->nd == 2) ? strings_flag_kernel<2, PhiF64> : strings_flag_kernel<3, PhiF64>;
-        hipLaunchKernelGGL(k, dim3(nblocks), dim3(kStrBlock), 0, c->stream, m.N, L,
-                           PhiF64{(const dvec2 *)fv.phi}, accept, words, bcount);
-    } else {
-        auto k = (c->nd == 2) ? strings_flag_kernel<2, PhiF32> : strings_flag_kernel<3, PhiF32>;
-        hipLaunchKernelGGL(k, dim3(nblocks), dim3(kStrBlock), 0, c->stream, m.N, L,
-                           PhiF32{(const fvec4 *)fv.phi}, accept, words, bcount);
-    }
-    hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, c->stream, bcount, off,
-                       nblocks);
-    AXS_HIP_TRY(hipGetLastError());
     uint64_t total = 0;
-    AXS_HIP_TRY(hipMemcpyAsync(&total, off + nblocks, 8, hipMemcpyDeviceToHost, c->stream));
-    AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+    if (cached) {
+        total = c->str_total;
+    } else {
+        // the angles need phi only (no FP32 velocity reconstruction)
+        const void *phi = m.state[c->cur][PHI];
+        AXS_TRY(ensure_scratch(c, (size_t)L * sizeof(double)));
+        double *ang = (double *)c->scratch;
+        const unsigned ga = (L + 255) / 256;
+        if (c->prec == AXS_FP64)
+            hipLaunchKernelGGL(angle_kernel<PhiF64>, dim3(ga), dim3(256), 0, c->stream, L,
+                               PhiF64{(const dvec2 *)phi}, ang);
+        else
+            hipLaunchKernelGGL(angle_kernel<PhiF32>, dim3(ga), dim3(256), 0, c->stream, L,
+                               PhiF32{(const fvec4 *)phi}, ang);
+        double accept = accept_threshold(c->p.thr);
+        auto k = (c->nd == 2) ? strings_flag_kernel<2> : strings_flag_kernel<3>;
+        hipLaunchKernelGGL(k, dim3(nblocks), dim3(kStrBlock), 0, c->stream, m.N, L, ang, accept,
+                           words, bcount);
+        hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, c->stream, bcount, off,
+                           nblocks);
+        AXS_HIP_TRY(hipGetLastError());
+        AXS_HIP_TRY(hipMemcpyAsync(&total, off + nblocks, 8, hipMemcpyDeviceToHost, c->stream));
+        AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+        c->str_member = member;
+        c->str_version = m.version;
+        c->str_total = total;
+    }
     *count = (int64_t)total;
 
     bool need_list = (coords != nullptr && cap > 0) || (num_cores != nullptr && c->nd == 2);

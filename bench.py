@@ -83,6 +83,39 @@ def gpu_leg(np, axs, N, precision, warmup, steps, T0, depth=0):
     return lat, res
 
 
+def sampling_leg(np, lat, N, reps=3):
+    """Time of each sampling call at a sampling point, as a run makes them
+    (median over `reps` sampling points, one step apart, each call
+    synchronous and in this order), with the effective HBM rate on the
+    call's algorithmic bytes: strings read phi (FP32 8 B/site, FP64 16),
+    background and energy read phi and phidot (16 / 32 B/site; the first of
+    them in the FP32 form also rebuilds phidot and ker), the power spectrum
+    is FFT-bound (rate on one read of phi2).  The coordinate list reuses the
+    count's search of the same state."""
+    L = N ** 3
+    w = 4 if lat.precision == "fp32" else 8
+    calls = {
+        "string_counts": (lambda: lat.string_counts(), 2 * w),
+        "strings_with_coords": (lambda: lat.strings(), 2 * w),
+        "background": (lambda: lat.background(), 4 * w),
+        "energy": (lambda: lat.energy(), 4 * w),
+        "powerspec_log64": (lambda: lat.powerspec(binning="log", nbins=64), w),
+        "powerspec_linear": (lambda: lat.powerspec(binning="linear"), w),
+    }
+    times = {name: [] for name in calls}
+    for _ in range(reps):
+        lat.step(1)               # a new state: nothing cached
+        for name, (fn, _) in calls.items():
+            t0 = time.perf_counter()
+            fn()
+            times[name].append(time.perf_counter() - t0)
+    out = {}
+    for name, (_, bps) in calls.items():
+        t = sorted(times[name])[len(times[name]) // 2]
+        out[name] = {"ms": 1e3 * t, "alg_GBs": bps * L / t / 1e9}
+    return out
+
+
 def accuracy(np, lat32, lat64):
     s32 = lat32.get_state(fields=("phi1", "phi2"))
     s64 = lat64.get_state(fields=("phi1", "phi2"))
@@ -303,6 +336,8 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (for calibrating HBM "
                          "traffic counters) and exit")
+    ap.add_argument("--skip-sampling", action="store_true",
+                    help="skip timing the sampling calls (strings, reductions, spectra)")
     ap.add_argument("--skip-snapshot", action="store_true",
                     help="skip the asynchronous-snapshot overlap leg")
     ap.add_argument("--skip-forms", action="store_true",
@@ -338,10 +373,15 @@ def main() -> int:
     lat32, r32 = gpu_leg(np, axs, N, "fp32", W, K, args.T_initial, args.depth)
     out64 = None
     acc = None
+    sampling = None
     if not args.skip_fp64 and "fp64" in forms:
         lat64, r64 = gpu_leg(np, axs, N, "fp64", W, K, args.T_initial, args.depth)
         log("[bench] comparing FP32 with FP64")
         acc = accuracy(np, lat32, lat64)
+        if not args.skip_sampling:
+            sampling = {"fp32": sampling_leg(np, lat32, N), "fp64": sampling_leg(np, lat64, N)}
+            log("[bench] sampling: " + ", ".join(
+                f"{k} {v['ms']:.1f} ms" for k, v in sampling["fp32"].items()))
         lat64.close()
         out64 = {
             "value": r64["site_updates_per_s"],
@@ -434,6 +474,7 @@ def main() -> int:
         "fp64": out64,
         "fp32_speedup_over_fp64": (r32["site_updates_per_s"] / out64["value"]) if out64 else None,
         "accuracy_fp32_vs_fp64": acc,
+        "sampling": sampling,
         "ensembles": ens,
         "forms": forms_out,
         "snapshot": snap,

@@ -32,7 +32,8 @@ def test_strerror():
 
 
 @pytest.mark.parametrize("field,value", [("ndims", 4), ("N", 7), ("N", 2), ("stencil_setting", 5),
-                                         ("precision", 9), ("time_step", 0.0)])
+                                         ("precision", 9), ("time_step", 0.0), ("eom", 2),
+                                         ("laplacian", 3), ("eom", -1)])
 def test_invalid_params_rejected(field, value):
     lib = axs.load_library()
     p = axs.AxsParams(2, 16, 0.5, 0.1, 1.0, 4.0, 0, 1, 0, 0)
@@ -59,3 +60,14 @@ def test_batch_rejects_bad_member_size():
     Ns = (ctypes.c_int * 3)(16, 33, 64)
     ctx = ctypes.c_void_p()
     assert lib.axs_create_batch(ctypes.byref(p), 3, Ns, ctypes.byref(ctx)) == axs.AXS_ERR_INVALID_ARG
+
+
+@pytest.mark.parametrize("precision,eom,lap", [(1, 1, 0), (1, 0, 1), (0, 1, 1)])
+def test_unsupported_form_combinations_rejected(precision, eom, lap):
+    # the Hamiltonian form and the spectral Laplacian are FP64 paths of the
+    # reference, and it has no Hamiltonian step with the spectral Laplacian
+    lib = axs.load_library()
+    p = axs.AxsParams(2, 16, 0.5, 0.1, 1.0, 4.0, 0, 1, precision, 0, eom, lap)
+    ctx = ctypes.c_void_p()
+    assert lib.axs_create(ctypes.byref(p), ctypes.byref(ctx)) == axs.AXS_ERR_UNSUPPORTED
+    assert not ctx
