@@ -331,6 +331,15 @@ static int validate_params(const axs_params *p, int N)
     return AXS_OK;
 }
 
+// steps per pass of the higher-order stencils' multi-step kernels unless
+// configured otherwise (measured best at 512^3 and 4096^2): 3D FP32 two
+// (radius 4: one), FP64 one; 2D FP32 four at radius 2, else two, FP64 two
+static int high_order_default_depth(int nd, int prec, int stencil)
+{
+    if (nd == 2) return (prec == AXS_FP32 && stencil == 1) ? 4 : 2;
+    return (prec == AXS_FP32 && stencil < 3) ? 2 : 1;
+}
+
 static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ctx **out)
 {
     if (!p || !out || n_members < 1 || !Ns) {
@@ -411,20 +420,24 @@ static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ct
         set_error("stream synchronize failed after allocation");
         return fail(AXS_ERR_HIP);
     }
-    // standard stencil: several steps per pass unless configured otherwise
-    // (the fastest shapes measured): FP32 3D four steps from side 256 on,
-    // three below, 16-wave work-groups; FP64 3D three steps, 8-wave
-    // work-groups, two rows per wave; 2D six steps in both forms
+    // several steps per pass unless configured otherwise (the fastest
+    // shapes measured).  Standard stencil: FP32 3D four steps from side 256
+    // on, three below, 16-wave work-groups; FP64 3D three steps, 8-wave
+    // work-groups, two rows per wave; 2D six steps in both forms.  Higher
+    // orders: high_order_default_depth.
     int min_side = 1 << 30;
     for (const Member &m : c->members) min_side = std::min(min_side, m.N);
-    if (c->nd == 2)
+    if (c->p.stencil_setting > 0)
+        c->tune.depth = high_order_default_depth(c->nd, c->prec, c->p.stencil_setting);
+    else if (c->nd == 2)
         c->tune.depth = 6;
     else
         c->tune.depth = (c->prec == AXS_FP32) ? ((min_side >= 256) ? 4 : 3) : 3;
-    if (multi_step_depth(c) == 1)
+    if (c->nd == 3) c->tune.flags |= (c->prec == AXS_FP32) ? kTuneWG1024 : kTuneWG512;
+    if (multi_depth(c) == 0) {
         c->tune.depth = 1;
-    else if (c->nd == 3)
-        c->tune.flags |= (c->prec == AXS_FP32) ? kTuneWG1024 : kTuneWG512;
+        c->tune.flags &= ~(kTuneWG512 | kTuneWG1024);
+    }
     (void)rc;
     *out = h;
     return AXS_OK;
@@ -647,11 +660,12 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
     else
         AXS_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     int rc = AXS_OK;
-    const int depth = multi_step_depth(c);
+    // multi-step passes of `depth` steps; a remainder runs one step at a time
+    const int depth = multi_depth(c);
     for (int64_t n = 0; n < nsteps && rc == AXS_OK;) {
         double tau_next = c->tau + dt;                    // s_integrate.cpp:317
         StepScalars s = make_scalars(c->p, tau_next);
-        if (depth > 1 && nsteps - n >= depth) {
+        if (depth >= 1 && nsteps - n >= depth) {
             StepScalars ss[kMaxDepth];
             double t = c->tau;
             for (int l = 0; l < depth; ++l) {
@@ -792,30 +806,48 @@ int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
     }
     Context *c = &ctx->impl;
     const int depth = cfg->depth ? cfg->depth : 1;
+    const int RAD = stencil_radius_of(c);
+    const int wg = cfg->flags & (AXS_MARCH_WG512 | AXS_MARCH_WG1024);
+    // the multi-step kernels' work-group size (3D): configured, else the
+    // built one of the higher orders, else 4 waves
+    const int W = (RAD > 1 && !wg) ? ((c->prec == AXS_FP64) ? 8 : 16) : multi_waves(cfg->flags);
     bool rows_ok;
     if (depth >= 2 && c->nd == 2) {
-        // several steps per pass, 2D: 5-point stencil, one strip per wave
-        rows_ok = cfg->rows == 0 && (cfg->flags & (AXS_MARCH_WG512 | AXS_MARCH_WG1024)) == 0;
-        bool fits = cfg->kernel != AXS_KERNEL_SIMPLE && march2d_depth_built(depth);
+        // several steps per pass, 2D: one strip per wave
+        rows_ok = cfg->rows == 0 && wg == 0;
+        bool fits = cfg->kernel != AXS_KERNEL_SIMPLE && march2d_depth_built(depth, RAD, c->prec);
         for (const Member &m : c->members) fits = fits && march2d_supported(c, m.N);
         if (!fits) {
-            set_error("depth %d in 2D needs stencil 0, the stencil Laplacian, sides >= 64, "
-                      "a non-simple kernel and depth 2, 3, 4, 6 or 8", depth);
+            set_error("depth %d in 2D needs the stencil Laplacian (field-rescaled form for "
+                      "stencil_setting > 0), sides >= 64, a non-simple kernel and a built "
+                      "depth (stencil 0: 2, 3, 4, 6, 8; higher orders: 1-4 FP32, 1-2 FP64)",
+                      depth);
             return AXS_ERR_UNSUPPORTED;
         }
     } else if (depth >= 2) {
-        // several steps per pass: FP32 3D standard stencil on every member
-        const int W = multi_waves(cfg->flags);
-        const int R = cfg->rows ? cfg->rows : (W == 4 ? 4 : (W == 8 ? 3 : 2));
-        rows_ok = !((cfg->flags & AXS_MARCH_WG512) && (cfg->flags & AXS_MARCH_WG1024)) &&
-                  multi_shape_built(depth, W, R, c->prec);
-        bool fits = cfg->kernel != AXS_KERNEL_SIMPLE && depth <= 4;
+        // several steps per pass, 3D, on every member
+        const int R = cfg->rows ? cfg->rows
+                                : (RAD > 1 ? high_order_rows(c->prec, RAD)
+                                           : (W == 4 ? 4 : (W == 8 ? 3 : 2)));
+        rows_ok = wg != (AXS_MARCH_WG512 | AXS_MARCH_WG1024) &&
+                  multi_shape_built(depth, W, R, c->prec, RAD);
+        bool fits = cfg->kernel != AXS_KERNEL_SIMPLE && depth <= (RAD > 1 ? 2 : 4);
         for (const Member &m : c->members) fits = fits && march2_supported(c, m.N);
         if (!fits) {
-            set_error("depth %d needs 3D, stencil 0, sides >= 64, a non-simple kernel and "
-                      "depth <= 4", depth);
+            set_error("depth %d needs 3D, the stencil Laplacian (field-rescaled form for "
+                      "stencil_setting > 0), sides >= 64, a non-simple kernel and depth <= 4 "
+                      "(stencil 0) or <= 2 (higher orders)", depth);
             return AXS_ERR_UNSUPPORTED;
         }
+    } else if (RAD > 1) {
+        // one step per pass: the multi-step kernel where it applies, with
+        // its built shape (3D) or one strip per wave (2D)
+        if (c->nd == 2)
+            rows_ok = cfg->rows == 0 && wg == 0;
+        else
+            rows_ok = wg != (AXS_MARCH_WG512 | AXS_MARCH_WG1024) &&
+                      multi_shape_built(1, W, cfg->rows ? cfg->rows : high_order_rows(c->prec, RAD),
+                                        c->prec, RAD);
     } else {
         rows_ok = cfg->rows == 0 || cfg->rows == 4 || (cfg->rows == 8 && c->prec == AXS_FP32) ||
                   (cfg->rows == 2 && c->prec == AXS_FP64);
@@ -832,7 +864,7 @@ int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
             return AXS_ERR_INVALID_ARG;
         }
     c->tune.kernel = cfg->kernel;
-    if (depth >= 2)
+    if (depth >= 2 || RAD > 1)
         c->tune.rtb = cfg->rows;
     else if (c->prec == AXS_FP32)
         c->tune.rj32 = cfg->rows ? cfg->rows : 8;
@@ -853,7 +885,7 @@ int axs_get_kernel_config(const axs_ctx *ctx, axs_kernel_config *cfg)
     }
     const Context *c = &ctx->impl;
     cfg->kernel = c->tune.kernel;
-    cfg->rows = (c->tune.depth >= 2) ? c->tune.rtb
+    cfg->rows = (c->tune.depth >= 2 || c->p.stencil_setting > 0) ? c->tune.rtb
                                      : ((c->prec == AXS_FP32) ? c->tune.rj32 : c->tune.rj64);
     cfg->chunk = (c->prec == AXS_FP32) ? c->tune.ci32 : c->tune.ci64;
     cfg->depth = c->tune.depth;

@@ -122,7 +122,8 @@ struct Tune {
     int rj64 = 4;      // rows per work-item, FP64 march
     int ci32 = 0;      // planes per chunk, 0 = automatic
     int ci64 = 0;
-    int depth = 1;     // time steps per pass over the lattice: 1, or 2-4 (FP32 3D)
+    int depth = 1;     // time steps per pass of the multi-step kernels (1: the
+                       // one-step kernels for the 7/5-point stencil)
     int rtb = 0;       // rows per work-item of the multi-step kernel, 0 = by side
     int flags = kTunePrefetch | kTuneNT;
 };
@@ -213,11 +214,21 @@ int launch_step(Context *c, const StepScalars &s);
 bool march3d_supported(const Context *c, int N);
 int launch_march3d(Context *c, Member &m, int src, int dst, const StepScalars &s);
 bool march2_supported(const Context *c, int N);
-// time steps per launch the configuration gives every member (1 to 4)
+// steps per launch of the multi-step kernels for this configuration, or 0
+// when the one-step kernels run (radius 1 at depth 1, simple kernel, ...)
+int multi_depth(const Context *c);
+// time steps per launch the configuration gives every member (>= 1)
 int multi_step_depth(const Context *c);
-// the multi-step kernel is built for depth D, W waves, R rows per wave
-bool multi_shape_built(int D, int W, int R, int prec);
+// the multi-step kernel is built for depth D, W waves, R rows per wave,
+// form `prec` and stencil radius RAD
+bool multi_shape_built(int D, int W, int R, int prec, int RAD);
 int multi_waves(int flags);
+int multi_waves_of(const Context *c);
+int multi_rows(const Context *c, int N, int D, int W);
+int high_order_rows(int prec, int RAD);
+int stencil_radius_of(const Context *c);
+// higher-order stencils (radius 2-4), 3D (axs_march2_hi.hip)
+int launch_multi_high_order(Context *c, int src, int dst, const StepScalars *s, int depth);
 // `depth` steps (scalars s[0..depth-1] of tau+dt, tau+2dt, ...) in one pass
 // per member
 int launch_multi_steps(Context *c, const StepScalars *s, int depth);
@@ -225,9 +236,11 @@ int launch_multi_steps(Context *c, const StepScalars *s, int depth);
 int prepare_multi_batch(Context *c);
 // 2D multi-step kernel (axs_march2d.hip)
 bool march2d_supported(const Context *c, int N);
-bool march2d_depth_built(int D);
+bool march2d_depth_built(int D, int RAD, int prec);
 int prepare_march2d_items(Context *c);
 int launch_march2d(Context *c, int src, const StepScalars *s, int depth);
+// higher-order stencils (radius 2-4), 2D (axs_march2d_hi.hip)
+int launch_march2d_high_order(Context *c, int src, const StepScalars *s, int depth, bool pf);
 // field-rescaled step with the spectral Laplacian, one member
 int run_spectral_step(Context *c, Member &m, int src, int dst, const StepScalars &s);
 // Laplacian of the current field (AXS_LAPLACIAN_*) into host arrays

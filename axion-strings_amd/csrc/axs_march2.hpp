@@ -1,0 +1,380 @@
+// axs_march2.hpp -- the 3D multi-step (temporally blocked) step kernels,
+// templated on the storage form, the steps per pass D, the work-group shape
+// (W waves of R rows) and the stencil radius RAD (stencil_setting + 1).
+//
+// A work-group of W waves covers ER = W*R rows (j) of 64 consecutive k and
+// marches along i over a chunk of CI planes.  Level 0 is the state loaded
+// from HBM; level l (1..D) is the lattice after l steps, computed on the
+// loaded region minus an l*RAD-site rim, so the work-group writes an
+// (ER - 2 D RAD) x (64 - 2 D RAD) output tile and reads a D*RAD-site halo on
+// every side (re-read by the neighbouring tiles).  Per plane i:
+//
+//   load the state of plane i + D RAD and drift it           (level 0 -> x_1)
+//   for l = 1..D: at plane q = i + (D - l) RAD, one step of the form
+//      (axs_forms.hpp) from x_l and the site's own data
+//   write the state of plane i
+//
+// Each level keeps x_l of planes q-RAD .. q+RAD-1 in registers (plane
+// q+RAD is the value the level below has just produced) -- the i neighbours
+// -- and the own data of planes q .. q+RAD-1 (produced RAD planes before
+// they are used); x_l of plane q sits in one LDS plane per level for the j
+// and k neighbours, written after a barrier that ends the plane's reads and
+// read after a second barrier.  The Laplacian sums in the one-step kernels'
+// order -- centre, then for each distance m: +-i, +-j, +-k, each times the
+// stencil coefficient -- with no contraction, so a pass is bit-identical to
+// D one-step launches.  Tiles at the lattice edge overhang it: loads wrap
+// periodically and stores are masked.
+#pragma once
+
+#include <type_traits>
+
+#include "axs_forms.hpp"
+#include "axs_internal.hpp"
+#include "axs_lattice.hpp"
+
+namespace axs {
+
+constexpr int kTBW = 64;   // columns loaded per row (one wave)
+
+struct TBGeom {
+    int N, CI, nTj, nTk, remap;
+};
+
+template <class T, int D, int W, int R>
+using TBLds = T[D][2][W * R * kTBW];   // x_l of the plane level l works on
+
+// One work-group: tile (tj, tk) of the lattice of side N, planes
+// [ch*CI, ch*CI + CI).
+template <class Tr, int D, int W, int R, int RAD, bool PF>
+__device__ __forceinline__ void marchd_tile(const int N, const int CI, const int tj, const int tk,
+                                            const int ch, const typename Tr::In &in,
+                                            const typename Tr::Out &out,
+                                            const LevelScalars<typename Tr::Sc, D> &sc,
+                                            TBLds<typename Tr::T, D, W, R> &X)
+{
+    using T = typename Tr::T;
+    using Own = typename Tr::Own;
+    using Raw = typename Tr::Raw;
+    constexpr int ER = W * R;            // rows loaded
+    constexpr int H = D * RAD;           // halo width
+    constexpr int TJ = ER - 2 * H;       // rows written
+    constexpr int TK = kTBW - 2 * H;     // columns written
+    static_assert(TJ > 0 && TK > 0, "tile shape");
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t NN = (uint32_t)N * (uint32_t)N;
+    const int j0 = tj * TJ, k0 = tk * TK, i0 = ch * CI;
+
+    // column of this lane, wrapped: -H <= k0 - H + lane < 2N for N >= 64
+    int kk = k0 - H + lane;
+    kk = (kk < 0) ? kk + N : ((kk >= N) ? kk - N : kk);
+    const bool colout = lane >= H && lane < kTBW - H && (k0 + lane - H) < N;
+
+    uint32_t rowoff[R];
+    int erow[R];
+    bool rowout[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int e = wave * R + r;                 // row within the loaded region
+        int j = j0 - H + e;                         // -H <= j < 2N (ER <= 64 <= N)
+        j = (j < 0) ? j + N : ((j >= N) ? j - N : j);
+        rowoff[r] = (uint32_t)j * (uint32_t)N + (uint32_t)kk;
+        erow[r] = e;
+        rowout[r] = e >= H && e < ER - H && (j0 + e - H) < N;
+    }
+    auto plane = [&](int i) -> uint32_t {
+        int w = (i < 0) ? i + N : ((i >= N) ? i - N : i);
+        return (uint32_t)w * NN;
+    };
+    auto load = [&](int i, Raw (&raw)[R]) {
+        const uint32_t po = plane(i);
+#pragma unroll
+        for (int r = 0; r < R; ++r) Tr::fetch(in, po + rowoff[r], raw[r]);
+    };
+    // site (row e, this lane) lies in level l's region
+    auto inside = [&](int e, int l) -> bool {
+        return e >= l * RAD && e < ER - l * RAD && lane >= l * RAD && lane < kTBW - l * RAD;
+    };
+
+    // per level l = 1..D (index l-1): x_l of planes q-RAD .. q+RAD-1 (ring)
+    // and the own data of planes q .. q+RAD-1 (fifo)
+    T ring[D][2 * RAD][R][2];
+    Own fifo[D][RAD][R];
+#pragma unroll
+    for (int l = 0; l < D; ++l)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+#pragma unroll
+            for (int p = 0; p < 2 * RAD; ++p) ring[l][p][r][0] = ring[l][p][r][1] = (T)0;
+#pragma unroll
+            for (int p = 0; p < RAD; ++p) fifo[l][p][r] = Own{};
+        }
+    Raw raw[R], pf[R];
+
+    if (PF) load(i0 - H, pf);
+    for (int it = -2 * H; it < CI; ++it) {
+        const int i = i0 + it;
+        // level 0: plane i+H
+        if (PF) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) raw[r] = pf[r];
+            if (it + 1 < CI) load(i + H + 1, pf);
+        } else {
+            load(i + H, raw);
+        }
+        // values handed up the levels: x_l and the own data of plane q+RAD
+        T fx[R][2];
+        Own fo[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) Tr::drift0(raw[r], sc.s[0], fx[r][0], fx[r][1], fo[r]);
+#pragma unroll
+        for (int l = 1; l <= D; ++l) {
+            const typename Tr::Sc &s = sc.s[l - 1];
+            const typename Tr::Sc &sn = sc.s[l < D ? l : D - 1];
+            const T *L1 = X[l - 1][0], *L2 = X[l - 1][1];
+            const bool active = it >= 2 * RAD * (l - D);   // x_l of planes q-RAD .. q+RAD exist
+            T nx[R][2];
+            Own no[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                nx[r][0] = nx[r][1] = (T)0;
+                no[r] = Own{};
+            }
+            if (active) {
+                const uint32_t po = plane(i + (D - l) * RAD);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (!inside(erow[r], l)) continue;
+                    const int o = erow[r] * kTBW + lane;
+                    const T x = ring[l - 1][RAD][r][0], y = ring[l - 1][RAD][r][1];
+                    T a = s.c[0] * x, c = s.c[0] * y;
+#pragma unroll
+                    for (int m = 1; m <= RAD; ++m) {
+                        // the 7-point stencil's coefficient is 1: its
+                        // products are exact and left out
+                        auto term = [&](T v) -> T { return (RAD == 1) ? v : (T)s.c[m] * v; };
+                        const int up = (RAD + m < 2 * RAD) ? RAD + m : 2 * RAD - 1;
+                        const T ux = (m == RAD) ? fx[r][0] : ring[l - 1][up][r][0];
+                        const T uy = (m == RAD) ? fx[r][1] : ring[l - 1][up][r][1];
+                        a = a + term(ux);
+                        c = c + term(uy);                                   // i+m
+                        a = a + term(ring[l - 1][RAD - m][r][0]);
+                        c = c + term(ring[l - 1][RAD - m][r][1]);           // i-m
+                        a = a + term(L1[o + m * kTBW]);
+                        c = c + term(L2[o + m * kTBW]);                     // j+m
+                        a = a + term(L1[o - m * kTBW]);
+                        c = c + term(L2[o - m * kTBW]);                     // j-m
+                        a = a + term(L1[o + m]);
+                        c = c + term(L2[o + m]);                            // k+m
+                        a = a + term(L1[o - m]);
+                        c = c + term(L2[o - m]);                            // k-m
+                    }
+                    Own on;
+                    T ex, ey;
+                    Tr::level(x, y, a, c, fifo[l - 1][0][r], s, sn, on, ex, ey);
+                    if (l == D) {
+                        if (it >= 0 && rowout[r] && colout)
+                            Tr::store(out, po + rowoff[r], x, y, on);
+                    } else {
+                        nx[r][0] = ex;
+                        nx[r][1] = ey;
+                        no[r] = on;
+                    }
+                }
+            }
+            // shift level l by one plane: the value handed up enters at the top
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+#pragma unroll
+                for (int p = 0; p + 1 < 2 * RAD; ++p) {
+                    ring[l - 1][p][r][0] = ring[l - 1][p + 1][r][0];
+                    ring[l - 1][p][r][1] = ring[l - 1][p + 1][r][1];
+                }
+                ring[l - 1][2 * RAD - 1][r][0] = fx[r][0];
+                ring[l - 1][2 * RAD - 1][r][1] = fx[r][1];
+#pragma unroll
+                for (int p = 0; p + 1 < RAD; ++p) fifo[l - 1][p][r] = fifo[l - 1][p + 1][r];
+                fifo[l - 1][RAD - 1][r] = fo[r];
+            }
+            if (l < D) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    fx[r][0] = nx[r][0];
+                    fx[r][1] = nx[r][1];
+                    fo[r] = no[r];
+                }
+            }
+        }
+        __syncthreads();   // every read of this plane's LDS values is done
+        // each level's next plane (q+1) is now the middle of its ring
+#pragma unroll
+        for (int l = 0; l < D; ++l) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int o = erow[r] * kTBW + lane;
+                X[l][0][o] = ring[l][RAD][r][0];
+                X[l][1][o] = ring[l][RAD][r][1];
+            }
+        }
+        __syncthreads();   // the next plane's values are visible
+    }
+}
+
+template <class Tr, int D, int W, int R, int RAD, bool PF>
+__global__ void __launch_bounds__(64 * W)
+marchd_kernel(TBGeom g, typename Tr::In in, typename Tr::Out out,
+              LevelScalars<typename Tr::Sc, D> sc)
+{
+    __shared__ TBLds<typename Tr::T, D, W, R> X;
+    uint32_t b = blockIdx.x;
+    if (g.remap) b = (b & 7u) * (gridDim.x >> 3) + (b >> 3);   // XCD-aware order
+    const int tk = (int)(b % (uint32_t)g.nTk);
+    b /= (uint32_t)g.nTk;
+    const int tj = (int)(b % (uint32_t)g.nTj);
+    const int ch = (int)(b / (uint32_t)g.nTj);
+    marchd_tile<Tr, D, W, R, RAD, PF>(g.N, g.CI, tj, tk, ch, in, out, sc, X);
+}
+
+// the form's array views from a member-table entry (device) or a member (host)
+template <class Tr>
+__host__ __device__ __forceinline__ typename Tr::In form_in(void *const *p)
+{
+    if constexpr (std::is_same<Tr, F32Form>::value)
+        return {(const fvec4 *)p[PHI]};
+    else
+        return {(const dvec2 *)p[PHI], (const dvec2 *)p[VEL], (const dvec2 *)p[KER]};
+}
+template <class Tr>
+__host__ __device__ __forceinline__ typename Tr::Out form_out(void *const *p)
+{
+    if constexpr (std::is_same<Tr, F32Form>::value)
+        return {(fvec4 *)p[PHI]};
+    else
+        return {(dvec2 *)p[PHI], (dvec2 *)p[VEL], (dvec2 *)p[KER]};
+}
+
+// Ensemble form: one persistent launch for every member.  Work items are
+// the (member, tile, chunk) triples, members in order, tiles k fastest;
+// work-group b takes items b, b + gridDim.x, ...  The first barrier of a
+// tile orders its LDS writes after the previous tile's reads.
+template <class Tr, int D, int W, int R, int RAD, bool PF>
+__global__ void __launch_bounds__(64 * W)
+marchd_batch_kernel(const BatchMember *__restrict__ mem, int n_members, uint32_t n_items,
+                    LevelScalars<typename Tr::Sc, D> sc)
+{
+    __shared__ TBLds<typename Tr::T, D, W, R> X;
+    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        int lo = 0, hi = n_members - 1;        // last member starting at or before item
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (mem[mid].item_begin <= item)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const BatchMember &m = mem[lo];
+        uint32_t t = item - m.item_begin;
+        const int tk = (int)(t % (uint32_t)m.nTk);
+        t /= (uint32_t)m.nTk;
+        const int tj = (int)(t % (uint32_t)m.nTj);
+        const int ch = (int)(t / (uint32_t)m.nTj);
+        marchd_tile<Tr, D, W, R, RAD, PF>(m.N, m.CI, tj, tk, ch, form_in<Tr>(m.in),
+                                          form_out<Tr>(m.out), sc, X);
+    }
+}
+
+// planes per chunk: the configured count if it divides N, else long enough
+// to amortise the halo planes and short enough to give the chip work
+// (measured best: 64 at 512^3 and 1024^3, 32 at 256^3, 8 at 128^3)
+inline int multi_chunk(const Context *c, int N)
+{
+    int ci = c->tune.ci32;
+    if (ci > 0 && N % ci == 0) return ci;
+    ci = (N >= 512) ? 64 : ((N >= 256) ? 32 : 8);
+    while (N % ci != 0) --ci;
+    return ci;
+}
+
+template <class Sc>
+inline Sc level_scalars(const StepScalars &s)
+{
+    if constexpr (std::is_same<Sc, StepScalarsF>::value)
+        return to_float(s);
+    else
+        return s;
+}
+
+template <class Tr, int D, int RAD>
+struct SingleLaunch {
+    Context *c;
+    int N;
+    typename Tr::In in;
+    typename Tr::Out out;
+    LevelScalars<typename Tr::Sc, D> sc;
+    bool pf;
+    template <int W, int R>
+    int run()
+    {
+        constexpr int TJ = W * R - 2 * D * RAD;
+        constexpr int TK = kTBW - 2 * D * RAD;
+        static_assert(TJ > 0 && W * R <= 64, "tile shape");
+        TBGeom g;
+        g.N = N;
+        g.nTj = (N + TJ - 1) / TJ;
+        g.nTk = (N + TK - 1) / TK;
+        g.CI = multi_chunk(c, N);
+        const uint32_t nb = (uint32_t)g.nTj * (uint32_t)g.nTk * (uint32_t)(N / g.CI);
+        g.remap = (nb % 8 == 0) ? 1 : 0;
+        if (pf)
+            hipLaunchKernelGGL((marchd_kernel<Tr, D, W, R, RAD, true>), dim3(nb), dim3(64 * W), 0,
+                               c->stream, g, in, out, sc);
+        else
+            hipLaunchKernelGGL((marchd_kernel<Tr, D, W, R, RAD, false>), dim3(nb), dim3(64 * W),
+                               0, c->stream, g, in, out, sc);
+        return AXS_OK;
+    }
+};
+
+template <class Tr, int D, int RAD>
+struct BatchLaunch {
+    Context *c;
+    int src;
+    LevelScalars<typename Tr::Sc, D> sc;
+    bool pf;
+    template <int W, int R>
+    int run()
+    {
+        auto kern = pf ? marchd_batch_kernel<Tr, D, W, R, RAD, true>
+                       : marchd_batch_kernel<Tr, D, W, R, RAD, false>;
+        int per_cu = 0;
+        AXS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, 0));
+        const uint32_t cap = (uint32_t)c->num_cu * (uint32_t)(per_cu > 0 ? per_cu : 1);
+        const uint32_t grid = c->mbatch_items < cap ? c->mbatch_items : cap;
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * W), 0, c->stream, c->mbatch_tab[src],
+                           (int)c->members.size(), c->mbatch_items, sc);
+        return AXS_OK;
+    }
+};
+
+// one multi-step pass of every member with work-group shape (W, R):
+// Dispatch(W, R, f) calls f.run<W, R>() for the built shapes
+template <class Tr, int D, int RAD, class Dispatch>
+int launch_members(Context *c, int src, int dst, const StepScalars *s, int W, int Rbatch,
+                   int (*rows_of)(const Context *, int, int, int), Dispatch dispatch)
+{
+    LevelScalars<typename Tr::Sc, D> sc;
+    for (int l = 0; l < D; ++l) sc.s[l] = level_scalars<typename Tr::Sc>(s[l]);
+    const bool pf = (c->tune.flags & kTunePrefetch) != 0;
+    if (c->members.size() > 1 && c->mbatch_tab[src]) {
+        BatchLaunch<Tr, D, RAD> f{c, src, sc, pf};
+        return dispatch(W, Rbatch, f);
+    }
+    for (Member &m : c->members) {
+        SingleLaunch<Tr, D, RAD> f{c, m.N, form_in<Tr>(m.state[src]), form_out<Tr>(m.state[dst]),
+                                   sc, pf};
+        AXS_TRY(dispatch(W, rows_of(c, m.N, D, W), f));
+    }
+    return AXS_OK;
+}
+
+}  // namespace axs

@@ -1,0 +1,217 @@
+// axs_march2d.hpp -- the 2D multi-step (temporally blocked) step kernels,
+// templated on the storage form, the steps per pass D and the stencil
+// radius RAD (stencil_setting + 1).
+//
+// Layout m = i + N*j (i contiguous).  One wave owns a strip of 64
+// consecutive columns i and marches along j over a chunk of CJ rows; level 0
+// is the state loaded from HBM, level l (1..D) the lattice after l steps,
+// computed on the strip minus an l*RAD-column rim, so the wave writes
+// 64 - 2 D RAD columns.  Per row j:
+//
+//   load the state of row j + D RAD and drift it           (level 0 -> x_1)
+//   for l = 1..D, at row q = j + (D - l) RAD: one step of the form
+//      (axs_forms.hpp)
+//   write the state of row j
+//
+// Each level keeps x_l of rows q-RAD .. q+RAD-1 and the site's own data
+// (FP32: u; FP64: phidot, ker) of rows q .. q+RAD-1 in registers; row q+RAD
+// is the value the level below has just produced (the j neighbours), and
+// the i neighbours are the neighbouring lanes, read with cross-lane
+// shuffles -- no LDS, no barriers, every wave independent.  The Laplacian
+// sums in the one-step kernels' order (centre, then for each distance m:
+// +i, -i, +j, -j, each times the stencil coefficient) with the same
+// expressions, so a pass is bit-identical to D one-step launches.  Work
+// items (member, strip, chunk) are dealt to waves from a persistent list;
+// strips at the lattice edge overhang it (columns wrap periodically, stores
+// are masked).
+#pragma once
+
+#include <type_traits>
+
+#include "axs_forms.hpp"
+#include "axs_internal.hpp"
+#include "axs_lattice.hpp"
+
+namespace axs {
+
+constexpr int kW2 = 64;           // columns per strip (one wave)
+constexpr int kWaves2D = 4;       // waves per work-group
+
+template <class Tr, int D, int RAD, bool PF>
+__device__ __forceinline__ void march2d_strip(const int N, const int CJ, const int ti,
+                                              const int ch, const typename Tr::In &in,
+                                              const typename Tr::Out &out,
+                                              const LevelScalars<typename Tr::Sc, D> &sc)
+{
+    using T = typename Tr::T;
+    using Own = typename Tr::Own;
+    using Raw = typename Tr::Raw;
+    constexpr int H = D * RAD;           // halo width
+    constexpr int TI = kW2 - 2 * H;      // columns written
+    static_assert(TI > 0, "strip shape");
+    const int lane = threadIdx.x & 63;
+    const int i0 = ti * TI, j0 = ch * CJ;
+    int ii = i0 - H + lane;              // -H <= ii < 2N for N >= 64
+    ii = (ii < 0) ? ii + N : ((ii >= N) ? ii - N : ii);
+    const bool colout = lane >= H && lane < kW2 - H && (i0 + lane - H) < N;
+
+    auto row = [&](int j) -> uint32_t {
+        int w = (j < 0) ? j + N : ((j >= N) ? j - N : j);
+        return (uint32_t)w * (uint32_t)N + (uint32_t)ii;
+    };
+
+    // per level l = 1..D (index l-1): x_l of rows q-RAD .. q+RAD-1 (ring)
+    // and the own data of rows q .. q+RAD-1 (fifo)
+    T ring[D][2 * RAD][2];
+    Own fifo[D][RAD];
+#pragma unroll
+    for (int l = 0; l < D; ++l) {
+#pragma unroll
+        for (int p = 0; p < 2 * RAD; ++p) ring[l][p][0] = ring[l][p][1] = (T)0;
+#pragma unroll
+        for (int p = 0; p < RAD; ++p) fifo[l][p] = Own{};
+    }
+    Raw raw, pf;
+    if (PF) Tr::fetch(in, row(j0 - H), pf);
+
+    for (int it = -2 * H; it < CJ; ++it) {
+        const int j = j0 + it;
+        if (PF) {
+            raw = pf;
+            if (it + 1 < CJ) Tr::fetch(in, row(j + H + 1), pf);
+        } else {
+            Tr::fetch(in, row(j + H), raw);
+        }
+        T fx[2];                           // x_l of row q+RAD, handed up
+        Own fo;                            // own data of row q+RAD
+        Tr::drift0(raw, sc.s[0], fx[0], fx[1], fo);
+#pragma unroll
+        for (int l = 1; l <= D; ++l) {
+            const typename Tr::Sc &s = sc.s[l - 1];
+            const typename Tr::Sc &sn = sc.s[l < D ? l : D - 1];
+            const bool active = it >= 2 * RAD * (l - D);   // x_l of rows q-RAD .. q+RAD exist
+            T nx[2] = {(T)0, (T)0};
+            Own no{};
+            if (active) {
+                const T x = ring[l - 1][RAD][0], y = ring[l - 1][RAD][1];
+                // i neighbours from the lanes m away (all lanes take part)
+                T xr[RAD], yr[RAD], xl[RAD], yl[RAD];
+#pragma unroll
+                for (int m = 1; m <= RAD; ++m) {
+                    xr[m - 1] = __shfl_down(x, m);
+                    yr[m - 1] = __shfl_down(y, m);
+                    xl[m - 1] = __shfl_up(x, m);
+                    yl[m - 1] = __shfl_up(y, m);
+                }
+                if (lane >= l * RAD && lane < kW2 - l * RAD) {
+                    T a = s.c[0] * x, c = s.c[0] * y;
+#pragma unroll
+                    for (int m = 1; m <= RAD; ++m) {
+                        // the 5-point stencil's coefficient is 1: its
+                        // products are exact and left out
+                        auto term = [&](T v) -> T { return (RAD == 1) ? v : (T)s.c[m] * v; };
+                        const int up = (RAD + m < 2 * RAD) ? RAD + m : 2 * RAD - 1;
+                        const T ux = (m == RAD) ? fx[0] : ring[l - 1][up][0];
+                        const T uy = (m == RAD) ? fx[1] : ring[l - 1][up][1];
+                        a = a + term(xr[m - 1]);
+                        c = c + term(yr[m - 1]);                 // i+m
+                        a = a + term(xl[m - 1]);
+                        c = c + term(yl[m - 1]);                 // i-m
+                        a = a + term(ux);
+                        c = c + term(uy);                        // j+m
+                        a = a + term(ring[l - 1][RAD - m][0]);
+                        c = c + term(ring[l - 1][RAD - m][1]);   // j-m
+                    }
+                    Own on;
+                    T ex, ey;
+                    Tr::level(x, y, a, c, fifo[l - 1][0], s, sn, on, ex, ey);
+                    if (l == D) {
+                        if (it >= 0 && colout) Tr::store(out, row(j), x, y, on);
+                    } else {
+                        nx[0] = ex;
+                        nx[1] = ey;
+                        no = on;
+                    }
+                }
+            }
+            // shift level l by one row: the value handed up enters at the top
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+#pragma unroll
+                for (int p = 0; p + 1 < 2 * RAD; ++p) ring[l - 1][p][q] = ring[l - 1][p + 1][q];
+                ring[l - 1][2 * RAD - 1][q] = fx[q];
+                fx[q] = nx[q];
+            }
+#pragma unroll
+            for (int p = 0; p + 1 < RAD; ++p) fifo[l - 1][p] = fifo[l - 1][p + 1];
+            fifo[l - 1][RAD - 1] = fo;
+            fo = no;
+        }
+    }
+}
+
+template <class Tr>
+__device__ __forceinline__ typename Tr::In strip_in(void *const *p)
+{
+    if constexpr (std::is_same<Tr, F32Form>::value)
+        return {(const fvec4 *)p[PHI]};
+    else
+        return {(const dvec2 *)p[PHI], (const dvec2 *)p[VEL], (const dvec2 *)p[KER]};
+}
+template <class Tr>
+__device__ __forceinline__ typename Tr::Out strip_out(void *const *p)
+{
+    if constexpr (std::is_same<Tr, F32Form>::value)
+        return {(fvec4 *)p[PHI]};
+    else
+        return {(dvec2 *)p[PHI], (dvec2 *)p[VEL], (dvec2 *)p[KER]};
+}
+
+// one work item per wave: (member, strip, chunk) from the member table
+template <class Tr, int D, int RAD, bool PF>
+__global__ void __launch_bounds__(64 * kWaves2D)
+march2d_kernel(const BatchMember *__restrict__ mem, int n_members, uint32_t n_items,
+               LevelScalars<typename Tr::Sc, D> sc)
+{
+    const uint32_t wave = blockIdx.x * kWaves2D + (threadIdx.x >> 6);
+    const uint32_t stride = gridDim.x * kWaves2D;
+    for (uint32_t item = wave; item < n_items; item += stride) {
+        int lo = 0, hi = n_members - 1;        // last member starting at or before item
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (mem[mid].item_begin <= item)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const BatchMember &m = mem[lo];
+        const uint32_t t = item - m.item_begin;
+        const int ti = (int)(t % (uint32_t)m.nTj);
+        const int ch = (int)(t / (uint32_t)m.nTj);
+        march2d_strip<Tr, D, RAD, PF>(m.N, m.CI, ti, ch, strip_in<Tr>(m.in),
+                                      strip_out<Tr>(m.out), sc);
+    }
+}
+
+template <class Tr, int D, int RAD>
+int launch_march2d_pass(Context *c, int src, const StepScalars *s, bool pf)
+{
+    LevelScalars<typename Tr::Sc, D> sc;
+    for (int l = 0; l < D; ++l) {
+        if constexpr (std::is_same<typename Tr::Sc, StepScalarsF>::value)
+            sc.s[l] = to_float(s[l]);
+        else
+            sc.s[l] = s[l];
+    }
+    auto kern = pf ? march2d_kernel<Tr, D, RAD, true> : march2d_kernel<Tr, D, RAD, false>;
+    int per_cu = 0;
+    AXS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * kWaves2D, 0));
+    const uint32_t cap = (uint32_t)c->num_cu * (uint32_t)(per_cu > 0 ? per_cu : 1);
+    const uint32_t need = (c->mbatch_items + kWaves2D - 1) / kWaves2D;
+    const uint32_t grid = need < cap ? need : cap;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kWaves2D), 0, c->stream, c->mbatch_tab[src],
+                       (int)c->members.size(), c->mbatch_items, sc);
+    return AXS_OK;
+}
+
+}  // namespace axs

@@ -89,7 +89,7 @@ int time_trial(Context *c, int trials, double *ms)
     AXS_TRY(prepare_batch(c));
     const int cur0 = c->cur;
     const double dt = (double)c->p.time_step;
-    const int depth = multi_step_depth(c);
+    const int depth = multi_depth(c);     // 0: the one-step kernels
     StepScalars ss[kMaxDepth];
     double t = c->tau;
     for (int l = 0; l < kMaxDepth; ++l) {
@@ -98,7 +98,7 @@ int time_trial(Context *c, int trials, double *ms)
     }
     const StepScalars &s = ss[0];
     // one trial = one launch of one or several steps
-    auto one = [&]() { return depth > 1 ? launch_multi_steps(c, ss, depth) : launch_step(c, s); };
+    auto one = [&]() { return depth >= 1 ? launch_multi_steps(c, ss, depth) : launch_step(c, s); };
     AXS_TRY(one());                             // warm-up (code objects, caches)
     c->cur = cur0;
     const bool graph = (c->tune.flags & kTuneGraph) != 0;
@@ -134,7 +134,7 @@ int time_trial(Context *c, int trials, double *ms)
     AXS_HIP_TRY(hipEventElapsedTime(&f, c->ev0, c->ev1));
     if (ge) (void)hipGraphExecDestroy(ge);
     if (g) (void)hipGraphDestroy(g);
-    *ms = (double)f / trials / depth;
+    *ms = (double)f / trials / (depth >= 1 ? depth : 1);
     return AXS_OK;
 }
 
@@ -167,7 +167,10 @@ int run_autotune(Context *c, int trials, double *best_ms)
     std::vector<Tune> cands;
     int minN = 1 << 30;
     for (const Member &m : c->members) minN = std::min(minN, m.N);
-    const bool march_ok = c->nd == 3 && c->p.stencil_setting == 0;
+    const int RAD = stencil_radius_of(c);
+    const bool march_ok = c->nd == 3 && RAD == 1;
+    bool hi_ok = c->nd == 3 && RAD > 1;
+    for (const Member &m : c->members) hi_ok = hi_ok && march2_supported(c, m.N);
     bool strips_ok = c->nd == 2;
     for (const Member &m : c->members) strips_ok = strips_ok && march2d_supported(c, m.N);
     for (int graph = 0; graph < 2; ++graph) {
@@ -177,7 +180,7 @@ int run_autotune(Context *c, int trials, double *best_ms)
         cands.push_back(t);
         if (strips_ok) {
             // 2D: several steps per pass, one strip per wave
-            const int depths[5] = {2, 3, 4, 6, 8};
+            const int depths[8] = {1, 2, 3, 4, 5, 6, 7, 8};
             const int cj_opts[4] = {0, 32, 64, 128};
             for (int d : depths)
                 for (int pf = 0; pf < 2; ++pf)
@@ -186,11 +189,34 @@ int run_autotune(Context *c, int trials, double *best_ms)
                         for (const Member &m : c->members)
                             ok = ok && (cj_opts[ci] == 0 || m.N % cj_opts[ci] == 0);
                         if (!ok) continue;
+                        if (!march2d_depth_built(d, RAD, c->prec)) continue;
                         Tune u;
                         u.kernel = AXS_KERNEL_MARCH;
                         u.depth = d;
                         u.ci32 = u.ci64 = cj_opts[ci];
                         u.flags = (pf ? kTunePrefetch : 0) | kTuneNT | (graph ? kTuneGraph : 0);
+                        cands.push_back(u);
+                    }
+        }
+        if (hi_ok) {
+            // higher-order stencils, 3D: the multi-step kernel at one or two
+            // steps per pass in its built shape
+            const int W = (c->prec == AXS_FP64) ? 8 : 16;
+            const int chunk_opts3[3] = {0, 16, 32};
+            for (int d = 1; d <= 2; ++d)
+                for (int pf = 0; pf < 2; ++pf)
+                    for (int ci = 0; ci < 3; ++ci) {
+                        int chunk = chunk_opts3[ci];
+                        bool ok = true;
+                        for (const Member &m : c->members)
+                            ok = ok && (chunk == 0 || m.N % chunk == 0);
+                        if (!ok) continue;
+                        Tune u;
+                        u.kernel = AXS_KERNEL_MARCH;
+                        u.depth = d;
+                        u.ci32 = u.ci64 = chunk;
+                        u.flags = (pf ? kTunePrefetch : 0) | kTuneNT |
+                                  (W == 8 ? kTuneWG512 : kTuneWG1024) | (graph ? kTuneGraph : 0);
                         cands.push_back(u);
                     }
         }
@@ -246,8 +272,8 @@ int run_autotune(Context *c, int trials, double *best_ms)
     double best_t = 1e300;
     for (const Tune &t : cands) {
         c->tune = t;
-        if (t.depth >= 2) {
-            if (multi_step_depth(c) != t.depth) continue;
+        if (t.kernel == AXS_KERNEL_MARCH && (t.depth >= 2 || RAD > 1)) {
+            if (multi_depth(c) != t.depth) continue;
         } else if (t.kernel == AXS_KERNEL_MARCH) {
             bool ok = true;
             for (const Member &m : c->members) ok = ok && march3d_supported(c, m.N);

@@ -236,6 +236,38 @@ def forms_leg(np, axs, T0):
     return out
 
 
+def stencils_leg(np, axs, N, T0, steps=10):
+    """The higher-order Laplacians (stencil_setting 1-3: radius 2-4) on the
+    3D N^3 lattice in both forms: the library's default multi-step
+    configuration against the one-step site kernel."""
+    out = {}
+    for prec in ("fp32", "fp64"):
+        for st in (1, 2, 3):
+            lat = axs.Lattice(3, N, precision=prec, T_initial=T0, stencil_setting=st)
+            lat.init("kinv", seed=1998)
+            lat.tau = 1.0
+            cfg = lat.kernel_config()
+            lat.step(2 * cfg["depth"])
+            t0 = time.perf_counter()
+            lat.step(steps)
+            wall = time.perf_counter() - t0
+            kern = lat.last_step_ms
+            lat.set_kernel("simple")
+            lat.step(1)
+            t1 = time.perf_counter()
+            lat.step(3)
+            one = (time.perf_counter() - t1) / 3
+            out[f"{prec}_stencil{st}"] = {
+                "N": N, "steps": steps, "ms_per_step": 1e3 * wall / steps,
+                "site_updates_per_s": N ** 3 * steps / wall, "kernel_ms": kern,
+                "config": cfg, "one_step_kernel_ms_per_step": 1e3 * one,
+                "speedup_over_one_step": one / (wall / steps)}
+            lat.close()
+            log(f"[bench] stencil {st} {prec}: {N}^3, {1e3 * wall / steps:.3f} ms/step "
+                f"(one-step kernel {1e3 * one:.3f})")
+    return out
+
+
 def snapshot_leg(np, axs, N, T0, steps=200):
     """Field snapshot of the 1024^3 FP32 run (phi1, phi2 of every site,
     8.6 GB each as float64 on the host): the synchronous read-back against
@@ -342,6 +374,8 @@ def main() -> int:
                     help="skip the asynchronous-snapshot overlap leg")
     ap.add_argument("--skip-forms", action="store_true",
                     help="skip the Hamiltonian-form and spectral-Laplacian legs")
+    ap.add_argument("--skip-stencils", action="store_true",
+                    help="skip the higher-order Laplacian legs")
     ap.add_argument("--skip-ensembles", action="store_true",
                     help="skip the 2D (512-member) and 3D (48-member) ensemble legs")
     ap.add_argument("--ensemble-steps", type=int, default=0,
@@ -424,6 +458,13 @@ def main() -> int:
         except Exception as e:              # reported, the headline line still prints
             forms_out = {"error": str(e)}
 
+    stencils = None
+    if not args.skip_stencils:
+        try:
+            stencils = stencils_leg(np, axs, N, args.T_initial)
+        except Exception as e:              # reported, the headline line still prints
+            stencils = {"error": str(e)}
+
     snap = None
     if not args.skip_snapshot:
         try:
@@ -477,6 +518,7 @@ def main() -> int:
         "sampling": sampling,
         "ensembles": ens,
         "forms": forms_out,
+        "stencils": stencils,
         "snapshot": snap,
         "cpu_baseline": cpu,
     }
