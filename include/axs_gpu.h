@@ -197,15 +197,17 @@ int axs_powerspec(axs_ctx *ctx, int binning, int nbins,
 #define AXS_MARCH_NT_STORE 2   /* non-temporal stores of the new state        */
 #define AXS_MARCH_JFAST    4   /* tiles ordered j fastest (default k fastest) */
 #define AXS_STEP_GRAPH     8   /* axs_step captures its launches in a HIP graph */
-#define AXS_MARCH_WG512   16   /* depth >= 2: 8 waves per work-group (else 4)  */
-#define AXS_MARCH_WG1024  32   /* depth >= 2: 16 waves per work-group          */
+#define AXS_MARCH_WG512   16   /* 3D multi-step kernel: 8 waves per work-group
+                                  (else 4; higher orders: the built size)  */
+#define AXS_MARCH_WG1024  32   /* 3D multi-step kernel: 16 waves per work-group */
 
 typedef struct axs_kernel_config {
     int kernel;   /* AXS_KERNEL_*                                           */
     int rows;     /* rows per work-item: FP32 4 or 8, FP64 2 or 4; depth >= 2:
                      4 or 6 (8 at depth 2; depth 4 needs 8 or 16 waves),
                      with AXS_MARCH_WG512 2, 3 or 4, with AXS_MARCH_WG1024
-                     2 or 3                                                 */
+                     2 or 3.  Higher orders, 3D: FP32 2 (16 waves), FP64 2
+                     or, at stencil_setting 1, 3 (8 waves)                 */
     int chunk;    /* planes marched per work-group (divides N)              */
     int depth;    /* time steps per pass over the lattice: 1 (0 = 1), or,
                      with stencil 0, the stencil Laplacian and every side
@@ -213,7 +215,12 @@ typedef struct axs_kernel_config {
                      3 if a side is below 256, with AXS_MARCH_WG1024), 3D
                      FP64 2 or 3 with AXS_MARCH_WG512 (starts at 3), 2D
                      2, 3, 4, 6 or 8 (starts at 6; rows 0, no work-group
-                     flags).
+                     flags).  stencil_setting 1-3 (field-rescaled form,
+                     sides >= 64, kernel not SIMPLE): the multi-step kernels
+                     run from depth 1 -- 3D FP32 1 or 2 (stencil 3: 1), 3D
+                     FP64 1, 2D FP32 1-4, 2D FP64 1 or 2; a new context
+                     starts at 3D FP32 2 (stencil 3: 1), FP64 1, 2D FP32 4
+                     (stencil 1) or 2, FP64 2.
                      Step counts that are not a multiple end with one-step
                      launches.                                              */
     int flags;    /* AXS_MARCH_* bits                                       */
@@ -238,9 +245,10 @@ int axs_autotune(axs_ctx *ctx, int trials, axs_kernel_config *best, double *ms_p
 int axs_last_step_ms(const axs_ctx *ctx, double *ms_per_step);
 
 /* Bytes the configured step kernel must move per site-update (the scheme
- * bytes: FP64 96, FP32 32 / depth, as one pass reads and writes the state
- * once for `depth` updates) and the path's algorithmic bytes per
- * site-update (read+write of phi and its velocity: FP64 64, FP32 32).     */
+ * bytes: FP64 96, FP32 32, divided by the steps per pass, as one pass reads
+ * and writes the state once for `depth` updates) and the path's algorithmic
+ * bytes per site-update (read+write of phi and its velocity: FP64 64, FP32
+ * 32).                                                                    */
 int axs_bytes_per_site(const axs_ctx *ctx, double *scheme_bytes, double *alg_bytes);
 
 /* The Laplacian of one member's current field, per component, into host
