@@ -92,6 +92,8 @@ StepScalarsF to_float(const StepScalars &s)
     f.cl = (float)s.cl;
     f.tau2 = (float)s.tau2;
     f.T03 = (float)s.T03;
+    f.pot0 = (float)(s.T03 - s.tau2);
+    f.ac0 = (float)(s.alpha * s.c[0]);
     for (int i = 0; i < 5; ++i) f.c[i] = (float)s.c[i];
     f.ncoef = s.ncoef;
     return f;

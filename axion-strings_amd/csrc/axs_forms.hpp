@@ -5,10 +5,12 @@
 // sum `a` (centre coefficient times x, then the neighbours in COO order) it
 // already has:
 //     ker = alpha * a + cl * x * S,   S = x.x^2 + x.y^2 - tau2 + T03
-// which is the expression tree of the one-step kernels (s_integrate.cpp:
-// 322-335), then the form's kick and the next step's drift:
-//   F32Form  (phi, u) in one float4, u = phidot + dt/2 ker:
-//            u' = u + dt ker;   x' = x + dt u'
+// (a starts from the form's centre term), then the form's kick and the
+// next step's drift:
+//   F32Form  (phi, u) in one float4, u = phidot + dt/2 ker: a is the
+//            neighbour sum alone and ker = alpha a + x (alpha c0 + cl S);
+//            u' = u + dt ker;   x' = x + dt u'  (fused multiply-adds of
+//            F32Math, as in every FP32 kernel)
 //   F64Form  (phi, phidot, ker) in three double2 arrays, the reference's
 //            velocity-Verlet (s_integrate.cpp:311-347):
 //            v' = v + 0.5 (k + ker) dt;  x' = x + dt (v' + 0.5 ker dt)
@@ -39,11 +41,19 @@ struct F32Form {
     struct Own {
         float u1, u2;
     };
+    // own data of a site whose value is never used: any bits (no moves)
+    static __device__ __forceinline__ Own unset()
+    {
+        return Own{__builtin_nondeterministic_value(0.0f), __builtin_nondeterministic_value(0.0f)};
+    }
     static __device__ __forceinline__ void fetch(const In &a, uint32_t q, Raw &r) { r = a.st[q]; }
+    // Laplacian accumulation: start (the centre joins the potential), a + c v
+    static __device__ __forceinline__ T centre(const Sc &, T) { return -0.0f; }
+    static __device__ __forceinline__ T acc(T a, T c, T v) { return F32Math::acc(a, c, v); }
     static __device__ __forceinline__ void drift0(const Raw &r, const Sc &s, T &x, T &y, Own &o)
     {
-        x = r.x + s.dt * r.z;
-        y = r.y + s.dt * r.w;
+        x = F32Math::drift(r.x, r.z, s.dt);
+        y = F32Math::drift(r.y, r.w, s.dt);
         o.u1 = r.z;
         o.u2 = r.w;
     }
@@ -51,13 +61,13 @@ struct F32Form {
     static __device__ __forceinline__ void level(T x, T y, T a, T c, const Own &o, const Sc &s,
                                                  const Sc &sn, Own &on, T &nx, T &ny)
     {
-        const T S = x * x + y * y - s.tau2 + s.T03;
-        const T k1 = s.alpha * a + s.cl * x * S;
-        const T k2 = s.alpha * c + s.cl * y * S;
-        on.u1 = o.u1 + s.dt * k1;
-        on.u2 = o.u2 + s.dt * k2;
-        nx = x + sn.dt * on.u1;
-        ny = y + sn.dt * on.u2;
+        const T t = F32Math::own(s.cl, F32Math::pot(x, y, s.pot0), s.ac0);
+        const T k1 = F32Math::force(s.alpha, a, x, t);
+        const T k2 = F32Math::force(s.alpha, c, y, t);
+        on.u1 = F32Math::kick(o.u1, s.dt, k1);
+        on.u2 = F32Math::kick(o.u2, s.dt, k2);
+        nx = F32Math::drift(x, on.u1, sn.dt);
+        ny = F32Math::drift(y, on.u2, sn.dt);
     }
     static __device__ __forceinline__ void store(const Out &o, uint32_t q, T x, T y, const Own &w)
     {
@@ -86,12 +96,26 @@ struct F64FormT {
     struct Own {
         dvec2 v, k;
     };
+    // own data of a site whose value is never used: any bits (no moves)
+    static __device__ __forceinline__ Own unset()
+    {
+        Own o;
+        o.v.x = __builtin_nondeterministic_value(0.0);
+        o.v.y = __builtin_nondeterministic_value(0.0);
+        o.k.x = __builtin_nondeterministic_value(0.0);
+        o.k.y = __builtin_nondeterministic_value(0.0);
+        return o;
+    }
     static __device__ __forceinline__ void fetch(const In &a, uint32_t q, Raw &r)
     {
         r.p = a.phi[q];
         r.v = a.vel[q];
         r.k = a.ker[q];
     }
+    // Laplacian accumulation: c0 x, then a + c v, products and sums rounded
+    // separately (s_integrate.cpp:322-324 in COO row order)
+    static __device__ __forceinline__ T centre(const Sc &s, T x) { return s.c[0] * x; }
+    static __device__ __forceinline__ T acc(T a, T c, T v) { return a + c * v; }
     static __device__ __forceinline__ void drift(T p, T v, T k, const Sc &s, T &x)
     {
         if (HAM)

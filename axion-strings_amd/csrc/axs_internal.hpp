@@ -61,6 +61,8 @@ struct StepScalars {
 
 struct StepScalarsF {
     float dt, alpha, cl, tau2, T03, half_dt;
+    float pot0;     // T03 - tau2, rounded once from the double difference
+    float ac0;      // alpha * c[0]
     float c[5];
     int ncoef;
 };

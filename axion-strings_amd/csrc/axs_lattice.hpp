@@ -67,6 +67,49 @@ __device__ __forceinline__ void store_nt(V *p, V v)
     __builtin_nontemporal_store(v, p);
 }
 
+// The FP32 form's arithmetic, shared by every FP32 kernel so that one-step
+// and multi-step launches give identical bits.  The force is regrouped as
+//     ker = alpha (c0 x + sum_m c_m n_m) + cl x S
+//         = alpha * a + x * (alpha c0 + cl S),    a = sum_m c_m n_m
+// (the neighbour sum a starts from -0.0, the additive identity), and every
+// product-sum is a fused multiply-add: one rounding each and a third fewer
+// instructions per site than the reference's expression tree.  The FP64
+// form keeps the reference's tree and separate roundings, which its
+// bit-for-bit parity needs.
+struct F32Math {
+    // drifted field x = phi + dt u
+    static __device__ __forceinline__ float drift(float phi, float u, float dt)
+    {
+        return __builtin_fmaf(dt, u, phi);
+    }
+    // Laplacian accumulation a + c v (c = 1 for the 5/7-point stencil: the
+    // product is exact, so this is the plain sum a + v)
+    static __device__ __forceinline__ float acc(float a, float c, float v)
+    {
+        return __builtin_fmaf(c, v, a);
+    }
+    // potential factor S = x^2 + y^2 + pot0, pot0 = T0^2/3 - tau^2
+    static __device__ __forceinline__ float pot(float x, float y, float pot0)
+    {
+        return __builtin_fmaf(x, x, __builtin_fmaf(y, y, pot0));
+    }
+    // the site's own factor t = alpha c0 + cl S (shared by both components)
+    static __device__ __forceinline__ float own(float cl, float S, float ac0)
+    {
+        return __builtin_fmaf(cl, S, ac0);
+    }
+    // ker = alpha a + x t
+    static __device__ __forceinline__ float force(float alpha, float a, float x, float t)
+    {
+        return __builtin_fmaf(alpha, a, x * t);
+    }
+    // u' = u + dt ker
+    static __device__ __forceinline__ float kick(float u, float dt, float k)
+    {
+        return __builtin_fmaf(dt, k, u);
+    }
+};
+
 __device__ __forceinline__ int wrap_coord(int x, int N)
 {
     // valid for -N <= x < 2N, which holds for every stencil radius <= 4 on

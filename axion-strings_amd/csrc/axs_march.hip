@@ -47,7 +47,8 @@ struct ExactF64T {
         dvec2 p, v, k;
     };
     using Own = Raw;
-    static __device__ __forceinline__ double c0(const S &s) { return s.c[0]; }
+    // Laplacian start: the centre term c0 x (s_integrate.cpp:322-324, COO row order)
+    static __device__ __forceinline__ double centre(const S &s, double x) { return s.c[0] * x; }
     static __device__ __forceinline__ In make_in(void *const *p)
     {
         return In{(const dvec2 *)p[0], (const dvec2 *)p[1], (const dvec2 *)p[2]};
@@ -117,7 +118,8 @@ struct HalfF32 {
     struct Own {
         float u1, u2;
     };
-    static __device__ __forceinline__ float c0(const S &s) { return s.c[0]; }
+    // the neighbour sum alone: the centre term joins the potential (F32Math)
+    static __device__ __forceinline__ float centre(const S &, float) { return -0.0f; }
     static __device__ __forceinline__ In make_in(void *const *p) { return In{(const fvec4 *)p[0]}; }
     static __device__ __forceinline__ Out make_out(void *const *p) { return Out{(fvec4 *)p[0]}; }
     static __device__ __forceinline__ void load(const In &a, uint32_t q, Raw &r) { r = a.st[q]; }
@@ -126,21 +128,21 @@ struct HalfF32 {
     {
         o.u1 = r.z;
         o.u2 = r.w;
-        x1 = r.x + s.dt * r.z;
-        x2 = r.y + s.dt * r.w;
+        x1 = F32Math::drift(r.x, r.z, s.dt);
+        x2 = F32Math::drift(r.y, r.w, s.dt);
     }
     template <bool NT>
     static __device__ __forceinline__ void update(const Out &o, uint32_t q, const S &s, float x1,
                                                   float x2, float sum1, float sum2, const Own &w)
     {
-        float S_ = x1 * x1 + x2 * x2 - s.tau2 + s.T03;
-        float k1 = s.alpha * sum1 + s.cl * x1 * S_;
-        float k2 = s.alpha * sum2 + s.cl * x2 * S_;
+        float t = F32Math::own(s.cl, F32Math::pot(x1, x2, s.pot0), s.ac0);
+        float k1 = F32Math::force(s.alpha, sum1, x1, t);
+        float k2 = F32Math::force(s.alpha, sum2, x2, t);
         fvec4 r;
         r.x = x1;
         r.y = x2;
-        r.z = w.u1 + s.dt * k1;
-        r.w = w.u2 + s.dt * k2;
+        r.z = F32Math::kick(w.u1, s.dt, k1);
+        r.w = F32Math::kick(w.u2, s.dt, k2);
         if (NT)
             store_nt(o.st + q, r);
         else
@@ -267,7 +269,6 @@ __device__ __forceinline__ void march_tile(const int N, const int CI, const int 
     if (PF) load_plane(i0 + 1, r1, h1r);
     __syncthreads();
 
-    const T c0 = Tr::c0(s);
     for (int it = 0; it < CI; ++it) {
         const int i = i0 + it;
         const int buf = it & 1;
@@ -286,8 +287,8 @@ __device__ __forceinline__ void march_tile(const int N, const int CI, const int 
 #pragma unroll
         for (int rr = 0; rr < RJ; ++rr) {
             int o = (wave + kMW * rr + 1) * kLDK + lane + 1;
-            T s1 = c0 * xc[rr][0];
-            T s2 = c0 * xc[rr][1];
+            T s1 = Tr::centre(s, xc[rr][0]);
+            T s2 = Tr::centre(s, xc[rr][1]);
             s1 = s1 + xn[rr][0];
             s2 = s2 + xn[rr][1];      // (i+1, j, k)
             s1 = s1 + xp[rr][0];

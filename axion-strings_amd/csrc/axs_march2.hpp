@@ -35,13 +35,28 @@
 namespace axs {
 
 constexpr int kTBW = 64;   // columns loaded per row (one wave)
+// planes per unrolled block of the march loop: the register rings then
+// rotate partly by renaming (measured: three planes 8% faster than one at
+// 1024^3, FP32 depth 4; two and six no better).  Shapes whose rings and
+// FIFOs hold many registers per lane (weighted double for 16-wave groups,
+// which have half the registers) stay at one plane: unrolled, they spill.
+template <class Tr, int D, int W, int R, int RAD>
+constexpr int plane_unroll()
+{
+    constexpr int words = D * R * RAD * (4 * (int)sizeof(typename Tr::T) +
+                                         (int)sizeof(typename Tr::Own)) / 4;
+    return words * (W >= 16 ? 2 : 1) <= 96 ? 3 : 1;
+}
 
 struct TBGeom {
     int N, CI, nTj, nTk, remap;
 };
 
-template <class T, int D, int W, int R>
-using TBLds = T[D][2][W * R * kTBW];   // x_l of the plane level l works on
+// x_l of the plane level l works on: the ER loaded rows plus RAD padding
+// rows above and below, so that the neighbour reads of the rim rows and
+// lanes stay inside the array (their values are never used)
+template <class T, int D, int W, int R, int RAD>
+using TBLds = T[D][2][(W * R + 2 * RAD) * kTBW];
 
 // One work-group: tile (tj, tk) of the lattice of side N, planes
 // [ch*CI, ch*CI + CI).
@@ -50,7 +65,7 @@ __device__ __forceinline__ void marchd_tile(const int N, const int CI, const int
                                             const int ch, const typename Tr::In &in,
                                             const typename Tr::Out &out,
                                             const LevelScalars<typename Tr::Sc, D> &sc,
-                                            TBLds<typename Tr::T, D, W, R> &X)
+                                            TBLds<typename Tr::T, D, W, R, RAD> &X)
 {
     using T = typename Tr::T;
     using Own = typename Tr::Own;
@@ -61,7 +76,10 @@ __device__ __forceinline__ void marchd_tile(const int N, const int CI, const int
     constexpr int TK = kTBW - 2 * H;     // columns written
     static_assert(TJ > 0 && TK > 0, "tile shape");
 
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave index is uniform: held in a scalar register, so the row
+    // tests below are scalar branches
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t NN = (uint32_t)N * (uint32_t)N;
     const int j0 = tj * TJ, k0 = tk * TK, i0 = ch * CI;
 
@@ -91,10 +109,11 @@ __device__ __forceinline__ void marchd_tile(const int N, const int CI, const int
 #pragma unroll
         for (int r = 0; r < R; ++r) Tr::fetch(in, po + rowoff[r], raw[r]);
     };
-    // site (row e, this lane) lies in level l's region
-    auto inside = [&](int e, int l) -> bool {
-        return e >= l * RAD && e < ER - l * RAD && lane >= l * RAD && lane < kTBW - l * RAD;
-    };
+    // row e lies in level l's region.  Every lane of such a row computes:
+    // masking the rim lanes would save no issue cycles, and the values of
+    // lanes (and rows, planes) outside level l's region only ever reach
+    // sites outside level l+1's region, which are never stored
+    auto row_in = [&](int e, int l) -> bool { return e >= l * RAD && e < ER - l * RAD; };
 
     // per level l = 1..D (index l-1): x_l of planes q-RAD .. q+RAD-1 (ring)
     // and the own data of planes q .. q+RAD-1 (fifo)
@@ -112,7 +131,7 @@ __device__ __forceinline__ void marchd_tile(const int N, const int CI, const int
     Raw raw[R], pf[R];
 
     if (PF) load(i0 - H, pf);
-    for (int it = -2 * H; it < CI; ++it) {
+    auto plane_step = [&](const int it) __attribute__((always_inline)) {
         const int i = i0 + it;
         // level 0: plane i+H
         if (PF) {
@@ -132,42 +151,46 @@ __device__ __forceinline__ void marchd_tile(const int N, const int CI, const int
             const typename Tr::Sc &s = sc.s[l - 1];
             const typename Tr::Sc &sn = sc.s[l < D ? l : D - 1];
             const T *L1 = X[l - 1][0], *L2 = X[l - 1][1];
-            const bool active = it >= 2 * RAD * (l - D);   // x_l of planes q-RAD .. q+RAD exist
+            // level l computes from the first plane on: until x_l of planes
+            // q-RAD .. q+RAD exist (it >= 2 RAD (l - D)) its results are
+            // unused -- they leave the ring of level l+1 before it fills
             T nx[R][2];
             Own no[R];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                nx[r][0] = nx[r][1] = (T)0;
-                no[r] = Own{};
-            }
-            if (active) {
+            {
                 const uint32_t po = plane(i + (D - l) * RAD);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    if (!inside(erow[r], l)) continue;
-                    const int o = erow[r] * kTBW + lane;
+                    // rows outside the region: any value (never used)
+                    nx[r][0] = __builtin_nondeterministic_value((T)0);
+                    nx[r][1] = __builtin_nondeterministic_value((T)0);
+                    no[r] = Tr::unset();
+                    if (!row_in(erow[r], l)) continue;
+                    const int o = (erow[r] + RAD) * kTBW + lane;
                     const T x = ring[l - 1][RAD][r][0], y = ring[l - 1][RAD][r][1];
-                    T a = s.c[0] * x, c = s.c[0] * y;
+                    T a = Tr::centre(s, x), c = Tr::centre(s, y);
 #pragma unroll
                     for (int m = 1; m <= RAD; ++m) {
                         // the 7-point stencil's coefficient is 1: its
                         // products are exact and left out
-                        auto term = [&](T v) -> T { return (RAD == 1) ? v : (T)s.c[m] * v; };
+                        const T cm = (T)s.c[m];
+                        auto add = [&](T acc, T v) -> T {
+                            return (RAD == 1) ? acc + v : Tr::acc(acc, cm, v);
+                        };
                         const int up = (RAD + m < 2 * RAD) ? RAD + m : 2 * RAD - 1;
                         const T ux = (m == RAD) ? fx[r][0] : ring[l - 1][up][r][0];
                         const T uy = (m == RAD) ? fx[r][1] : ring[l - 1][up][r][1];
-                        a = a + term(ux);
-                        c = c + term(uy);                                   // i+m
-                        a = a + term(ring[l - 1][RAD - m][r][0]);
-                        c = c + term(ring[l - 1][RAD - m][r][1]);           // i-m
-                        a = a + term(L1[o + m * kTBW]);
-                        c = c + term(L2[o + m * kTBW]);                     // j+m
-                        a = a + term(L1[o - m * kTBW]);
-                        c = c + term(L2[o - m * kTBW]);                     // j-m
-                        a = a + term(L1[o + m]);
-                        c = c + term(L2[o + m]);                            // k+m
-                        a = a + term(L1[o - m]);
-                        c = c + term(L2[o - m]);                            // k-m
+                        a = add(a, ux);
+                        c = add(c, uy);                                     // i+m
+                        a = add(a, ring[l - 1][RAD - m][r][0]);
+                        c = add(c, ring[l - 1][RAD - m][r][1]);             // i-m
+                        a = add(a, L1[o + m * kTBW]);
+                        c = add(c, L2[o + m * kTBW]);                       // j+m
+                        a = add(a, L1[o - m * kTBW]);
+                        c = add(c, L2[o - m * kTBW]);                       // j-m
+                        a = add(a, L1[o + m]);
+                        c = add(c, L2[o + m]);                              // k+m
+                        a = add(a, L1[o - m]);
+                        c = add(c, L2[o - m]);                              // k-m
                     }
                     Own on;
                     T ex, ey;
@@ -211,13 +234,22 @@ __device__ __forceinline__ void marchd_tile(const int N, const int CI, const int
         for (int l = 0; l < D; ++l) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const int o = erow[r] * kTBW + lane;
+                const int o = (erow[r] + RAD) * kTBW + lane;
                 X[l][0][o] = ring[l][RAD][r][0];
                 X[l][1][o] = ring[l][RAD][r][1];
             }
         }
         __syncthreads();   // the next plane's values are visible
+    };
+    // blocks of P planes, each unrolled, so that the register rings rotate
+    // by renaming instead of moves; then the remaining planes
+    constexpr int P = plane_unroll<Tr, D, W, R, RAD>();
+    int it = -2 * H;
+    for (; it + P <= CI; it += P) {
+#pragma unroll
+        for (int k = 0; k < P; ++k) plane_step(it + k);
     }
+    for (; it < CI; ++it) plane_step(it);
 }
 
 template <class Tr, int D, int W, int R, int RAD, bool PF>
@@ -225,7 +257,7 @@ __global__ void __launch_bounds__(64 * W)
 marchd_kernel(TBGeom g, typename Tr::In in, typename Tr::Out out,
               LevelScalars<typename Tr::Sc, D> sc)
 {
-    __shared__ TBLds<typename Tr::T, D, W, R> X;
+    __shared__ TBLds<typename Tr::T, D, W, R, RAD> X;
     uint32_t b = blockIdx.x;
     if (g.remap) b = (b & 7u) * (gridDim.x >> 3) + (b >> 3);   // XCD-aware order
     const int tk = (int)(b % (uint32_t)g.nTk);
@@ -262,7 +294,7 @@ __global__ void __launch_bounds__(64 * W)
 marchd_batch_kernel(const BatchMember *__restrict__ mem, int n_members, uint32_t n_items,
                     LevelScalars<typename Tr::Sc, D> sc)
 {
-    __shared__ TBLds<typename Tr::T, D, W, R> X;
+    __shared__ TBLds<typename Tr::T, D, W, R, RAD> X;
     for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
         int lo = 0, hi = n_members - 1;        // last member starting at or before item
         while (lo < hi) {

@@ -89,10 +89,13 @@ __device__ __forceinline__ void march2d_strip(const int N, const int CJ, const i
         for (int l = 1; l <= D; ++l) {
             const typename Tr::Sc &s = sc.s[l - 1];
             const typename Tr::Sc &sn = sc.s[l < D ? l : D - 1];
-            const bool active = it >= 2 * RAD * (l - D);   // x_l of rows q-RAD .. q+RAD exist
-            T nx[2] = {(T)0, (T)0};
-            Own no{};
-            if (active) {
+            // every lane computes from the first row on: results of lanes
+            // outside level l's region, or of rows before x_l of rows
+            // q-RAD .. q+RAD exist (it < 2 RAD (l - D)), only reach sites
+            // outside level l+1's region, which are never stored
+            T nx[2];
+            Own no;
+            {
                 const T x = ring[l - 1][RAD][0], y = ring[l - 1][RAD][1];
                 // i neighbours from the lanes m away (all lanes take part)
                 T xr[RAD], yr[RAD], xl[RAD], yl[RAD];
@@ -103,24 +106,27 @@ __device__ __forceinline__ void march2d_strip(const int N, const int CJ, const i
                     xl[m - 1] = __shfl_up(x, m);
                     yl[m - 1] = __shfl_up(y, m);
                 }
-                if (lane >= l * RAD && lane < kW2 - l * RAD) {
-                    T a = s.c[0] * x, c = s.c[0] * y;
+                {
+                    T a = Tr::centre(s, x), c = Tr::centre(s, y);
 #pragma unroll
                     for (int m = 1; m <= RAD; ++m) {
                         // the 5-point stencil's coefficient is 1: its
                         // products are exact and left out
-                        auto term = [&](T v) -> T { return (RAD == 1) ? v : (T)s.c[m] * v; };
+                        const T cm = (T)s.c[m];
+                        auto add = [&](T acc, T v) -> T {
+                            return (RAD == 1) ? acc + v : Tr::acc(acc, cm, v);
+                        };
                         const int up = (RAD + m < 2 * RAD) ? RAD + m : 2 * RAD - 1;
                         const T ux = (m == RAD) ? fx[0] : ring[l - 1][up][0];
                         const T uy = (m == RAD) ? fx[1] : ring[l - 1][up][1];
-                        a = a + term(xr[m - 1]);
-                        c = c + term(yr[m - 1]);                 // i+m
-                        a = a + term(xl[m - 1]);
-                        c = c + term(yl[m - 1]);                 // i-m
-                        a = a + term(ux);
-                        c = c + term(uy);                        // j+m
-                        a = a + term(ring[l - 1][RAD - m][0]);
-                        c = c + term(ring[l - 1][RAD - m][1]);   // j-m
+                        a = add(a, xr[m - 1]);
+                        c = add(c, yr[m - 1]);                   // i+m
+                        a = add(a, xl[m - 1]);
+                        c = add(c, yl[m - 1]);                   // i-m
+                        a = add(a, ux);
+                        c = add(c, uy);                          // j+m
+                        a = add(a, ring[l - 1][RAD - m][0]);
+                        c = add(c, ring[l - 1][RAD - m][1]);     // j-m
                     }
                     Own on;
                     T ex, ey;

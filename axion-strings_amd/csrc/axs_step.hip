@@ -105,29 +105,29 @@ __device__ __forceinline__ void site_fp32(int N, uint32_t m, const fvec4 *__rest
     Site<ND> site = decompose<ND>(m, N);
 
     fvec4 c = in[m];
-    float x1 = c.x + s.dt * c.z;
-    float x2 = c.y + s.dt * c.w;
-    float s1 = s.c[0] * x1, s2 = s.c[0] * x2;
+    float x1 = F32Math::drift(c.x, c.z, s.dt);
+    float x2 = F32Math::drift(c.y, c.w, s.dt);
+    float s1 = -0.0f, s2 = -0.0f;     // the centre term joins the potential
     for (int l = 1; l < s.ncoef; ++l) {
         float cl = s.c[l];
 #pragma unroll
         for (int a = 0; a < ND; ++a) {
             fvec4 p = in[shifted<ND>(site, a, l, N)];
             fvec4 q = in[shifted<ND>(site, a, -l, N)];
-            s1 = s1 + cl * (p.x + s.dt * p.z);
-            s2 = s2 + cl * (p.y + s.dt * p.w);
-            s1 = s1 + cl * (q.x + s.dt * q.z);
-            s2 = s2 + cl * (q.y + s.dt * q.w);
+            s1 = F32Math::acc(s1, cl, F32Math::drift(p.x, p.z, s.dt));
+            s2 = F32Math::acc(s2, cl, F32Math::drift(p.y, p.w, s.dt));
+            s1 = F32Math::acc(s1, cl, F32Math::drift(q.x, q.z, s.dt));
+            s2 = F32Math::acc(s2, cl, F32Math::drift(q.y, q.w, s.dt));
         }
     }
-    float S = x1 * x1 + x2 * x2 - s.tau2 + s.T03;
-    float k1 = s.alpha * s1 + s.cl * x1 * S;
-    float k2 = s.alpha * s2 + s.cl * x2 * S;
+    float t = F32Math::own(s.cl, F32Math::pot(x1, x2, s.pot0), s.ac0);
+    float k1 = F32Math::force(s.alpha, s1, x1, t);
+    float k2 = F32Math::force(s.alpha, s2, x2, t);
     fvec4 o;
     o.x = x1;
     o.y = x2;
-    o.z = c.z + s.dt * k1;
-    o.w = c.w + s.dt * k2;
+    o.z = F32Math::kick(c.z, s.dt, k1);
+    o.w = F32Math::kick(c.w, s.dt, k2);
     out[m] = o;
 }
 
@@ -237,22 +237,22 @@ canonical_fp32_kernel(int N, uint32_t L, const fvec4 *__restrict__ in, fvec2 *__
     fvec2 k;
     if (recompute) {
         Site<ND> site = decompose<ND>(m, N);
-        float s1 = s.c[0] * c.x, s2 = s.c[0] * c.y;
+        float s1 = -0.0f, s2 = -0.0f;
         for (int l = 1; l < s.ncoef; ++l) {
             float cl = s.c[l];
 #pragma unroll
             for (int a = 0; a < ND; ++a) {
                 fvec4 p = in[shifted<ND>(site, a, l, N)];
                 fvec4 q = in[shifted<ND>(site, a, -l, N)];
-                s1 = s1 + cl * p.x;
-                s2 = s2 + cl * p.y;
-                s1 = s1 + cl * q.x;
-                s2 = s2 + cl * q.y;
+                s1 = F32Math::acc(s1, cl, p.x);
+                s2 = F32Math::acc(s2, cl, p.y);
+                s1 = F32Math::acc(s1, cl, q.x);
+                s2 = F32Math::acc(s2, cl, q.y);
             }
         }
-        float S = c.x * c.x + c.y * c.y - s.tau2 + s.T03;
-        k.x = s.alpha * s1 + s.cl * c.x * S;
-        k.y = s.alpha * s2 + s.cl * c.y * S;
+        float t = F32Math::own(s.cl, F32Math::pot(c.x, c.y, s.pot0), s.ac0);
+        k.x = F32Math::force(s.alpha, s1, c.x, t);
+        k.y = F32Math::force(s.alpha, s2, c.y, t);
         ker[m] = k;
     } else {
         k = ker[m];
