@@ -20,10 +20,17 @@
 // they are used); x_l of plane q sits in one LDS plane per level for the j
 // and k neighbours, written after a barrier that ends the plane's reads and
 // read after a second barrier.  The Laplacian sums in the one-step kernels'
-// order -- centre, then for each distance m: +-i, +-j, +-k, each times the
-// stencil coefficient -- with no contraction, so a pass is bit-identical to
-// D one-step launches.  Tiles at the lattice edge overhang it: loads wrap
-// periodically and stores are masked.
+// order -- the form's centre term, then for each distance m: +-i, +-j,
+// +-k, each times the stencil coefficient -- with the forms' expressions,
+// so a pass is bit-identical to D one-step launches.  Tiles at the lattice
+// edge overhang it: loads wrap periodically and stores are masked.
+//
+// Issue cost (the kernel is bound by vector instructions, not HBM): a row
+// outside a level's region is skipped with a scalar branch (the wave index
+// is uniform), every lane of the other rows computes (lanes outside the
+// region cost no extra issue cycles and their values never reach a stored
+// site), the LDS planes carry RAD padding rows so that those lanes' reads
+// stay inside them, and the march loop runs in unrolled blocks of planes.
 #pragma once
 
 #include <type_traits>
