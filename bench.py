@@ -302,27 +302,52 @@ def snapshot_leg(np, axs, N, T0, steps=200):
 
 
 def cpu_baseline(np, seconds: float):
-    """The oracle's reference-structured FP64 step on a 256^3 sub-sample of
-    the workload (same per-site work), all host threads."""
+    """The CPU oracle on the host's threads, on bounded samples of the
+    workload: the reference-structured FP64 step (five sweeps + CSR SpMV,
+    the reference's own structure) at 3D 256^3 -- the reported value -- and
+    a fused two-sweep stencil stepper (same bits) at 2D 256^2, 3D 256^3 and
+    3D 512^3, the last standing in for 1024^3 per site (its FP64 state and
+    the reference's int32 COO matrix do not fit the reference structure)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle as O
     from fields import kinv_field
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+
+    def timed(run, budget, sites):
+        run(1)                                  # first touch
+        n, t = 0, 0.0
+        while t < budget and n < 5000:
+            t0 = time.perf_counter()
+            run(1)
+            t += time.perf_counter() - t0
+            n += 1
+        return sites * n / t, n, t
+
     Ns = 256
     p1, p2 = kinv_field(3, Ns, seed=1998)
     z = np.zeros(Ns ** 3)
-    st = O.State(p1, p2, z, z)
     p = O.params(3, Ns)
     A = O.Matrix(3, Ns)
-    st.run(p, A, 1)                             # first touch
-    n, t = 0, 0.0
-    while t < seconds and n < 5000:
-        t0 = time.perf_counter()
-        st.run(p, A, 1)
-        t += time.perf_counter() - t0
-        n += 1
-    rate = Ns ** 3 * n / t
+    st = O.State(p1, p2, z, z)
+    rate, n, t = timed(lambda k: st.run(p, A, k), 0.5 * seconds, Ns ** 3)
+    del A
+    sf = O.State(p1, p2, z, z)
+    f256, _, _ = timed(lambda k: sf.run_fused(p, k), 0.15 * seconds, Ns ** 3)
+    q1, q2 = kinv_field(2, 256, seed=1998)
+    z2 = np.zeros(256 ** 2)
+    p2d = O.params(2, 256)
+    A2 = O.Matrix(2, 256)
+    s2a, s2b = O.State(q1, q2, z2, z2), O.State(q1, q2, z2, z2)
+    r2d, _, _ = timed(lambda k: s2a.run(p2d, A2, 20 * k), 0.05 * seconds, 20 * 256 ** 2)
+    f2d, _, _ = timed(lambda k: s2b.run_fused(p2d, 20 * k), 0.05 * seconds, 20 * 256 ** 2)
+    del sf, s2a, s2b, A2
+    N5 = 512
+    rng = np.random.default_rng(1998)
+    z5 = np.zeros(N5 ** 3)
+    s5 = O.State(rng.standard_normal(N5 ** 3), rng.standard_normal(N5 ** 3), z5, z5)
+    f512, _, _ = timed(lambda k: s5.run_fused(O.params(3, N5), k), 0.2 * seconds, N5 ** 3)
+    del s5
     return {
         "value": rate,
         "unit": "site-updates/s",
@@ -331,6 +356,10 @@ def cpu_baseline(np, seconds: float):
         "sample": f"3D {Ns}^3 FP64, {n} reference-structured steps (drift, 2x CSR SpMV "
                   f"Laplacian, potential, kick, copy) of the CPU oracle, {t:.1f} s, "
                   f"OMP_NUM_THREADS={threads}",
+        "reference_structured_2d_256": r2d,
+        "fused_stencil": {"2d_256": f2d, "3d_256": f256, "3d_512": f512,
+                          "note": "two-sweep fused stepper, bit-identical to the reference "
+                                  "structure; 3d_512 stands in for 1024^3 per site"},
     }
 
 

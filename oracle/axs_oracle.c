@@ -281,6 +281,115 @@ void orc_vvsl_run(const orc_params *p, const orc_matrix *A, int64_t nsteps,
 }
 
 /* ---------------------------------------------------------------------- */
+/* vvsl_field_rescaled as a fused stencil stepper (CPU baseline).  The row
+ * sum runs in the matrix's COO order (centre; per distance l: +-first,
+ * +-second, +-third coordinate) with the same coefficients, alpha last,
+ * then the potential, kick and ker update of s_integrate.cpp:331-347 on
+ * that row -- each site's values depend only on its own ker/phidot and the
+ * drifted field, so the update is in place after the drift sweep.       */
+
+static void fused_row(int64_t m, double x1, double x2, double s1, double s2, double alpha,
+                      double c, double lam, double tau2, double T03, double ddt,
+                      double *phidot1, double *phidot2, double *ker1, double *ker2)
+{
+    double k1 = alpha * s1, k2 = alpha * s2;
+    double s = x1 * x1 + x2 * x2 - tau2 + T03;
+    k1 = k1 + c * lam * x1 * s;
+    k2 = k2 + c * lam * x2 * s;
+    phidot1[m] = phidot1[m] + 0.5 * (ker1[m] + k1) * ddt;
+    phidot2[m] = phidot2[m] + 0.5 * (ker2[m] + k2) * ddt;
+    ker1[m] = k1;
+    ker2[m] = k2;
+}
+
+void orc_vvsl_run_fused(const orc_params *p, int64_t nsteps,
+                        double *phi1, double *phi2, double *phidot1, double *phidot2,
+                        double *ker1, double *ker2, double *tau)
+{
+    double cf[5];
+    const int ncoef = stencil_coefficients(p->stencil_setting, p->ndims, cf);
+    const int64_t N = p->N, L = orc_length(p->ndims, p->N);
+    const float dt = p->time_step, dx = p->space_step;
+    const double ddt = (double)dt;
+    const double alpha = (double)1.0f / ((double)dx * (double)dx);
+    const double lam = (double)p->lambda, T0 = (double)p->T_initial;
+    const double T03 = (T0 * T0) / (double)3.0f;
+    for (int64_t step = 0; step < nsteps; step++) {
+        #pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < L; i++) {
+            phi1[i] = phi1[i] + ddt * (phidot1[i] + 0.5 * ker1[i] * ddt);
+            phi2[i] = phi2[i] + ddt * (phidot2[i] + 0.5 * ker2[i] * ddt);
+        }
+        *tau = *tau + ddt;
+        const double t = *tau, c = (double)(-1.0f) / (t * t), tau2 = t * t;
+        if (p->ndims == 2) {
+            /* m = x + N y; first coordinate x (fast), second y */
+            const int R = ncoef - 1;
+            #pragma omp parallel for schedule(static)
+            for (int64_t y = 0; y < N; y++) {
+                const int64_t row = N * y;
+                int64_t yb[4][2];    /* [l-1]: rows y+l, y-l */
+                for (int l = 1; l <= R; l++) {
+                    yb[l - 1][0] = N * periodic(y + l, N);
+                    yb[l - 1][1] = N * periodic(y - l, N);
+                }
+                for (int64_t x = 0; x < N; x++) {
+                    const int64_t m = x + row;
+                    const int edge = x < R || x >= N - R;
+                    double s1 = cf[0] * phi1[m], s2 = cf[0] * phi2[m];
+                    for (int l = 1; l <= R; l++) {
+                        const int64_t nb[4] = {edge ? row + periodic(x + l, N) : m + l,
+                                               edge ? row + periodic(x - l, N) : m - l,
+                                               yb[l - 1][0] + x, yb[l - 1][1] + x};
+                        for (int q = 0; q < 4; q++) {
+                            s1 = s1 + cf[l] * phi1[nb[q]];
+                            s2 = s2 + cf[l] * phi2[nb[q]];
+                        }
+                    }
+                    fused_row(m, phi1[m], phi2[m], s1, s2, alpha, c, lam, tau2, T03, ddt,
+                              phidot1, phidot2, ker1, ker2);
+                }
+            }
+        } else {
+            /* m = (x N + y) N + z; first coordinate x (slow), then y, z.
+             * Row bases of the x and y neighbours once per row; interior z
+             * without wrapping. */
+            const int R = ncoef - 1;
+            #pragma omp parallel for collapse(2) schedule(static)
+            for (int64_t x = 0; x < N; x++) {
+                for (int64_t y = 0; y < N; y++) {
+                    const int64_t row = (x * N + y) * N;
+                    int64_t nb[4][4];    /* [l-1]: x+l, x-l, y+l, y-l row bases */
+                    for (int l = 1; l <= R; l++) {
+                        nb[l - 1][0] = (periodic(x + l, N) * N + y) * N;
+                        nb[l - 1][1] = (periodic(x - l, N) * N + y) * N;
+                        nb[l - 1][2] = (x * N + periodic(y + l, N)) * N;
+                        nb[l - 1][3] = (x * N + periodic(y - l, N)) * N;
+                    }
+                    for (int64_t z = 0; z < N; z++) {
+                        const int64_t m = row + z;
+                        const int edge = z < R || z >= N - R;
+                        double s1 = cf[0] * phi1[m], s2 = cf[0] * phi2[m];
+                        for (int l = 1; l <= R; l++) {
+                            const int64_t zp = edge ? row + periodic(z + l, N) : m + l;
+                            const int64_t zm = edge ? row + periodic(z - l, N) : m - l;
+                            const int64_t q[6] = {nb[l - 1][0] + z, nb[l - 1][1] + z,
+                                                  nb[l - 1][2] + z, nb[l - 1][3] + z, zp, zm};
+                            for (int e = 0; e < 6; e++) {
+                                s1 = s1 + cf[l] * phi1[q[e]];
+                                s2 = s2 + cf[l] * phi2[q[e]];
+                            }
+                        }
+                        fused_row(m, phi1[m], phi2[m], s1, s2, alpha, c, lam, tau2, T03, ddt,
+                                  phidot1, phidot2, ker1, ker2);
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------- */
 /* compute_laplacian_fft, s_integrate.cpp:179-296: the field times
  * (-1)^(i+j[+k]) (shift2D/3D, :153-177), forward complex FFT, each mode
  * times -k^2 with k[i] = (-N/2 + i) 2 pi / L on every axis (:192, :242),

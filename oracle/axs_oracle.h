@@ -63,6 +63,16 @@ void orc_vvsl_run(const orc_params *p, const orc_matrix *A, int64_t nsteps,
                   double *ker1_curr, double *ker2_curr,
                   double *ker1_next, double *ker2_next, double *tau);
 
+/* The same step as a fused CPU stencil stepper (a CPU baseline, not the
+ * reference's structure): one drift sweep in place, then one sweep that
+ * evaluates the Laplacian matrix-free in the matrix's row order, the
+ * potential, the kick and the ker update -- two sweeps instead of five plus
+ * two SpMVs, no matrix, no ker_next arrays.  Bit-identical to orc_vvsl_run
+ * (same expression trees and summation order); ker must hold ker_curr.   */
+void orc_vvsl_run_fused(const orc_params *p, int64_t nsteps,
+                        double *phi1, double *phi2, double *phidot1, double *phidot2,
+                        double *ker1, double *ker2, double *tau);
+
 /* axion = atan2(phi1, phi2), src/standard/s_run.cpp:92-94 */
 void orc_axion(int64_t L, const double *phi1, const double *phi2, double *axion);
 

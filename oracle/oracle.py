@@ -47,6 +47,7 @@ def lib() -> ctypes.CDLL:
         "orc_laplacian3D": ([_DP, c_int, c_int, c_int, c_float, c_int], c_double),
         "orc_vvsl_step": ([POINTER(OrcParams), c_void_p] + [_DP] * 9, None),
         "orc_vvsl_run": ([POINTER(OrcParams), c_void_p, c_int64] + [_DP] * 9, None),
+        "orc_vvsl_run_fused": ([POINTER(OrcParams), c_int64] + [_DP] * 7, None),
         "orc_axion": ([c_int64, _DP, _DP, _DP], None),
         "orc_cores2": ([_DP, c_int, c_int, _IP, c_int64], c_int64),
         "orc_cores3": ([_DP, c_int, c_int, _IP, c_int64], c_int64),
@@ -138,6 +139,15 @@ class State:
         lib().orc_vvsl_run(ctypes.byref(p), A.h, int(nsteps), _dp(self.phi1), _dp(self.phi2),
                            _dp(self.phidot1), _dp(self.phidot2), _dp(self.ker1),
                            _dp(self.ker2), _dp(self._k1n), _dp(self._k2n), ctypes.byref(t))
+        self.tau = t.value
+        return self
+
+    def run_fused(self, p: OrcParams, nsteps: int) -> "State":
+        """the same steps with the fused CPU stencil stepper (no matrix)"""
+        t = c_double(self.tau)
+        lib().orc_vvsl_run_fused(ctypes.byref(p), int(nsteps), _dp(self.phi1), _dp(self.phi2),
+                                 _dp(self.phidot1), _dp(self.phidot2), _dp(self.ker1),
+                                 _dp(self.ker2), ctypes.byref(t))
         self.tau = t.value
         return self
 
