@@ -8,7 +8,7 @@ import pytest
 
 import axs
 import oracle as O
-from fields import random_state
+from fields import kinv_field, random_state, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -181,3 +181,22 @@ def test_autotune_higher_order(nd, N, precision):
     b.set_state(*state, tau=1.0)
     b.step(nsteps)
     _assert_same(a, b)
+
+
+@pytest.mark.parametrize("nd,N,stencil,nsteps", [(3, 64, 2, 200), (2, 256, 3, 300)])
+def test_fp32_form_accuracy_higher_order(nd, N, stencil, nsteps):
+    # the FP32 form (fused multiply-adds, centre term in the potential) on
+    # the multi-step kernels against the FP64 form on the same field
+    p1, p2 = kinv_field(nd, N, seed=1998)
+    z = np.zeros(N ** nd)
+    out = {}
+    for prec in ("fp32", "fp64"):
+        lat = axs.Lattice(nd, N, precision=prec, stencil_setting=stencil)
+        assert lat.kernel_config()["depth"] >= 1
+        lat.set_state(p1, p2, z, z, tau=1.0)
+        lat.step(nsteps)
+        out[prec] = (lat.get_state(), lat.strings(coords=False)[0])
+    for k in ("phi1", "phi2"):
+        assert rel_l2(out["fp32"][0][k], out["fp64"][0][k]) < 1e-4, k
+    n32, n64 = out["fp32"][1], out["fp64"][1]
+    assert abs(n32 - n64) <= max(1, 0.01 * n64)
