@@ -423,18 +423,17 @@ static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ct
         return fail(AXS_ERR_HIP);
     }
     // several steps per pass unless configured otherwise (the fastest
-    // shapes measured).  Standard stencil: FP32 3D four steps from side 256
-    // on, three below, 16-wave work-groups; FP64 3D three steps, 8-wave
-    // work-groups, two rows per wave; 2D six steps in both forms.  Higher
-    // orders: high_order_default_depth.
-    int min_side = 1 << 30;
-    for (const Member &m : c->members) min_side = std::min(min_side, m.N);
+    // shapes measured).  Standard stencil: FP32 3D four steps when the
+    // lattices give four-step tiles chunks of 32 planes or more (192^3 and
+    // up, ensembles), else three, 16-wave work-groups; FP64 3D three steps,
+    // 8-wave work-groups, two rows per wave; 2D six steps in both forms.
+    // Higher orders: high_order_default_depth.
     if (c->p.stencil_setting > 0)
         c->tune.depth = high_order_default_depth(c->nd, c->prec, c->p.stencil_setting);
     else if (c->nd == 2)
         c->tune.depth = 6;
-    else
-        c->tune.depth = (c->prec == AXS_FP32) ? ((min_side >= 256) ? 4 : 3) : 3;
+    else   // four steps, 16 waves of 2 rows: tiles of 24 x 56 sites
+        c->tune.depth = (c->prec == AXS_FP32) ? (multi_chunk_cap(c, 24, 56) >= 32 ? 4 : 3) : 3;
     if (c->nd == 3) c->tune.flags |= (c->prec == AXS_FP32) ? kTuneWG1024 : kTuneWG512;
     if (multi_depth(c) == 0) {
         c->tune.depth = 1;

@@ -227,6 +227,15 @@ bool multi_shape_built(int D, int W, int R, int prec, int RAD);
 int multi_waves(int flags);
 int multi_waves_of(const Context *c);
 int multi_rows(const Context *c, int N, int D, int W);
+// chunk length cap of the 3D multi-step kernels for tiles TJ x TK
+int multi_chunk_cap(const Context *c, int TJ, int TK);
+// largest divisor of N not above cap
+inline int divisor_below(int N, int cap)
+{
+    int d = cap < N ? cap : N;
+    while (N % d != 0) --d;
+    return d;
+}
 int high_order_rows(int prec, int RAD);
 int stencil_radius_of(const Context *c);
 // higher-order stencils (radius 2-4), 3D (axs_march2_hi.hip)

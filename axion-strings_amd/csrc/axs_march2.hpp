@@ -322,16 +322,13 @@ marchd_batch_kernel(const BatchMember *__restrict__ mem, int n_members, uint32_t
     }
 }
 
-// planes per chunk: the configured count if it divides N, else long enough
-// to amortise the halo planes and short enough to give the chip work
-// (measured best: 64 at 512^3 and 1024^3, 32 at 256^3, 8 at 128^3)
-inline int multi_chunk(const Context *c, int N)
+// planes per chunk of a member of side N, tiles TJ x TK: the configured
+// count if it divides N, else the ensemble-wide choice of multi_chunk_cap
+inline int multi_chunk(const Context *c, int N, int TJ, int TK)
 {
-    int ci = c->tune.ci32;
+    const int ci = c->tune.ci32;
     if (ci > 0 && N % ci == 0) return ci;
-    ci = (N >= 512) ? 64 : ((N >= 256) ? 32 : 8);
-    while (N % ci != 0) --ci;
-    return ci;
+    return divisor_below(N, multi_chunk_cap(c, TJ, TK));
 }
 
 template <class Sc>
@@ -361,7 +358,7 @@ struct SingleLaunch {
         g.N = N;
         g.nTj = (N + TJ - 1) / TJ;
         g.nTk = (N + TK - 1) / TK;
-        g.CI = multi_chunk(c, N);
+        g.CI = multi_chunk(c, N, TJ, TK);
         const uint32_t nb = (uint32_t)g.nTj * (uint32_t)g.nTk * (uint32_t)(N / g.CI);
         g.remap = (nb % 8 == 0) ? 1 : 0;
         if (pf)

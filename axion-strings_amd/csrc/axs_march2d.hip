@@ -21,14 +21,6 @@ bool march2d_depth_built(int D, int RAD, int prec)
     return D >= 1 && D <= ((prec == AXS_FP64) ? 2 : 4);
 }
 
-// largest divisor of N not above `cap`
-static int divisor_below(int N, int cap)
-{
-    int cj = cap < N ? cap : N;
-    while (N % cj != 0) --cj;
-    return cj;
-}
-
 // rows per chunk for every member: the configured count if it divides N;
 // else the longest chunk of 128, 64 or 32 rows (fewer halo rows per chunk)
 // that still deals the chip about 8 waves per SIMD (FP32; 4 in FP64, whose

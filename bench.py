@@ -149,7 +149,7 @@ ENSEMBLES = {
 }
 
 
-def ensemble_leg(np, axs, name, T0, steps_override=0):
+def ensemble_leg(np, axs, name, T0, steps_override=0, autotune=True):
     """All members advance together (one persistent launch per pass of
     several steps: the 3D and 2D multi-step kernels' work lists), launch
     shape from the autotuner; in-loop sampling of every realization (string
@@ -170,8 +170,11 @@ def ensemble_leg(np, axs, name, T0, steps_override=0):
         for r in range(len(Ns)):
             lat.init("kinv", seed=1998 + r, member=r)
         lat.tau = 1.0
-        cfg, tuned_ms = lat.autotune(trials=3)
-        log(f"[bench] {name} {prec}: {len(Ns)} members, tuned {cfg} ({tuned_ms:.3f} ms)")
+        if autotune:
+            cfg, tuned_ms = lat.autotune(trials=3)
+            log(f"[bench] {name} {prec}: {len(Ns)} members, tuned {cfg} ({tuned_ms:.3f} ms)")
+        else:
+            cfg = lat.kernel_config()
         step_s = sample_s = 0.0
         kern_ms_sum = 0.0
         done = nsamples = 0
@@ -407,6 +410,8 @@ def main() -> int:
                     help="skip the higher-order Laplacian legs")
     ap.add_argument("--skip-ensembles", action="store_true",
                     help="skip the 2D (512-member) and 3D (48-member) ensemble legs")
+    ap.add_argument("--no-ensemble-autotune", action="store_true",
+                    help="run the ensembles in the library's default launch shape")
     ap.add_argument("--ensemble-steps", type=int, default=0,
                     help="override the ensembles' step counts (2000 / 1500)")
     ap.add_argument("--depth", type=int, default=0,
@@ -476,7 +481,8 @@ def main() -> int:
         ens = {}
         for name in ENSEMBLES:
             try:
-                ens[name] = ensemble_leg(np, axs, name, args.T_initial, args.ensemble_steps)
+                ens[name] = ensemble_leg(np, axs, name, args.T_initial, args.ensemble_steps,
+                                         not args.no_ensemble_autotune)
             except Exception as e:          # reported, the headline line still prints
                 ens[name] = {"error": str(e)}
 
