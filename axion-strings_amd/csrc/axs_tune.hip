@@ -202,7 +202,7 @@ int run_autotune(Context *c, int trials, double *best_ms)
             // higher-order stencils, 3D: the multi-step kernel at one or two
             // steps per pass in its built shape
             const int W = (c->prec == AXS_FP64) ? 8 : 16;
-            const int chunk_opts3[3] = {0, 16, 32};
+            const int chunk_opts3[3] = {0, 32, 64};
             for (int d = 1; d <= 2; ++d)
                 for (int pf = 0; pf < 2; ++pf)
                     for (int ci = 0; ci < 3; ++ci) {
@@ -249,7 +249,7 @@ int run_autotune(Context *c, int trials, double *best_ms)
             for (const auto &x : shapes32) shapes.push_back({x[0], x[1], x[2]});
         else
             for (const auto &x : shapes64) shapes.push_back({x[0], x[1], x[2]});
-        const int chunk_opts2[3] = {0, 16, 32};
+        const int chunk_opts2[3] = {0, 32, 64};
         for (const auto &sh : shapes)
             for (int pf = 0; pf < 2; ++pf)
                 for (int ci = 0; ci < 3; ++ci) {
