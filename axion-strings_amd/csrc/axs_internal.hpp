@@ -19,10 +19,6 @@ namespace axs {
 
 void set_error(const char *fmt, ...);
 
-struct Status {
-    int code;
-};
-
 #define AXS_HIP_TRY(expr)                                                     \
     do {                                                                      \
         hipError_t e_ = (expr);                                               \
@@ -73,12 +69,6 @@ StepScalarsF to_float(const StepScalars &s);
 
 // most time steps one multi-step launch takes (2D strips: 8, 3D: 4)
 constexpr int kMaxDepth = 8;
-
-// scalars of the D steps of one multi-step launch (tau+dt, tau+2dt, ...)
-template <int D>
-struct TBScalars {
-    StepScalarsF s[D];
-};
 
 // ---------------------------------------------------------------------------
 // one lattice of the context
