@@ -819,10 +819,9 @@ int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
         bool fits = cfg->kernel != AXS_KERNEL_SIMPLE && march2d_depth_built(depth, RAD, c->prec);
         for (const Member &m : c->members) fits = fits && march2d_supported(c, m.N);
         if (!fits) {
-            set_error("depth %d in 2D needs the stencil Laplacian (field-rescaled form for "
-                      "stencil_setting > 0), sides >= 64, a non-simple kernel and a built "
-                      "depth (stencil 0: 2, 3, 4, 6, 8; higher orders: 1-4 FP32, 1-2 FP64)",
-                      depth);
+            set_error("depth %d in 2D needs the stencil Laplacian, sides >= 64, a non-simple "
+                      "kernel and a built depth (stencil 0: 2, 3, 4, 6, 8; higher orders: 1-4 "
+                      "FP32, 1-2 FP64)", depth);
             return AXS_ERR_UNSUPPORTED;
         }
     } else if (depth >= 2) {
@@ -835,9 +834,8 @@ int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
         bool fits = cfg->kernel != AXS_KERNEL_SIMPLE && depth <= (RAD > 1 ? 2 : 4);
         for (const Member &m : c->members) fits = fits && march2_supported(c, m.N);
         if (!fits) {
-            set_error("depth %d needs 3D, the stencil Laplacian (field-rescaled form for "
-                      "stencil_setting > 0), sides >= 64, a non-simple kernel and depth <= 4 "
-                      "(stencil 0) or <= 2 (higher orders)", depth);
+            set_error("depth %d needs 3D, the stencil Laplacian, sides >= 64, a non-simple "
+                      "kernel and depth <= 4 (stencil 0) or <= 2 (higher orders)", depth);
             return AXS_ERR_UNSUPPORTED;
         }
     } else if (RAD > 1) {

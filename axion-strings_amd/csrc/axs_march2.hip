@@ -8,8 +8,7 @@ namespace axs {
 
 bool march2_supported(const Context *c, int N)
 {
-    return c->nd == 3 && c->p.laplacian == AXS_LAPLACIAN_STENCIL && N >= 64 &&
-           (c->p.stencil_setting == 0 || c->p.eom == AXS_EOM_FIELD_RESCALED);
+    return c->nd == 3 && c->p.laplacian == AXS_LAPLACIAN_STENCIL && N >= 64;
 }
 
 // shapes built per form and stencil radius (none spills registers).

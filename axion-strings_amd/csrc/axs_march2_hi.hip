@@ -1,8 +1,8 @@
 // axs_march2_hi.hip -- the 3D multi-step kernels (axs_march2.hpp) for the
 // higher-order stencils (stencil_setting 1-3: radius 2-4, s_integrate.cpp:
-// 9-14), field-rescaled form: FP32 one or two steps per pass (radius 4:
-// one) with 16 waves of 2 rows, FP64 one step with 8 waves of 2 rows (radius
-// 2: or 3).
+// 9-14): FP32 one or two steps per pass (radius 4: one) with 16 waves of
+// 2 rows; FP64 (field-rescaled or Hamiltonian) one step with 8 waves of 2
+// rows (radius 2: or 3).
 #include "axs_march2.hpp"
 
 namespace axs {
@@ -55,11 +55,8 @@ static int launch_tr(Context *c, int src, int dst, const StepScalars *s, int dep
 
 int launch_multi_high_order(Context *c, int src, int dst, const StepScalars *s, int depth)
 {
-    if (c->p.eom != AXS_EOM_FIELD_RESCALED) {
-        set_error("the higher-order stencils' multi-step kernels take the field-rescaled form");
-        return AXS_ERR_UNSUPPORTED;
-    }
     if (c->prec == AXS_FP32) return launch_tr<F32Form>(c, src, dst, s, depth);
+    if (c->p.eom == AXS_EOM_HAMILTONIAN) return launch_tr<F64FormT<true>>(c, src, dst, s, depth);
     return launch_tr<F64FormT<false>>(c, src, dst, s, depth);
 }
 

@@ -8,8 +8,7 @@ namespace axs {
 
 bool march2d_supported(const Context *c, int N)
 {
-    return c->nd == 2 && c->p.laplacian == AXS_LAPLACIAN_STENCIL && N >= kW2 &&
-           (c->p.stencil_setting == 0 || c->p.eom == AXS_EOM_FIELD_RESCALED);
+    return c->nd == 2 && c->p.laplacian == AXS_LAPLACIAN_STENCIL && N >= kW2;
 }
 
 // radius 1: 2, 3, 4, 6 or 8 steps per pass; radii 2-4: 1-4 steps (FP32),

@@ -1,6 +1,6 @@
 // axs_march2d_hi.hip -- the 2D multi-step kernels (axs_march2d.hpp) for the
 // higher-order stencils (stencil_setting 1-3: radius 2-4, s_integrate.cpp:
-// 9-14), field-rescaled form: FP32 1-4 steps per pass, FP64 1-2.
+// 9-14): FP32 1-4 steps per pass, FP64 (field-rescaled or Hamiltonian) 1-2.
 #include "axs_march2d.hpp"
 
 namespace axs {
@@ -39,11 +39,8 @@ static int launch_tr(Context *c, int src, const StepScalars *s, int depth, bool 
 
 int launch_march2d_high_order(Context *c, int src, const StepScalars *s, int depth, bool pf)
 {
-    if (c->p.eom != AXS_EOM_FIELD_RESCALED) {
-        set_error("the higher-order stencils' multi-step kernels take the field-rescaled form");
-        return AXS_ERR_UNSUPPORTED;
-    }
     if (c->prec == AXS_FP32) return launch_tr<F32Form>(c, src, s, depth, pf);
+    if (c->p.eom == AXS_EOM_HAMILTONIAN) return launch_tr<F64FormT<true>>(c, src, s, depth, pf);
     return launch_tr<F64FormT<false>>(c, src, s, depth, pf);
 }
 

@@ -211,12 +211,13 @@ typedef struct axs_kernel_config {
     int chunk;    /* planes marched per work-group (divides N)              */
     int depth;    /* time steps per pass over the lattice: 1 (0 = 1), or,
                      with stencil 0, the stencil Laplacian and every side
-                     >= 64: 3D FP32 2 to 4 (a new context starts at 4, or
-                     3 if a side is below 256, with AXS_MARCH_WG1024), 3D
-                     FP64 2 or 3 with AXS_MARCH_WG512 (starts at 3), 2D
-                     2, 3, 4, 6 or 8 (starts at 6; rows 0, no work-group
-                     flags).  stencil_setting 1-3 (field-rescaled form,
-                     sides >= 64, kernel not SIMPLE): the multi-step kernels
+                     >= 64: 3D FP32 2 to 4 (a new context starts at 4
+                     when its lattices fill the chip in 32-plane chunks --
+                     192^3 and up, ensembles -- else 3, with
+                     AXS_MARCH_WG1024), 3D FP64 2 or 3 with AXS_MARCH_WG512
+                     (starts at 3), 2D 2, 3, 4, 6 or 8 (starts at 6; rows
+                     0, no work-group flags).  stencil_setting 1-3 (sides
+                     >= 64, kernel not SIMPLE): the multi-step kernels
                      run from depth 1 -- 3D FP32 1 or 2 (stencil 3: 1), 3D
                      FP64 1, 2D FP32 1-4, 2D FP64 1 or 2; a new context
                      starts at 3D FP32 2 (stencil 3: 1), FP64 1, 2D FP32 4

@@ -144,7 +144,8 @@ def test_default_configuration(nd, N, stencil, precision, depth, flags):
     (2, 2, "fp64", 3, 0, PF | NT, {}),
     (2, 1, "fp32", 5, 0, PF | NT, {}),
     (2, 1, "fp32", 1, 0, PF | NT | WG512, {}),
-    (3, 1, "fp64", 2, 0, PF | NT | WG512, {"eom": "hamiltonian"})])
+    (3, 1, "fp64", 2, 0, PF | NT | WG512, {"eom": "hamiltonian"}),
+    (2, 2, "fp64", 3, 0, PF | NT, {"eom": "hamiltonian"})])
 def test_unbuilt_configurations_rejected(nd, stencil, precision, depth, rows, flags, kw):
     lat = axs.Lattice(nd, 64, precision=precision, stencil_setting=stencil, **kw)
     before = lat.kernel_config()
@@ -153,13 +154,15 @@ def test_unbuilt_configurations_rejected(nd, stencil, precision, depth, rows, fl
     assert lat.kernel_config() == before
 
 
-def test_hamiltonian_form_higher_order_uses_one_step_kernels():
-    # the multi-step kernels of the higher orders take the field-rescaled
-    # form; the Hamiltonian form keeps the one-step kernels, bit for bit
-    nd, N, stencil, nsteps = 3, 64, 1, 3
+@pytest.mark.parametrize("nd,N,stencil,depth,nsteps", [(3, 64, 1, 1, 3), (3, 80, 3, 1, 2),
+                                                   (2, 128, 2, 2, 7), (2, 64, 3, 1, 5)])
+def test_hamiltonian_form_higher_order_multi_step_bit_exact(nd, N, stencil, depth, nsteps):
+    # the Hamiltonian form (vvsl_hamiltonian_form, s_integrate.cpp:358-395)
+    # with the higher-order stencils on the multi-step kernels, bit for bit
     state = random_state(nd, N, seed=90, ker=True)
     lat = axs.Lattice(nd, N, eom="hamiltonian", stencil_setting=stencil)
-    assert lat.kernel_config()["depth"] == 1
+    lat.set_kernel("march", depth=depth, flags=PF | NT | (WG512 if nd == 3 else 0))
+    assert lat.kernel_config()["depth"] == depth
     lat.set_state(*state, tau=1.0)
     lat.step(nsteps)
     p = O.params(nd, N, stencil_setting=stencil)
