@@ -67,6 +67,38 @@ __device__ __forceinline__ void store_nt(V *p, V v)
     __builtin_nontemporal_store(v, p);
 }
 
+// Value of the next (lane + 1) or previous (lane - 1) lane of the wave, by
+// the whole-wave DPP shifts (wave_shl:1, wave_shr:1): a register operand,
+// no LDS crossbar, no address.  The end lanes (63, 0) read zero.
+__device__ __forceinline__ uint32_t dpp_next_u32(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xf, 0xf, true);
+}
+__device__ __forceinline__ uint32_t dpp_prev_u32(uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x138, 0xf, 0xf, true);
+}
+__device__ __forceinline__ float lane_next(float v)
+{
+    return __builtin_bit_cast(float, dpp_next_u32(__builtin_bit_cast(uint32_t, v)));
+}
+__device__ __forceinline__ float lane_prev(float v)
+{
+    return __builtin_bit_cast(float, dpp_prev_u32(__builtin_bit_cast(uint32_t, v)));
+}
+__device__ __forceinline__ double lane_next(double v)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint64_t lo = dpp_next_u32((uint32_t)b), hi = dpp_next_u32((uint32_t)(b >> 32));
+    return __builtin_bit_cast(double, lo | (hi << 32));
+}
+__device__ __forceinline__ double lane_prev(double v)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint64_t lo = dpp_prev_u32((uint32_t)b), hi = dpp_prev_u32((uint32_t)(b >> 32));
+    return __builtin_bit_cast(double, lo | (hi << 32));
+}
+
 // The FP32 form's arithmetic, shared by every FP32 kernel so that one-step
 // and multi-step launches give identical bits.  The force is regrouped as
 //     ker = alpha (c0 x + sum_m c_m n_m) + cl x S

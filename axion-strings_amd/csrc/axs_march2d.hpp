@@ -114,10 +114,17 @@ __device__ __forceinline__ void march2d_strip(const int N, const int CJ, const i
                 T xr[RAD], yr[RAD], xl[RAD], yl[RAD];
 #pragma unroll
                 for (int m = 1; m <= RAD; ++m) {
-                    xr[m - 1] = __shfl_down(x, m);
-                    yr[m - 1] = __shfl_down(y, m);
-                    xl[m - 1] = __shfl_up(x, m);
-                    yl[m - 1] = __shfl_up(y, m);
+                    if (RAD == 1) {   // one lane away: DPP wave shifts
+                        xr[m - 1] = lane_next(x);
+                        yr[m - 1] = lane_next(y);
+                        xl[m - 1] = lane_prev(x);
+                        yl[m - 1] = lane_prev(y);
+                    } else {
+                        xr[m - 1] = __shfl_down(x, m);
+                        yr[m - 1] = __shfl_down(y, m);
+                        xl[m - 1] = __shfl_up(x, m);
+                        yl[m - 1] = __shfl_up(y, m);
+                    }
                 }
                 {
                     T a = Tr::centre(s, x), c = Tr::centre(s, y);
