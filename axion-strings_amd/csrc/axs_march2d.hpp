@@ -114,17 +114,11 @@ __device__ __forceinline__ void march2d_strip(const int N, const int CJ, const i
                 T xr[RAD], yr[RAD], xl[RAD], yl[RAD];
 #pragma unroll
                 for (int m = 1; m <= RAD; ++m) {
-                    if (RAD == 1) {   // one lane away: DPP wave shifts
-                        xr[m - 1] = lane_next(x);
-                        yr[m - 1] = lane_next(y);
-                        xl[m - 1] = lane_prev(x);
-                        yl[m - 1] = lane_prev(y);
-                    } else {
-                        xr[m - 1] = __shfl_down(x, m);
-                        yr[m - 1] = __shfl_down(y, m);
-                        xl[m - 1] = __shfl_up(x, m);
-                        yl[m - 1] = __shfl_up(y, m);
-                    }
+                    // m lanes away: m chained one-lane DPP wave shifts
+                    xr[m - 1] = lane_next(m == 1 ? x : xr[m - 2]);
+                    yr[m - 1] = lane_next(m == 1 ? y : yr[m - 2]);
+                    xl[m - 1] = lane_prev(m == 1 ? x : xl[m - 2]);
+                    yl[m - 1] = lane_prev(m == 1 ? y : yl[m - 2]);
                 }
                 {
                     T a = Tr::centre(s, x), c = Tr::centre(s, y);
