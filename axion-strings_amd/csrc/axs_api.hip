@@ -334,12 +334,20 @@ static int validate_params(const axs_params *p, int N)
 }
 
 // steps per pass of the higher-order stencils' multi-step kernels unless
-// configured otherwise (measured best at 512^3 and 4096^2): 3D FP32 two
-// (radius 4: one), FP64 one; 2D FP32 four at radius 2, else two, FP64 two
+// configured otherwise (measured best at 512^3 and 8192^2): 3D FP32 two
+// (radius 4: one), FP64 one; 2D FP32 four, three, two at radius 2, 3, 4,
+// FP64 two
 static int high_order_default_depth(int nd, int prec, int stencil)
 {
-    if (nd == 2) return (prec == AXS_FP32 && stencil == 1) ? 4 : 2;
+    if (nd == 2) return (prec == AXS_FP32) ? 5 - stencil : 2;
     return (prec == AXS_FP32 && stencil < 3) ? 2 : 1;
+}
+
+static int64_t total_sites(const Context *c)
+{
+    int64_t n = 0;
+    for (const Member &m : c->members) n += m.L;
+    return n;
 }
 
 static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ctx **out)
@@ -426,12 +434,12 @@ static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ct
     // shapes measured).  Standard stencil: FP32 3D four steps when the
     // lattices give four-step tiles chunks of 32 planes or more (192^3 and
     // up, ensembles), else three, 16-wave work-groups; FP64 3D three steps,
-    // 8-wave work-groups, two rows per wave; 2D six steps in both forms.
-    // Higher orders: high_order_default_depth.
+    // 8-wave work-groups, two rows per wave; 2D six steps (FP32 from 2^25
+    // sites: eight).  Higher orders: high_order_default_depth.
     if (c->p.stencil_setting > 0)
         c->tune.depth = high_order_default_depth(c->nd, c->prec, c->p.stencil_setting);
-    else if (c->nd == 2)
-        c->tune.depth = 6;
+    else if (c->nd == 2)   // FP32 from 2^25 sites on: eight (13% at 8192^2)
+        c->tune.depth = (c->prec == AXS_FP32 && total_sites(c) >= (1LL << 25)) ? 8 : 6;
     else   // four steps, 16 waves of 2 rows: tiles of 24 x 56 sites
         c->tune.depth = (c->prec == AXS_FP32) ? (multi_chunk_cap(c, 24, 56) >= 32 ? 4 : 3) : 3;
     if (c->nd == 3) c->tune.flags |= (c->prec == AXS_FP32) ? kTuneWG1024 : kTuneWG512;

@@ -215,13 +215,13 @@ typedef struct axs_kernel_config {
                      when its lattices fill the chip in 32-plane chunks --
                      192^3 and up, ensembles -- else 3, with
                      AXS_MARCH_WG1024), 3D FP64 2 or 3 with AXS_MARCH_WG512
-                     (starts at 3), 2D 2, 3, 4, 6 or 8 (starts at 6; rows
-                     0, no work-group flags).  stencil_setting 1-3 (sides
+                     (starts at 3), 2D 2, 3, 4, 6 or 8 (starts at 6, FP32
+                     from 2^25 sites at 8; rows 0, no work-group flags).  stencil_setting 1-3 (sides
                      >= 64, kernel not SIMPLE): the multi-step kernels
                      run from depth 1 -- 3D FP32 1 or 2 (stencil 3: 1), 3D
                      FP64 1, 2D FP32 1-4, 2D FP64 1 or 2; a new context
-                     starts at 3D FP32 2 (stencil 3: 1), FP64 1, 2D FP32 4
-                     (stencil 1) or 2, FP64 2.
+                     starts at 3D FP32 2 (stencil 3: 1), FP64 1, 2D FP32 4,
+                     3, 2 (stencil 1, 2, 3), FP64 2.
                      Step counts that are not a multiple end with one-step
                      launches.                                              */
     int flags;    /* AXS_MARCH_* bits                                       */

@@ -126,7 +126,8 @@ def test_multi_step_ensembles_match_one_step(nd, Ns, stencil, precision, depth, 
 @pytest.mark.parametrize("nd,N,stencil,precision,depth,flags", [
     (3, 64, 1, "fp32", 2, PF | NT | WG1024), (3, 64, 3, "fp32", 1, PF | NT | WG1024),
     (3, 64, 1, "fp64", 1, PF | NT | WG512), (3, 64, 3, "fp64", 1, PF | NT | WG512),
-    (2, 64, 1, "fp32", 4, PF | NT), (2, 64, 2, "fp32", 2, PF | NT), (2, 64, 3, "fp64", 2, PF | NT),
+    (2, 64, 1, "fp32", 4, PF | NT), (2, 64, 2, "fp32", 3, PF | NT), (2, 64, 3, "fp32", 2, PF | NT),
+    (2, 64, 3, "fp64", 2, PF | NT),
     (3, 48, 1, "fp32", 1, PF | NT), (2, 48, 2, "fp64", 1, PF | NT)])
 def test_default_configuration(nd, N, stencil, precision, depth, flags):
     # the multi-step kernel from side 64 on, in its fastest measured depth

@@ -288,3 +288,11 @@ def test_fp32_default_depth_follows_the_work(Ns, depth):
     # chunks of 32 planes or more (192^3 and up, ensembles), else three
     lat = axs.Lattice(3, members=Ns, precision="fp32")
     assert lat.kernel_config()["depth"] == depth
+
+
+@pytest.mark.parametrize("Ns,precision,depth", [([4096], "fp32", 6), ([8192], "fp32", 8),
+                                                ([8192], "fp64", 6), ([1024] * 32, "fp32", 8)])
+def test_2d_default_depth(Ns, precision, depth):
+    # FP32 2D contexts of 2^25 sites and more start at eight steps per pass
+    lat = axs.Lattice(2, members=Ns, precision=precision)
+    assert lat.kernel_config()["depth"] == depth
