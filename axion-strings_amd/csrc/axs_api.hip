@@ -252,8 +252,9 @@ Context::~Context()
     if (strbuf) (void)hipFree(strbuf);
     for (int h = 0; h < 2; ++h)
         if (batch_tab[h]) (void)hipFree(batch_tab[h]);
-    for (int h = 0; h < 2; ++h)
-        if (mbatch_tab[h]) (void)hipFree(mbatch_tab[h]);
+    for (MultiTables &t : mtab)
+        for (int h = 0; h < 2; ++h)
+            if (t.tab[h]) (void)hipFree(t.tab[h]);
     if (pinned) (void)hipHostFree(pinned);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -655,6 +656,16 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
         }
     if (nsteps == 0) return AXS_OK;
     AXS_TRY(prepare_batch(c));
+    // multi-step passes of `depth` steps; a remainder runs in shallower
+    // multi-step passes where one is built, else one step at a time (their
+    // work-item tables are built here, before any graph capture)
+    const int depth = multi_depth(c);
+    if (depth >= 1)
+        for (int64_t r = nsteps % depth; r > 0;) {
+            const int d = remainder_depth(c, (int)r);
+            if (d >= 1) AXS_TRY(prepare_multi_batch(c, d));
+            r -= (d >= 1) ? d : 1;
+        }
     const double dt = (double)c->p.time_step;
     // the spectral step's rocFFT calls stay out of graph capture
     const bool graph = (c->tune.flags & kTuneGraph) && nsteps > 1 &&
@@ -669,21 +680,21 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
     else
         AXS_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     int rc = AXS_OK;
-    // multi-step passes of `depth` steps; a remainder runs one step at a time
-    const int depth = multi_depth(c);
     for (int64_t n = 0; n < nsteps && rc == AXS_OK;) {
         double tau_next = c->tau + dt;                    // s_integrate.cpp:317
         StepScalars s = make_scalars(c->p, tau_next);
-        if (depth >= 1 && nsteps - n >= depth) {
+        const int64_t left = nsteps - n;
+        const int d = (depth < 1) ? 0 : (left >= depth ? depth : remainder_depth(c, (int)left));
+        if (d >= 1) {
             StepScalars ss[kMaxDepth];
             double t = c->tau;
-            for (int l = 0; l < depth; ++l) {
+            for (int l = 0; l < d; ++l) {
                 t = t + dt;                               // the same sequence of sums
                 ss[l] = make_scalars(c->p, t);
             }
-            rc = launch_multi_steps(c, ss, depth);
+            rc = launch_multi_steps(c, ss, d);
             if (rc == AXS_OK) c->tau = t;
-            n += depth;
+            n += d;
         } else {
             rc = launch_step(c, s);
             if (rc == AXS_OK) c->tau = tau_next;

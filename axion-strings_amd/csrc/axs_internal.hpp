@@ -174,8 +174,15 @@ struct Context {
     uint32_t batch_items = 0;
     int batch_key = -1;          // rows * 100000 + chunk the tables were built for
     bool batch_march = false;    // 3D ensemble on the march kernel
-    // the same for the multi-step kernel (tiles of its own shape)
-    BatchMember *mbatch_tab[2] = {nullptr, nullptr};
+    // work-item tables of the multi-step kernels for each number of steps
+    // per pass in use (the configured one and the shallower passes that
+    // finish a step count), both parities
+    struct MultiTables {
+        int key = -1;
+        BatchMember *tab[2] = {nullptr, nullptr};
+        uint32_t items = 0;
+    };
+    MultiTables mtab[kMaxDepth + 1];
     // string finder results of the last call (ballot words, block counts,
     // offsets), reused while the member's state is unchanged
     void *strbuf = nullptr;
@@ -183,8 +190,6 @@ struct Context {
     int str_member = -1;
     uint64_t str_version = 0;
     uint64_t str_total = 0;
-    uint32_t mbatch_items = 0;
-    int mbatch_key = -1;
 
     ~Context();
 };
@@ -209,6 +214,13 @@ bool march2_supported(const Context *c, int N);
 // steps per launch of the multi-step kernels for this configuration, or 0
 // when the one-step kernels run (radius 1 at depth 1, simple kernel, ...)
 int multi_depth(const Context *c);
+// the same for D steps per pass instead of the configured number
+int multi_depth_for(const Context *c, int D);
+// steps per pass of the multi-step pass that finishes a remainder of r < D
+// steps (the deepest usable one, at most r), or 0: one-step launches
+int remainder_depth(const Context *c, int r);
+// forget the work-item tables (the launch shape changed)
+void invalidate_multi_tables(Context *c);
 // time steps per launch the configuration gives every member (>= 1)
 int multi_step_depth(const Context *c);
 // the multi-step kernel is built for depth D, W waves, R rows per wave,
@@ -234,11 +246,11 @@ int launch_multi_high_order(Context *c, int src, int dst, const StepScalars *s, 
 // per member
 int launch_multi_steps(Context *c, const StepScalars *s, int depth);
 // work-item tables of the multi-step kernel for an ensemble
-int prepare_multi_batch(Context *c);
+int prepare_multi_batch(Context *c, int D);
 // 2D multi-step kernel (axs_march2d.hip)
 bool march2d_supported(const Context *c, int N);
 bool march2d_depth_built(int D, int RAD, int prec);
-int prepare_march2d_items(Context *c);
+int prepare_march2d_items(Context *c, int D);
 int launch_march2d(Context *c, int src, const StepScalars *s, int depth);
 // higher-order stencils (radius 2-4), 2D (axs_march2d_hi.hip)
 int launch_march2d_high_order(Context *c, int src, const StepScalars *s, int depth, bool pf);

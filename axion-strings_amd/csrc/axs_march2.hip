@@ -146,10 +146,24 @@ static int launch_form(Context *c, int src, int dst, const StepScalars *s, int d
     return launch_rad1<Tr, 2>(c, src, dst, s);
 }
 
-int multi_depth(const Context *c)
+int multi_depth(const Context *c) { return multi_depth_for(c, c->tune.depth); }
+
+int remainder_depth(const Context *c, int r)
 {
-    if (c->tune.kernel == AXS_KERNEL_SIMPLE) return 0;
-    const int D = c->tune.depth;
+    const int D = multi_depth(c);
+    for (int d = (r < D - 1 ? r : D - 1); d >= 1; --d)
+        if (multi_depth_for(c, d) == d) return d;
+    return 0;
+}
+
+void invalidate_multi_tables(Context *c)
+{
+    for (Context::MultiTables &t : c->mtab) t.key = -1;
+}
+
+int multi_depth_for(const Context *c, int D)
+{
+    if (c->tune.kernel == AXS_KERNEL_SIMPLE || D > kMaxDepth) return 0;
     const int RAD = stencil_radius_of(c);
     if (RAD == 1 ? D < 2 : D < 1) return 0;      // radius 1, depth 1: the one-step kernels
     if (c->nd == 2) {
@@ -171,18 +185,18 @@ int multi_step_depth(const Context *c)
     return D > 0 ? D : 1;
 }
 
-int prepare_multi_batch(Context *c)
+int prepare_multi_batch(Context *c, int D)
 {
-    if (c->nd == 2) return prepare_march2d_items(c);
-    if (c->members.size() < 2) return AXS_OK;
-    const int D = multi_depth(c);
     if (D < 1) return AXS_OK;
+    if (c->nd == 2) return prepare_march2d_items(c, D);
+    if (c->members.size() < 2) return AXS_OK;
+    Context::MultiTables &mt = c->mtab[D];
     const int RAD = stencil_radius_of(c);
     const int W = multi_waves_of(c);
     // one row count for the ensemble: that of its first member's side
     const int R = multi_rows(c, c->members[0].N, D, W);
     const int key = (((D * 10 + RAD) * 100 + W) * 100 + R) * 10000 + c->tune.ci32;
-    if (key == c->mbatch_key && c->mbatch_tab[0]) return AXS_OK;
+    if (key == mt.key && mt.tab[0]) return AXS_OK;
     const int TJ = W * R - 2 * D * RAD, TK = kTBW - 2 * D * RAD;
     std::vector<BatchMember> tab[2];
     uint32_t items = 0;
@@ -204,15 +218,15 @@ int prepare_multi_batch(Context *c)
         }
     }
     for (int h = 0; h < 2; ++h) {
-        if (!c->mbatch_tab[h])
-            AXS_HIP_TRY(hipMalloc(&c->mbatch_tab[h], c->members.size() * sizeof(BatchMember)));
-        AXS_HIP_TRY(hipMemcpyAsync(c->mbatch_tab[h], tab[h].data(),
+        if (!mt.tab[h])
+            AXS_HIP_TRY(hipMalloc(&mt.tab[h], c->members.size() * sizeof(BatchMember)));
+        AXS_HIP_TRY(hipMemcpyAsync(mt.tab[h], tab[h].data(),
                                    tab[h].size() * sizeof(BatchMember), hipMemcpyHostToDevice,
                                    c->stream));
     }
     AXS_HIP_TRY(hipStreamSynchronize(c->stream));
-    c->mbatch_items = items;
-    c->mbatch_key = key;
+    mt.items = items;
+    mt.key = key;
     return AXS_OK;
 }
 

@@ -385,9 +385,10 @@ struct BatchLaunch {
         int per_cu = 0;
         AXS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, 0));
         const uint32_t cap = (uint32_t)c->num_cu * (uint32_t)(per_cu > 0 ? per_cu : 1);
-        const uint32_t grid = c->mbatch_items < cap ? c->mbatch_items : cap;
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * W), 0, c->stream, c->mbatch_tab[src],
-                           (int)c->members.size(), c->mbatch_items, sc);
+        const Context::MultiTables &mt = c->mtab[D];
+        const uint32_t grid = mt.items < cap ? mt.items : cap;
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * W), 0, c->stream, mt.tab[src],
+                           (int)c->members.size(), mt.items, sc);
         return AXS_OK;
     }
 };
@@ -401,7 +402,7 @@ int launch_members(Context *c, int src, int dst, const StepScalars *s, int W, in
     LevelScalars<typename Tr::Sc, D> sc;
     for (int l = 0; l < D; ++l) sc.s[l] = level_scalars<typename Tr::Sc>(s[l]);
     const bool pf = (c->tune.flags & kTunePrefetch) != 0;
-    if (c->members.size() > 1 && c->mbatch_tab[src]) {
+    if (c->members.size() > 1 && c->mtab[D].tab[src]) {
         BatchLaunch<Tr, D, RAD> f{c, src, sc, pf};
         return dispatch(W, Rbatch, f);
     }

@@ -38,13 +38,13 @@ static int pick_cap(const Context *c, int TI)
     return 32;
 }
 
-int prepare_march2d_items(Context *c)
+int prepare_march2d_items(Context *c, int D)
 {
-    const int D = multi_depth(c);
     if (D < 1) return AXS_OK;
+    Context::MultiTables &mt = c->mtab[D];
     const int RAD = stencil_radius_of(c);
     const int key = ((D * 10 + RAD) * 100 + 2) * 10000 + c->tune.ci32;
-    if (key == c->mbatch_key && c->mbatch_tab[0]) return AXS_OK;
+    if (key == mt.key && mt.tab[0]) return AXS_OK;
     const int TI = kW2 - 2 * D * RAD;
     const int cap = pick_cap(c, TI);
     std::vector<BatchMember> tab[2];
@@ -68,19 +68,15 @@ int prepare_march2d_items(Context *c)
         }
     }
     for (int h = 0; h < 2; ++h) {
-        if (c->mbatch_tab[h]) {
-            (void)hipStreamSynchronize(c->stream);
-            (void)hipFree(c->mbatch_tab[h]);
-            c->mbatch_tab[h] = nullptr;
-        }
-        AXS_HIP_TRY(hipMalloc(&c->mbatch_tab[h], c->members.size() * sizeof(BatchMember)));
-        AXS_HIP_TRY(hipMemcpyAsync(c->mbatch_tab[h], tab[h].data(),
+        if (!mt.tab[h])
+            AXS_HIP_TRY(hipMalloc(&mt.tab[h], c->members.size() * sizeof(BatchMember)));
+        AXS_HIP_TRY(hipMemcpyAsync(mt.tab[h], tab[h].data(),
                                    tab[h].size() * sizeof(BatchMember), hipMemcpyHostToDevice,
                                    c->stream));
     }
     AXS_HIP_TRY(hipStreamSynchronize(c->stream));
-    c->mbatch_items = items;
-    c->mbatch_key = key;
+    mt.items = items;
+    mt.key = key;
     return AXS_OK;
 }
 
@@ -101,7 +97,7 @@ static int launch_form2d(Context *c, int src, const StepScalars *s, int depth, b
 
 int launch_march2d(Context *c, int src, const StepScalars *s, int depth)
 {
-    if (!c->mbatch_tab[src]) {
+    if (depth < 1 || depth > kMaxDepth || !c->mtab[depth].tab[src]) {
         set_error("2D multi-step work list missing");
         return AXS_ERR_STATE;
     }

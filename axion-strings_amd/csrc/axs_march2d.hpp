@@ -236,10 +236,11 @@ int launch_march2d_pass(Context *c, int src, const StepScalars *s, bool pf)
     int per_cu = 0;
     AXS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * kWaves2D, 0));
     const uint32_t cap = (uint32_t)c->num_cu * (uint32_t)(per_cu > 0 ? per_cu : 1);
-    const uint32_t need = (c->mbatch_items + kWaves2D - 1) / kWaves2D;
+    const Context::MultiTables &mt = c->mtab[D];
+    const uint32_t need = (mt.items + kWaves2D - 1) / kWaves2D;
     const uint32_t grid = need < cap ? need : cap;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kWaves2D), 0, c->stream, c->mbatch_tab[src],
-                       (int)c->members.size(), c->mbatch_items, sc);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kWaves2D), 0, c->stream, mt.tab[src],
+                       (int)c->members.size(), mt.items, sc);
     return AXS_OK;
 }
 

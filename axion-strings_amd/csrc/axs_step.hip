@@ -176,7 +176,7 @@ int prepare_batch(Context *c)
 {
     // the 2D multi-step kernel runs from a work list even for one lattice
     if (c->members.size() < 2)
-        return (c->nd == 2 && multi_depth(c) >= 1) ? prepare_march2d_items(c) : AXS_OK;
+        return (c->nd == 2) ? prepare_march2d_items(c, multi_depth(c)) : AXS_OK;
     // the spectral step runs member by member (rocFFT per lattice)
     if (c->p.laplacian != AXS_LAPLACIAN_STENCIL) return AXS_OK;
     // the Hamiltonian form's ensembles use the site kernel
@@ -185,7 +185,7 @@ int prepare_batch(Context *c)
     const int rows = march_rows(c);
     int key = march ? rows * 100000 + ((c->prec == AXS_FP32) ? c->tune.ci32 : c->tune.ci64) : 0;
     if (key == c->batch_key && c->batch_tab[0])
-        return (multi_depth(c) >= 1) ? prepare_multi_batch(c) : AXS_OK;
+        return prepare_multi_batch(c, multi_depth(c));
     std::vector<BatchMember> tab[2];
     uint32_t items = 0;
     for (const Member &m : c->members) {
@@ -221,7 +221,7 @@ int prepare_batch(Context *c)
     c->batch_items = items;
     c->batch_key = key;
     c->batch_march = march;
-    if (multi_depth(c) >= 1) AXS_TRY(prepare_multi_batch(c));
+    AXS_TRY(prepare_multi_batch(c, multi_depth(c)));
     return AXS_OK;
 }
 

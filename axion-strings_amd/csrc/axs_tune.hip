@@ -284,7 +284,7 @@ int run_autotune(Context *c, int trials, double *best_ms)
         if (rc != AXS_OK) {
             c->tune = saved;
             c->batch_key = -1;
-            c->mbatch_key = -1;
+            invalidate_multi_tables(c);
             return rc;
         }
         if (ms < best_t) {
@@ -294,7 +294,7 @@ int run_autotune(Context *c, int trials, double *best_ms)
     }
     c->tune = best;
     c->batch_key = -1;          // tables follow the chosen shape
-    c->mbatch_key = -1;
+    invalidate_multi_tables(c);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tune_cache[key] = {best, best_t};

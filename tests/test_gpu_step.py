@@ -175,7 +175,7 @@ def test_multi_step_kernel_matches_one_step(N, depth, rows, chunk, flags, nsteps
     # expressions as one-step launches: identical bits on every side (also
     # sides the tiles do not divide), launch shape (4, 8 or 16 waves per
     # work-group: AXS_MARCH_WG512 / AXS_MARCH_WG1024) and step count (a
-    # count that is not a multiple of the depth ends with one-step launches)
+    # count that is not a multiple of the depth ends with shallower passes)
     state = random_state(3, N, seed=11, ker=True)
     a = axs.Lattice(3, N, precision="fp32")
     a.set_kernel("simple")
