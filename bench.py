@@ -271,6 +271,64 @@ def stencils_leg(np, axs, N, T0, steps=10):
     return out
 
 
+# The reference's own small configurations as whole runs (FP64, field from
+# the host k^-1 generator, sampling at the stated cadence): 2D 256^2, 500
+# steps, strings every 50; 3D 256^3, 1000 steps, strings and energy every 20
+WORKFLOWS = {
+    "config0_2d_256": (2, 256, 500, 50, False),
+    "config1_3d_256": (3, 256, 1000, 20, True),
+}
+
+
+def workflow_leg(np, axs, T0, cpu_rate_3d=None):
+    """Wall time of each configuration run on the GPU (steps plus the
+    in-loop sampling, as the driver makes the calls), and of config 0 on
+    the CPU oracle; config 1 on the CPU is estimated from the measured
+    reference-structured step rate (steps only, no sampling)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle as O
+    from fields import kinv_field
+    out = {}
+    for name, (nd, N, steps, every, energy) in WORKFLOWS.items():
+        p1, p2 = kinv_field(nd, N, seed=1998)
+        z = np.zeros(N ** nd)
+        lat = axs.Lattice(nd, N, precision="fp64", T_initial=T0)
+        lat.set_state(p1, p2, z, z, tau=1.0)
+        lat.step(1)                                   # code objects, first touch
+        lat.set_state(p1, p2, z, z, tau=1.0)
+        t0 = time.perf_counter()
+        nstr = 0
+        for done in range(0, steps, every):
+            nstr = lat.strings(coords=False)[0]
+            if energy:
+                lat.energy()
+            lat.step(every)
+        gpu_s = time.perf_counter() - t0
+        lat.close()
+        rec = {"ndims": nd, "N": N, "steps": steps, "sample_every": every,
+               "gpu_s": gpu_s, "gpu_site_updates_per_s": N ** nd * steps / gpu_s,
+               "last_string_count": int(nstr)}
+        if nd == 2:
+            st = O.State(p1, p2, z, z)
+            pp, A = O.params(2, N, T_initial=T0), O.Matrix(2, N)
+            t0 = time.perf_counter()
+            for done in range(0, steps, every):
+                O.cores(O.axion(st.phi1, st.phi2), 2, N)
+                st.run(pp, A, every)
+            rec["cpu_s"] = time.perf_counter() - t0
+            rec["cpu_kind"] = "oracle, reference structure, measured"
+        elif cpu_rate_3d:
+            rec["cpu_s"] = N ** nd * steps / cpu_rate_3d
+            rec["cpu_kind"] = "oracle step rate at 256^3 x steps (sampling not included)"
+        if "cpu_s" in rec:
+            rec["speedup"] = rec["cpu_s"] / gpu_s
+        out[name] = rec
+        log(f"[bench] {name}: GPU {gpu_s:.3f} s" +
+            (f", CPU {rec['cpu_s']:.2f} s" if "cpu_s" in rec else ""))
+    return out
+
+
 def snapshot_leg(np, axs, N, T0, steps=200):
     """Field snapshot of the 1024^3 FP32 run (phi1, phi2 of every site,
     8.6 GB each as float64 on the host): the synchronous read-back against
@@ -406,6 +464,8 @@ def main() -> int:
                     help="skip the asynchronous-snapshot overlap leg")
     ap.add_argument("--skip-forms", action="store_true",
                     help="skip the Hamiltonian-form and spectral-Laplacian legs")
+    ap.add_argument("--skip-workflows", action="store_true",
+                    help="skip the whole-run timings of the 256^2 and 256^3 configurations")
     ap.add_argument("--skip-stencils", action="store_true",
                     help="skip the higher-order Laplacian legs")
     ap.add_argument("--skip-ensembles", action="store_true",
@@ -514,6 +574,13 @@ def main() -> int:
         log("[bench] CPU baseline (oracle)")
         cpu = cpu_baseline(np, args.cpu_seconds)
 
+    workflows = None
+    if not args.skip_workflows:
+        try:
+            workflows = workflow_leg(np, axs, args.T_initial, cpu["value"] if cpu else None)
+        except Exception as e:              # reported, the headline line still prints
+            workflows = {"error": str(e)}
+
     line = {
         "metric": "lattice site-updates/s (3D 1024^3, FP32 form; FP64 form alongside)",
         "value": r32["site_updates_per_s"],
@@ -555,6 +622,7 @@ def main() -> int:
         "forms": forms_out,
         "stencils": stencils,
         "snapshot": snap,
+        "workflows": workflows,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
