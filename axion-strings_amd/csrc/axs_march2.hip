@@ -39,17 +39,21 @@ int multi_waves(int flags)
 
 int stencil_radius_of(const Context *c) { return c->p.stencil_setting + 1; }
 
-// Longest chunk (64, 32, 16 or 8 planes: fewer halo planes per chunk) that
-// still deals the chip enough work items over all members: three per four
-// CUs for FP32, four per CU for FP64.  Measured: FP32 64 at 256^3 and up
-// and for the 48-lattice ensemble (25% faster than 8 and 32 per member),
-// 32 at 192^3, 8 at 128^3; FP64 64 at 512^3 and for the ensemble, 32 at
-// 256^3, 8 at 128^3.
+// Longest chunk (FP32 up to 256 planes, FP64 up to 64, at most half the
+// smallest side: fewer halo planes per chunk) that still deals the chip
+// enough work items over all members: three per four CUs for FP32, four
+// per CU for FP64.  Measured: FP32 256 at 512^3 and 1024^3 (7% faster than
+// 64 at 1024^3), 64 at 256^3 and for the 48-lattice ensemble, 32 at 192^3,
+// 8 at 128^3; FP64 64 at 512^3 and up and for the ensemble (128 slower),
+// 32 at 256^3, 8 at 128^3.
 int multi_chunk_cap(const Context *c, int TJ, int TK)
 {
-    const uint64_t target = (c->prec == AXS_FP32) ? (uint64_t)c->num_cu * 3 / 4
-                                                   : (uint64_t)c->num_cu * 4;
-    for (int cap = 64; cap > 8; cap >>= 1) {
+    const bool f32 = c->prec == AXS_FP32;
+    const uint64_t target = f32 ? (uint64_t)c->num_cu * 3 / 4 : (uint64_t)c->num_cu * 4;
+    int min_side = 1 << 30;
+    for (const Member &m : c->members) min_side = m.N < min_side ? m.N : min_side;
+    for (int cap = f32 ? 256 : 64; cap > 8; cap >>= 1) {
+        if (2 * cap > min_side) continue;
         uint64_t items = 0;
         for (const Member &m : c->members)
             items += (uint64_t)((m.N + TJ - 1) / TJ) * (uint64_t)((m.N + TK - 1) / TK) *
