@@ -7,15 +7,18 @@
 // is always in ascending site order (the single-thread order), which also
 // fixes the 2D core count of s_run.cpp:101-109.
 //
-// Pass 0 computes each site's angle (atan2(phi1, phi2) + pi) / (2 pi) in
-// FP64 once -- the reference's axion array (s_run.cpp:92-94) scaled as in
-// string_finding.cpp:14-16 -- so the FP64 atan2 runs once per site rather
-// than once per plaquette corner.  Pass 1 evaluates the plaquette test per
-// site from those angles and stores one 64-bit ballot word per wave (a
-// bitmask of flagged sites) plus a count per block; pass 2 scans the block
-// counts; pass 3 re-reads the ballot words and writes the coordinates at
-// their ordered positions.  Passes 0-2 are skipped when the same member's
-// unchanged state was searched by the previous call.
+// Each site's angle (atan2(phi1, phi2) + pi) / (2 pi) is computed in FP64
+// once -- the reference's axion array (s_run.cpp:92-94) scaled as in
+// string_finding.cpp:14-16 -- rather than once per plaquette corner, and
+// the plaquette test leaves one 64-bit ballot word (a bitmask of flagged
+// sites) per 64 sites in site order.  3D: one pass over phi
+// (strings_march_kernel) keeps the angles of two planes of a tile in LDS,
+// then a small kernel counts the flags per block.  2D: pass 0 writes the
+// angles to a scratch array, pass 1 evaluates the plaquettes from it and
+// counts.  Then one work-group scans the block counts, and the writer
+// re-reads the ballot words and writes the coordinates at their ordered
+// positions.  The search is skipped when the same member's unchanged state
+// was searched by the previous call.
 #include "axs_internal.hpp"
 #include "axs_lattice.hpp"
 
@@ -25,6 +28,34 @@ constexpr int kStrBlock = 256;
 constexpr int kStrIters = 8;                    // sites per thread per block
 constexpr int kStrSpan = kStrBlock * kStrIters; // sites per block
 
+// std::min(|b-a|, 1-|b-a|), string_finding.cpp:22-24
+__device__ __forceinline__ double edge_theta(double a, double b)
+{
+    double d = fabs(b - a);
+    double e = 1 - d;
+    return (e < d) ? e : d;
+}
+
+// the three plaquettes of string_finding.cpp:49-62 at site (i, j, k) from
+// the scaled angles of (i,j,k), (i+1,j,k), (i+1,j+1,k), (i,j+1,k),
+// (i,j+1,k+1), (i,j,k+1); plaquette b uses corner (i, j+1, k+1)
+__device__ __forceinline__ bool plaquettes3(double n0, double ni, double nij, double nj,
+                                            double njk, double nk, double accept)
+{
+    double sa = edge_theta(n0, ni) + edge_theta(ni, nij) + edge_theta(nij, nj);
+    double sb = edge_theta(n0, ni) + edge_theta(ni, njk) + edge_theta(njk, nk);
+    double sc = edge_theta(n0, nj) + edge_theta(nj, njk) + edge_theta(njk, nk);
+    return sa > accept || sb > accept || sc > accept;
+}
+
+// the reference's axion (s_run.cpp:94) scaled to [0, 1]
+// (string_finding.cpp:14-16)
+__device__ __forceinline__ double scaled_angle(double a1, double a2)
+{
+    double a = atan2(a1, a2);
+    return (a + M_PI) / (2 * M_PI);
+}
+
 template <class P>
 __global__ void __launch_bounds__(256)
 angle_kernel(uint32_t L, P phi, double *__restrict__ ang)
@@ -33,17 +64,8 @@ angle_kernel(uint32_t L, P phi, double *__restrict__ ang)
     if (m < L) {
         double a1, a2;
         phi.get(m, a1, a2);
-        double a = atan2(a1, a2);
-        ang[m] = (a + M_PI) / (2 * M_PI);
+        ang[m] = scaled_angle(a1, a2);
     }
-}
-
-// std::min(|b-a|, 1-|b-a|), string_finding.cpp:22-24
-__device__ __forceinline__ double edge_theta(double a, double b)
-{
-    double d = fabs(b - a);
-    double e = 1 - d;
-    return (e < d) ? e : d;
 }
 
 template <int ND>
@@ -72,16 +94,7 @@ __device__ __forceinline__ bool plaquette_flag(const double *__restrict__ ang, u
         Site<ND> sj = decompose<ND>(qj, N);
         uint32_t qjk = shifted<ND>(sj, 2, 1, N);                   // (i,   j+1, k+1)
         uint32_t qk = shifted<ND>(s, 2, 1, N);                     // (i,   j,   k+1)
-        double n0 = ang[m];
-        double ni = ang[qi];
-        double nij = ang[qij];
-        double nj = ang[qj];
-        double njk = ang[qjk];
-        double nk = ang[qk];
-        double sa = edge_theta(n0, ni) + edge_theta(ni, nij) + edge_theta(nij, nj);
-        double sb = edge_theta(n0, ni) + edge_theta(ni, njk) + edge_theta(njk, nk);
-        double sc = edge_theta(n0, nj) + edge_theta(nj, njk) + edge_theta(njk, nk);
-        return sa > accept || sb > accept || sc > accept;
+        return plaquettes3(ang[m], ang[qi], ang[qij], ang[qj], ang[qjk], ang[qk], accept);
     }
 }
 
@@ -111,33 +124,260 @@ strings_flag_kernel(int N, uint32_t L, const double *__restrict__ ang, double ac
     }
 }
 
-// exclusive scan of n block counts by one workgroup (1024 work-items),
-// offsets[n] receives the total
-__global__ void __launch_bounds__(1024)
-scan_counts_kernel(const uint32_t *__restrict__ cnt, uint64_t *__restrict__ off, uint32_t n)
+// 3D finder in one pass over phi.  A work-group of kSfW waves owns a tile
+// of kSfTJ rows (j) x 64 columns (k) and marches along i over a chunk of
+// planes.  Per plane it computes the angles of kSfTJ + 1 rows x 65 columns
+// (the +1 row and column are the plaquettes' j+1 / k+1 corners) into LDS
+// -- kSfTJ + 2 = 32 passes of 64 lanes (the 65th column is one pass of its
+// own), four per wave -- and tests the plaquettes of plane i from the LDS
+// planes i and i+1.  Each output row of 64 sites is one ballot word,
+// stored whole when N is a multiple of 64 (the row is then one aligned
+// word) and OR-ed into the zeroed words otherwise.
+//
+// Screening.  The LDS angles are FP32 (atan2f of the components rounded
+// to float).  A plaquette sum is a sum of three circular distances
+// min(|b-a|, 1-|b-a|), each 1-Lipschitz in both angles and blind to the
+// branch cut, so it is off the FP64 sum by at most six angle errors plus
+// the float rounding of the sums: below 2e-6 (atan2f within a few ulp,
+// inputs within half an ulp), against the margin kSfMargin = 2^-14.  A
+// plaquette above accept + margin is flagged, one below accept - margin
+// is not; a site whose plaquettes are all undecided that way (or NaN), or
+// that touches an input outside [1e-30, 1e30] in magnitude (where the
+// float angle could be off: zeros, underflow, overflow, NaN), is decided
+// by the FP64 angles of its six corners with the two-pass finder's
+// expressions.  The flags are therefore those of the FP64 test, bit for
+// bit; the FP64 path runs for a handful of sites per lattice.
+constexpr int kSfW = 8;
+constexpr int kSfTJ = 30;
+constexpr float kSfMargin = 1.0f / 16384.0f;
+
+struct SfGeom {
+    int N, nTj, nTk, CI;
+};
+
+// float components of phi at m; false where the float angle is not
+// trusted (both components below 1e-30 in magnitude, either above 1e30,
+// or NaN)
+__device__ __forceinline__ bool phi_float(const PhiF64 &p, uint32_t m, float &x, float &y)
 {
-    __shared__ uint64_t part[1024];
-    uint32_t per = (n + 1023) / 1024;
-    uint32_t b = threadIdx.x * per, e = min(n, b + per);
-    uint64_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += cnt[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t acc = 0;
-        for (int t = 0; t < 1024; ++t) {
-            uint64_t v = part[t];
-            part[t] = acc;
-            acc += v;
+    dvec2 v = p.p[m];
+    double ax = fabs(v.x), ay = fabs(v.y);
+    x = (float)v.x;
+    y = (float)v.y;
+    return (ax >= 1e-30 || ay >= 1e-30) && ax <= 1e30 && ay <= 1e30;
+}
+__device__ __forceinline__ bool phi_float(const PhiF32 &p, uint32_t m, float &x, float &y)
+{
+    fvec4 v = p.p[m];
+    float ax = fabsf(v.x), ay = fabsf(v.y);
+    x = v.x;
+    y = v.y;
+    return (ax >= 1e-30f || ay >= 1e-30f) && ax <= 1e30f && ay <= 1e30f;
+}
+
+__device__ __forceinline__ float edge_theta_f(float a, float b)
+{
+    float d = fabsf(b - a);
+    return fminf(d, 1.0f - d);
+}
+
+template <class P, bool ALIGNED>
+__global__ void __launch_bounds__(64 * kSfW)
+strings_march_kernel(SfGeom g, P phi, double accept, unsigned long long *__restrict__ words)
+{
+    __shared__ float A[2][kSfTJ + 1][65];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int N = g.N;
+    uint32_t b = blockIdx.x;
+    const int tk = (int)(b % (uint32_t)g.nTk);
+    b /= (uint32_t)g.nTk;
+    const int tj = (int)(b % (uint32_t)g.nTj);
+    b /= (uint32_t)g.nTj;
+    const int i0 = (int)b * g.CI;
+    const int i1 = (i0 + g.CI < N) ? i0 + g.CI : N;
+    const int j0 = tj * kSfTJ, k0 = tk * 64;
+    const float hi = (float)accept + kSfMargin, lo = (float)accept - kSfMargin;
+    const float two_pi_inv = (float)(1.0 / (2 * M_PI)), pi_f = (float)M_PI;
+
+    // periodic wrap of an index below N + 64
+    auto wrap = [N](int x) {
+        while (x >= N) x -= N;
+        return x;
+    };
+    auto site = [N](int i, int j, int k) {
+        return ((uint32_t)i * (uint32_t)N + (uint32_t)j) * (uint32_t)N + (uint32_t)k;
+    };
+    auto angle_d = [&](int i, int j, int k) {
+        double a1, a2;
+        phi.get(site(i, j, k), a1, a2);
+        return scaled_angle(a1, a2);
+    };
+    // the four angle passes of this wave per plane: pass q = wave + kSfW t
+    // below kSfTJ + 1 is row j0 + q (column k0 + lane), pass kSfTJ + 1 is
+    // column k0 + 64 of row j0 + lane (lanes up to kSfTJ); in-plane offsets
+    // j N + k and LDS slots, fixed for the whole march
+    constexpr int kPass = (kSfTJ + 2) / kSfW;
+    static_assert(kPass * kSfW == kSfTJ + 2, "angle passes must fill the waves evenly");
+    uint32_t off[kPass];
+    int slot[kPass];
+    bool live[kPass];
+    for (int t = 0; t < kPass; ++t) {
+        const int q = wave + kSfW * t;
+        const bool row = q <= kSfTJ;
+        const int j = wrap(j0 + (row ? q : lane)), k = wrap(k0 + (row ? lane : 64));
+        off[t] = (uint32_t)j * (uint32_t)N + (uint32_t)k;
+        slot[t] = row ? q * 65 + lane : lane * 65 + 64;
+        live[t] = row || lane <= kSfTJ;
+    }
+    const uint32_t NN = (uint32_t)N * (uint32_t)N;
+    // raw components of one plane (in flight while the previous plane is
+    // processed) and their FP32 angles into LDS plane s (NaN where the float
+    // angle is not trusted)
+    struct Raw {
+        float x[kPass], y[kPass];
+        bool ok[kPass];
+    };
+    auto fetch = [&](int i, Raw &r) {
+        for (int t = 0; t < kPass; ++t)
+            r.ok[t] = live[t] && phi_float(phi, (uint32_t)i * NN + off[t], r.x[t], r.y[t]);
+    };
+    auto store = [&](const Raw &r, int s) {
+        float *a = &A[s][0][0];
+        for (int t = 0; t < kPass; ++t) {
+            if (!live[t]) continue;
+            const float th = (atan2f(r.x[t], r.y[t]) + pi_f) * two_pi_inv;
+            a[slot[t]] = r.ok[t] ? th : __builtin_nanf("");
         }
-        off[n] = acc;
+    };
+
+    int s = 0;
+    Raw cur, nxt;
+    fetch(i0, cur);
+    store(cur, 0);
+    fetch(i0 + 1 == N ? 0 : i0 + 1, cur);
+    for (int i = i0; i < i1; ++i) {
+        const int ip = (i + 1 == N) ? 0 : i + 1;
+        if (i + 2 <= i1) fetch(i + 2 >= N ? i + 2 - N : i + 2, nxt);
+        store(cur, s ^ 1);
+        __syncthreads();
+        for (int r = wave; r < kSfTJ; r += kSfW) {
+            const int j = j0 + r, k = k0 + lane;
+            if (j >= N) break;                          // uniform over the wave
+            const float n0 = A[s][r][lane], ni = A[s ^ 1][r][lane], nij = A[s ^ 1][r + 1][lane];
+            const float nj = A[s][r + 1][lane], njk = A[s][r + 1][lane + 1];
+            const float nk = A[s][r][lane + 1];
+            const float sa = edge_theta_f(n0, ni) + edge_theta_f(ni, nij) + edge_theta_f(nij, nj);
+            const float sb = edge_theta_f(n0, ni) + edge_theta_f(ni, njk) + edge_theta_f(njk, nk);
+            const float sc = edge_theta_f(n0, nj) + edge_theta_f(nj, njk) + edge_theta_f(njk, nk);
+            const bool in = k < N;
+            bool f = in && (sa > hi || sb > hi || sc > hi);
+            const bool undecided = in && !f && !(sa < lo && sb < lo && sc < lo);
+            if (__ballot(undecided)) {
+                if (undecided) {
+                    const int jp = wrap(j + 1), kp = wrap(k + 1);
+                    f = plaquettes3(angle_d(i, j, k), angle_d(ip, j, k), angle_d(ip, jp, k),
+                                    angle_d(i, jp, k), angle_d(i, jp, kp), angle_d(i, j, kp),
+                                    accept);
+                }
+            }
+            const unsigned long long w = __ballot(f);
+            const uint64_t m0 = ((uint64_t)i * (uint64_t)N + (uint64_t)j) * (uint64_t)N +
+                                (uint64_t)k0;
+            if (ALIGNED) {
+                if (lane == 0) words[m0 >> 6] = w;
+            } else if (lane == 0 && w) {
+                const unsigned sh = (unsigned)(m0 & 63);
+                atomicOr(&words[m0 >> 6], w << sh);
+                if (sh && (w >> (64 - sh))) atomicOr(&words[(m0 >> 6) + 1], w >> (64 - sh));
+            }
+        }
+        __syncthreads();
+        s ^= 1;
+        cur = nxt;
+    }
+}
+
+// flagged sites per block of kStrSpan sites (32 words), one word per lane
+__global__ void __launch_bounds__(256)
+block_counts_kernel(const uint64_t *__restrict__ words, uint64_t nwords, uint32_t *__restrict__ bcount)
+{
+    const uint64_t w = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    uint32_t c = (w < nwords) ? (uint32_t)__popcll(words[w]) : 0u;
+    for (int o = 16; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 31) == 0 && w < nwords) bcount[w >> 5] = c;
+}
+
+// Exclusive scan of the n block counts into off[0..n), off[n] = total, in
+// three launches: tile_scan_kernel scans tiles of kScanTile counts (four
+// per work-item) to tile-relative offsets and tile sums; top_scan_kernel
+// scans the tile sums (one per work-item: lattices hold fewer than 2^31
+// sites, so n <= 2^20 and there are at most 256 tiles) and writes the
+// total; add_tile_offsets_kernel adds each tile's offset.
+constexpr int kScanTile = 4096;
+
+// exclusive prefix of v over the 1024 work-items; *total gets the sum
+template <typename T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T *lds, T *total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        T y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds[wave] = x;
+    __syncthreads();
+    if (wave == 0) {
+        T w = (lane < 16) ? lds[lane] : (T)0;
+        T z = w;
+        for (int o = 1; o < 16; o <<= 1) {
+            T y = __shfl_up(z, o, 64);
+            if (lane >= o) z += y;
+        }
+        if (lane < 16) lds[lane] = z - w;                // exclusive over waves
+        if (lane == 15) lds[16] = z;
     }
     __syncthreads();
-    uint64_t acc = part[threadIdx.x];
-    for (uint32_t i = b; i < e; ++i) {
-        off[i] = acc;
-        acc += cnt[i];
+    *total = lds[16];
+    return lds[wave] + x - v;
+}
+
+__global__ void __launch_bounds__(1024)
+tile_scan_kernel(const uint32_t *__restrict__ cnt, uint64_t *__restrict__ off,
+                 uint64_t *__restrict__ tsum, uint32_t n)
+{
+    __shared__ uint32_t lds[17];
+    const uint32_t e0 = blockIdx.x * (uint32_t)kScanTile + 4u * threadIdx.x;
+    uint32_t c[4], s = 0;
+    for (int u = 0; u < 4; ++u) {
+        c[u] = (e0 + u < n) ? cnt[e0 + u] : 0u;
+        s += c[u];
     }
+    uint32_t tot;
+    uint32_t acc = block_exclusive_scan<uint32_t>(s, lds, &tot);
+    for (int u = 0; u < 4; ++u) {
+        if (e0 + u < n) off[e0 + u] = acc;
+        acc += c[u];
+    }
+    if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(1024)
+top_scan_kernel(uint64_t *__restrict__ tsum, uint32_t ntiles, uint64_t *__restrict__ total)
+{
+    __shared__ uint64_t lds[17];
+    const uint64_t v = (threadIdx.x < ntiles) ? tsum[threadIdx.x] : 0ull;
+    uint64_t tot;
+    const uint64_t x = block_exclusive_scan<uint64_t>(v, lds, &tot);
+    if (threadIdx.x < ntiles) tsum[threadIdx.x] = x;
+    if (threadIdx.x == 0) *total = tot;
+}
+
+__global__ void __launch_bounds__(256)
+add_tile_offsets_kernel(uint64_t *__restrict__ off, const uint64_t *__restrict__ toff, uint32_t n)
+{
+    const uint32_t e = blockIdx.x * 256u + threadIdx.x;
+    if (e < n) off[e] += toff[e / (uint32_t)kScanTile];
 }
 
 template <int ND>
@@ -197,6 +437,46 @@ static double accept_threshold(int thr)
     return (double)f;
 }
 
+// one-pass 3D finder: ballot words of every site, then the per-block
+// counts the scan and the coordinate writer use
+static int launch_strings_march(Context *c, int N, uint32_t L, const void *phi, double accept,
+                                uint64_t *words, uint32_t *bcount)
+{
+    SfGeom g;
+    g.N = N;
+    g.nTj = (N + kSfTJ - 1) / kSfTJ;
+    g.nTk = (N + 63) / 64;
+    const uint64_t tiles = (uint64_t)g.nTj * (uint64_t)g.nTk;
+    // longest chunk (64 planes down to 4) that still gives two work-groups
+    // per CU: the first plane of each chunk is the previous chunk's last
+    int CI = 64;
+    while (CI > 4 && tiles * (uint64_t)((N + CI - 1) / CI) < 2 * (uint64_t)c->num_cu) CI >>= 1;
+    g.CI = CI;
+    const uint32_t grid = (uint32_t)(tiles * (uint64_t)((N + CI - 1) / CI));
+    const uint64_t nwords = ((uint64_t)L + 63) / 64;
+    unsigned long long *w = (unsigned long long *)words;
+    if (N % 64 == 0) {
+        if (c->prec == AXS_FP64)
+            hipLaunchKernelGGL((strings_march_kernel<PhiF64, true>), dim3(grid), dim3(64 * kSfW),
+                               0, c->stream, g, PhiF64{(const dvec2 *)phi}, accept, w);
+        else
+            hipLaunchKernelGGL((strings_march_kernel<PhiF32, true>), dim3(grid), dim3(64 * kSfW),
+                               0, c->stream, g, PhiF32{(const fvec4 *)phi}, accept, w);
+    } else {
+        AXS_HIP_TRY(hipMemsetAsync(words, 0, nwords * sizeof(uint64_t), c->stream));
+        if (c->prec == AXS_FP64)
+            hipLaunchKernelGGL((strings_march_kernel<PhiF64, false>), dim3(grid), dim3(64 * kSfW),
+                               0, c->stream, g, PhiF64{(const dvec2 *)phi}, accept, w);
+        else
+            hipLaunchKernelGGL((strings_march_kernel<PhiF32, false>), dim3(grid), dim3(64 * kSfW),
+                               0, c->stream, g, PhiF32{(const fvec4 *)phi}, accept, w);
+    }
+    hipLaunchKernelGGL(block_counts_kernel, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0,
+                       c->stream, words, nwords, bcount);
+    AXS_HIP_TRY(hipGetLastError());
+    return AXS_OK;
+}
+
 int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *count,
                 int64_t *num_cores)
 {
@@ -210,7 +490,8 @@ int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *co
     size_t words_b = ((size_t)nblocks * kStrSpan / 64) * 8;
     size_t cnt_b = ((size_t)nblocks * 4 + 255) & ~(size_t)255;
     size_t off_b = ((size_t)(nblocks + 1) * 8 + 255) & ~(size_t)255;
-    size_t tail_b = 256;
+    const uint32_t ntiles = (nblocks + kScanTile - 1) / kScanTile;
+    size_t tail_b = 256 + (((size_t)ntiles * 8 + 255) & ~(size_t)255);
     const bool cached = c->str_member == member && c->str_version == m.version && c->strbuf &&
                         c->strbuf_bytes >= words_b + cnt_b + off_b + tail_b;
     if (!cached) {
@@ -222,6 +503,7 @@ int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *co
     uint32_t *bcount = (uint32_t *)(sp + words_b);
     uint64_t *off = (uint64_t *)(sp + words_b + cnt_b);
     unsigned long long *pairs = (unsigned long long *)(sp + words_b + cnt_b + off_b);
+    uint64_t *tsum = (uint64_t *)(sp + words_b + cnt_b + off_b + 256);
 
     uint64_t total = 0;
     if (cached) {
@@ -229,21 +511,28 @@ int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *co
     } else {
         // the angles need phi only (no FP32 velocity reconstruction)
         const void *phi = m.state[c->cur][PHI];
-        AXS_TRY(ensure_scratch(c, (size_t)L * sizeof(double)));
-        double *ang = (double *)c->scratch;
-        const unsigned ga = (L + 255) / 256;
-        if (c->prec == AXS_FP64)
-            hipLaunchKernelGGL(angle_kernel<PhiF64>, dim3(ga), dim3(256), 0, c->stream, L,
-                               PhiF64{(const dvec2 *)phi}, ang);
-        else
-            hipLaunchKernelGGL(angle_kernel<PhiF32>, dim3(ga), dim3(256), 0, c->stream, L,
-                               PhiF32{(const fvec4 *)phi}, ang);
-        double accept = accept_threshold(c->p.thr);
-        auto k = (c->nd == 2) ? strings_flag_kernel<2> : strings_flag_kernel<3>;
-        hipLaunchKernelGGL(k, dim3(nblocks), dim3(kStrBlock), 0, c->stream, m.N, L, ang, accept,
-                           words, bcount);
-        hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, c->stream, bcount, off,
-                           nblocks);
+        const double accept = accept_threshold(c->p.thr);
+        if (c->nd == 3) {
+            AXS_TRY(launch_strings_march(c, m.N, L, phi, accept, words, bcount));
+        } else {
+            AXS_TRY(ensure_scratch(c, (size_t)L * sizeof(double)));
+            double *ang = (double *)c->scratch;
+            const unsigned ga = (L + 255) / 256;
+            if (c->prec == AXS_FP64)
+                hipLaunchKernelGGL(angle_kernel<PhiF64>, dim3(ga), dim3(256), 0, c->stream, L,
+                                   PhiF64{(const dvec2 *)phi}, ang);
+            else
+                hipLaunchKernelGGL(angle_kernel<PhiF32>, dim3(ga), dim3(256), 0, c->stream, L,
+                                   PhiF32{(const fvec4 *)phi}, ang);
+            hipLaunchKernelGGL(strings_flag_kernel<2>, dim3(nblocks), dim3(kStrBlock), 0,
+                               c->stream, m.N, L, ang, accept, words, bcount);
+        }
+        hipLaunchKernelGGL(tile_scan_kernel, dim3(ntiles), dim3(1024), 0, c->stream, bcount, off,
+                           tsum, nblocks);
+        hipLaunchKernelGGL(top_scan_kernel, dim3(1), dim3(1024), 0, c->stream, tsum, ntiles,
+                           off + nblocks);
+        hipLaunchKernelGGL(add_tile_offsets_kernel, dim3((nblocks + 255) / 256), dim3(256), 0,
+                           c->stream, off, tsum, nblocks);
         AXS_HIP_TRY(hipGetLastError());
         AXS_HIP_TRY(hipMemcpyAsync(&total, off + nblocks, 8, hipMemcpyDeviceToHost, c->stream));
         AXS_HIP_TRY(hipStreamSynchronize(c->stream));

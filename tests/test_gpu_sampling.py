@@ -22,7 +22,7 @@ def _evolved(nd, N, nsteps, precision="fp64", seed=1998, **kw):
 
 
 @pytest.mark.parametrize("nd,N,nsteps", [(2, 256, 0), (2, 256, 50), (2, 256, 250), (2, 256, 499),
-                                         (3, 64, 0), (3, 64, 40), (3, 96, 100)])
+                                         (3, 64, 0), (3, 64, 40), (3, 96, 100), (3, 288, 2)])
 def test_strings_bit_exact(nd, N, nsteps):
     lat = _evolved(nd, N, nsteps)
     g = lat.get_state()
@@ -37,6 +37,96 @@ def test_strings_bit_exact(nd, N, nsteps):
     else:
         assert ncores[0] == n_ref
     print(f"{nd}D N={N} step {nsteps}: {n} flagged sites")
+
+
+@pytest.mark.parametrize("N,prec", [(8, "fp64"), (30, "fp64"), (32, "fp32"), (34, "fp64"),
+                                    (64, "fp32"), (66, "fp32"), (100, "fp64"), (130, "fp32")])
+def test_strings_3d_sides(N, prec):
+    """The 3D one-pass finder on random phases (dense strings) at sides below,
+    across and off its 64-column and 30-row tiles: the oracle's set in the
+    oracle's order, in both storage forms (the FP32 form against the oracle
+    on its own widened state)."""
+    rng = np.random.default_rng(N)
+    p1, p2 = rng.standard_normal(N ** 3), rng.standard_normal(N ** 3)
+    z = np.zeros(N ** 3)
+    lat = axs.Lattice(3, N, precision=prec)
+    lat.set_state(p1, p2, z, z, tau=1.0)
+    g = lat.get_state()
+    n_ref, c_ref = O.cores(O.axion(g["phi1"], g["phi2"]), 3, N, thr=1)
+    n, c = lat.strings()
+    assert n == n_ref and n > 0
+    np.testing.assert_array_equal(c, c_ref)
+    assert lat.string_counts()[0][0] == n_ref
+
+
+@pytest.mark.parametrize("N,prec", [(64, "fp64"), (66, "fp64"), (64, "fp32"), (66, "fp32")])
+def test_strings_3d_threshold_straddle(N, prec):
+    """Every plaquette sum at the acceptance threshold: the phase turns by
+    accept (float 0.495, thr = 1) of a circle per site along k, plus noise
+    of 1e-9 turns per site, so plaquettes b and c of every site sum to
+    accept within 1e-8 (with a slight upward bias from the two short edges):
+    most sites are flagged and some are not.  The one-pass finder's float screen cannot
+    decide any of them: each site goes through its FP64 fallback, which
+    must give the oracle's set."""
+    rng = np.random.default_rng(5)
+    k = np.arange(N ** 3) % N
+    accept = float(np.float32(0.495))
+    turns = accept * k + 1e-9 * rng.standard_normal(N ** 3)
+    p1, p2 = np.cos(2 * np.pi * turns), np.sin(2 * np.pi * turns)
+    z = np.zeros(N ** 3)
+    lat = axs.Lattice(3, N, precision=prec)
+    lat.set_state(p1, p2, z, z, tau=1.0)
+    g = lat.get_state()
+    n_ref, c_ref = O.cores(O.axion(g["phi1"], g["phi2"]), 3, N, thr=1)
+    n, c = lat.strings()
+    assert 0.05 * N ** 3 < n_ref < 0.99 * N ** 3
+    assert n == n_ref
+    np.testing.assert_array_equal(c, c_ref)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_strings_3d_zero_tiny_and_nan_inputs(prec):
+    """Sites whose float angle is not trusted (exact zeros of either sign,
+    components below 1e-30, NaN in FP64) take the FP64 fallback: the set
+    is still the oracle's."""
+    N = 64
+    rng = np.random.default_rng(11)
+    p1, p2 = kinv_field(3, N, seed=3)
+    L = N ** 3
+    pick = rng.choice(L, 4000, replace=False)
+    p1[pick[:1000]], p2[pick[:1000]] = 0.0, 0.0
+    p1[pick[1000:2000]], p2[pick[1000:2000]] = -0.0, -0.0
+    p1[pick[2000:3000]] = 1e-40 * rng.standard_normal(1000)
+    p2[pick[2000:3000]] = 1e-40 * rng.standard_normal(1000)
+    if prec == "fp64":
+        p1[pick[3000:3100]] = np.nan
+        p1[pick[3100:3200]], p2[pick[3100:3200]] = 1e-300, -3e-310
+    z = np.zeros(L)
+    lat = axs.Lattice(3, N, precision=prec)
+    lat.set_state(p1, p2, z, z, tau=1.0)
+    g = lat.get_state()
+    n_ref, c_ref = O.cores(O.axion(g["phi1"], g["phi2"]), 3, N, thr=1)
+    n, c = lat.strings()
+    assert n == n_ref and n > 0
+    np.testing.assert_array_equal(c, c_ref)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("N", [64, 40])
+def test_strings_3d_straight_lines(axis, N):
+    """A straight string along each axis (winding in the plane of the other
+    two), which exercises the three plaquettes separately."""
+    idx = np.indices((N, N, N), dtype=np.float64)
+    u, v = [idx[a] for a in range(3) if a != axis]
+    phase = np.arctan2(u - 10.5, v - 17.5)
+    p1, p2 = np.cos(phase).ravel(), np.sin(phase).ravel()
+    z = np.zeros(N ** 3)
+    lat = axs.Lattice(3, N)
+    lat.set_state(p1, p2, z, z, tau=1.0)
+    n_ref, c_ref = O.cores(O.axion(p1, p2), 3, N, thr=1)
+    n, c = lat.strings()
+    assert n == n_ref and n >= N
+    np.testing.assert_array_equal(c, c_ref)
 
 
 def test_strings_hand_built_vortices():
