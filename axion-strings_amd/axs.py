@@ -54,6 +54,7 @@ EXPORTS = (
     "axs_set_state", "axs_get_state", "axs_step", "axs_member_strings", "axs_strings",
     "axs_string_counts", "axs_member_background", "axs_background",
     "axs_member_energy", "axs_energy", "axs_member_powerspec", "axs_powerspec",
+    "axs_powerspec_all",
     "axs_last_step_ms", "axs_bytes_per_site", "axs_strerror", "axs_last_error",
     "axs_set_kernel_config", "axs_get_kernel_config", "axs_autotune", "axs_member_laplacian",
     "axs_snapshot_begin", "axs_snapshot_end",
@@ -141,6 +142,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "axs_member_energy": ([c_void_p, c_int, _DP], c_int),
         "axs_energy": ([c_void_p, _DP], c_int),
         "axs_member_powerspec": ([c_void_p, c_int, c_int, c_int, _DP, _DP, POINTER(c_int32)], c_int),
+        "axs_powerspec_all": ([c_void_p, c_int, c_int, _DP, _DP, POINTER(c_int32)], c_int),
         "axs_powerspec": ([c_void_p, c_int, c_int, _DP, _DP, POINTER(c_int32)], c_int),
         "axs_last_step_ms": ([c_void_p, _DP], c_int),
         "axs_bytes_per_site": ([c_void_p, _DP, _DP], c_int),
@@ -392,6 +394,24 @@ class Lattice:
                                              cnt.ctypes.data_as(POINTER(c_int32))),
                "axs_member_powerspec")
         return ks, pk, cnt
+
+    def powerspec_all(self, binning: str = "linear", nbins: int = 64):
+        """[(ks, pk, count)] of every member from one call (members of one side
+        share batched transforms)."""
+        b = AXS_BIN_LINEAR if binning == "linear" else AXS_BIN_LOG
+        nbs = [N // 2 if b == AXS_BIN_LINEAR else nbins for N in self.Ns]
+        tot = sum(nbs)
+        ks = np.zeros(tot, np.float64)
+        pk = np.zeros(tot, np.float64)
+        cnt = np.zeros(tot, np.int32)
+        _check(self.lib.axs_powerspec_all(self._ctx, b, nbins, _dp(ks), _dp(pk),
+                                          cnt.ctypes.data_as(POINTER(c_int32))),
+               "axs_powerspec_all")
+        out, o = [], 0
+        for nb in nbs:
+            out.append((ks[o:o + nb], pk[o:o + nb], cnt[o:o + nb]))
+            o += nb
+        return out
 
 
 # ---------------------------------------------------------------------------

@@ -764,14 +764,7 @@ int axs_strings(axs_ctx *ctx, int32_t *coords, int64_t cap, int64_t *count)
 int axs_string_counts(axs_ctx *ctx, int64_t *n_plaquettes, int64_t *num_cores)
 {
     AXS_TRY(check_member(ctx, 0));
-    Context *c = &ctx->impl;
-    for (size_t r = 0; r < c->members.size(); ++r) {
-        int64_t n = 0, nc = 0;
-        AXS_TRY(run_strings(c, c->members[r], nullptr, 0, &n, &nc));
-        if (n_plaquettes) n_plaquettes[r] = n;
-        if (num_cores) num_cores[r] = nc;
-    }
-    return AXS_OK;
+    return run_string_counts(&ctx->impl, n_plaquettes, num_cores);
 }
 
 int axs_member_background(axs_ctx *ctx, int member, double sums[6])
@@ -816,6 +809,16 @@ int axs_member_powerspec(axs_ctx *ctx, int member, int binning, int nbins, doubl
 int axs_powerspec(axs_ctx *ctx, int binning, int nbins, double *ks, double *pk, int32_t *count)
 {
     return axs_member_powerspec(ctx, 0, binning, nbins, ks, pk, count);
+}
+
+int axs_powerspec_all(axs_ctx *ctx, int binning, int nbins, double *ks, double *pk, int32_t *count)
+{
+    AXS_TRY(check_member(ctx, 0));
+    if (!ks || !pk || !count) {
+        set_error("null output");
+        return AXS_ERR_INVALID_ARG;
+    }
+    return run_powerspec_all(&ctx->impl, binning, nbins, ks, pk, count);
 }
 
 int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)

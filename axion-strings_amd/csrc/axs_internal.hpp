@@ -272,12 +272,15 @@ int prepare_batch(Context *c);
 // FP32: compute ker = F(phi, tau) and phidot = u - dt/2 ker into aux buffers
 int launch_fp32_canonical(Context *c, Member &m, const StepScalars &s, bool recompute_ker);
 
+int run_string_counts(Context *c, int64_t *n_plaquettes, int64_t *num_cores);
 int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *count,
                 int64_t *num_cores);
 int run_background(Context *c, Member &m, double out[6]);
 int run_init(Context *c, Member &m, int mode, uint64_t seed);
 int run_autotune(Context *c, int trials, double *best_ms);
 int run_energy(Context *c, Member &m, double out[3]);
+int run_powerspec_all(Context *c, int binning, int nbins, double *ks, double *pk,
+                      int32_t *count);
 int run_powerspec(Context *c, Member &m, int binning, int nbins, double *ks, double *pk,
                   int32_t *count);
 

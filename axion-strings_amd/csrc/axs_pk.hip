@@ -20,13 +20,15 @@
 
 #include <math.h>
 
+#include <algorithm>
+
 #include "axs_internal.hpp"
 #include "axs_lattice.hpp"
 
 namespace axs {
 
 struct FftPlan {
-    int N = 0, nd = 0;
+    int N = 0, nd = 0, batch = 1;
     rocfft_plan plan = nullptr;
     rocfft_execution_info info = nullptr;
     void *work = nullptr;
@@ -45,10 +47,12 @@ struct FftPlan {
 
 static int rocfft_ready = 0;
 
-static int get_plan(Context *c, int N, FftPlan **out)
+// batched real-to-complex plan for `batch` lattices of side N, with its
+// input / output / work buffers; kept for the context's lifetime
+static int get_plan(Context *c, int N, int batch, FftPlan **out)
 {
     for (FftPlan *p : c->fft_plans)
-        if (p->N == N && p->nd == c->nd) {
+        if (p->N == N && p->nd == c->nd && p->batch == batch) {
             *out = p;
             return AXS_OK;
         }
@@ -62,12 +66,13 @@ static int get_plan(Context *c, int N, FftPlan **out)
     FftPlan *p = new FftPlan();
     p->N = N;
     p->nd = c->nd;
+    p->batch = batch;
     size_t lengths[3] = {(size_t)N, (size_t)N, (size_t)N};  // fastest first
-    size_t L = (c->nd == 3) ? (size_t)N * N * N : (size_t)N * N;
+    size_t L = ((c->nd == 3) ? (size_t)N * N * N : (size_t)N * N) * (size_t)batch;
     size_t half = L / N * (N / 2 + 1);
     if (rocfft_plan_create(&p->plan, rocfft_placement_notinplace,
                            rocfft_transform_type_real_forward, rocfft_precision_double,
-                           (size_t)c->nd, lengths, 1, nullptr) != rocfft_status_success) {
+                           (size_t)c->nd, lengths, (size_t)batch, nullptr) != rocfft_status_success) {
         delete p;
         set_error("rocfft_plan_create failed for N=%d ndims=%d", N, c->nd);
         return AXS_ERR_FFT;
@@ -98,15 +103,18 @@ static int get_plan(Context *c, int N, FftPlan **out)
     return AXS_OK;
 }
 
-// phi2 of the current state as a contiguous double array
+// phi2 of the current states of a batch of lattices (one per blockIdx.y,
+// field pointers in src) as contiguous double arrays, lattice b at y + b L
 template <class P>
-__global__ void __launch_bounds__(256) phi2_kernel(uint32_t L, P phi, double *__restrict__ y)
+__global__ void __launch_bounds__(256)
+phi2_kernel(uint32_t L, const void *const *__restrict__ src, double *__restrict__ y)
 {
-    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m < L) {
+        const P phi{(decltype(P::p))src[blockIdx.y]};
         double a, b;
         phi.get(m, a, b);
-        y[m] = b;
+        y[(size_t)blockIdx.y * L + m] = b;
     }
 }
 
@@ -158,6 +166,9 @@ bin_kernel(int N, const double2 *__restrict__ F, int nbins, const uint32_t *__re
     __shared__ KahanP sh_pk[kPkRows / 64], sh_ks[kPkRows / 64];
     __shared__ long long sh_n[kPkRows / 64];
     const uint32_t item = blockIdx.x;
+    const int H0 = N / 2 + 1;
+    F += (size_t)blockIdx.y * (size_t)((ND == 3) ? (int64_t)N * N * H0 : (int64_t)N * H0);
+    const size_t pslot = (size_t)blockIdx.y * gridDim.x + item;
     int lo_b = 0, hi_b = nbins - 1;              // last bin starting at or before item
     while (lo_b < hi_b) {
         int mid = (lo_b + hi_b + 1) >> 1;
@@ -228,21 +239,28 @@ bin_kernel(int N, const double2 *__restrict__ F, int nbins, const uint32_t *__re
             k = kmergep(k, sh_ks[w]);
             n += sh_n[w];
         }
-        part_pk[item] = a;
-        part_ks[item] = k;
-        part_cnt[item] = n;
+        part_pk[pslot] = a;
+        part_ks[pslot] = k;
+        part_cnt[pslot] = n;
     }
 }
 
-// bin b: its items' partials merged in item order
+// bin b of lattice blockIdx.y: its items' partials merged in item order
 __global__ void __launch_bounds__(64)
-bin_fold_kernel(int nbins, const uint32_t *__restrict__ item_begin,
+bin_fold_kernel(int nbins, uint32_t nitems, const uint32_t *__restrict__ item_begin,
                 const KahanP *__restrict__ part_pk, const KahanP *__restrict__ part_ks,
                 const long long *__restrict__ part_cnt, double *__restrict__ out_pk,
                 double *__restrict__ out_ks, long long *__restrict__ out_cnt)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nbins) return;
+    const size_t pofs = (size_t)blockIdx.y * nitems, oofs = (size_t)blockIdx.y * nbins;
+    part_pk += pofs;
+    part_ks += pofs;
+    part_cnt += pofs;
+    out_pk += oofs;
+    out_ks += oofs;
+    out_cnt += oofs;
     KahanP a{0.0, 0.0}, k{0.0, 0.0};
     long long n = 0;
     for (uint32_t t = item_begin[b]; t < item_begin[b + 1]; ++t) {
@@ -255,43 +273,26 @@ bin_fold_kernel(int nbins, const uint32_t *__restrict__ item_begin,
     out_cnt[b] = n;
 }
 
-int run_powerspec(Context *c, Member &m, int binning, int nbins, double *ks, double *pk,
-                  int32_t *count)
+// bins of a lattice of side N as ranges [lo, hi) of the integer |k|^2, and
+// the work items (bounding-box rows in chunks of kPkRows) of each bin
+struct BinLayout {
+    int nbins = 0;
+    std::vector<int64_t> lo, hi;
+    std::vector<uint32_t> items;      // first item of each bin, items[nbins] = total
+};
+
+static int bin_layout(int N, int nd, int binning, int nbins, BinLayout &B)
 {
-    const int N = m.N;
     if (binning == AXS_BIN_LINEAR)
         nbins = N / 2;
     else if (binning != AXS_BIN_LOG || nbins < 1 || nbins > 4096) {
         set_error("powerspec: bad binning %d / nbins %d", binning, nbins);
         return AXS_ERR_INVALID_ARG;
     }
-    FftPlan *plan = nullptr;
-    AXS_TRY(get_plan(c, N, &plan));
-
-    FieldView fv;
-    AXS_TRY(current_fields(c, m, &fv));
-    uint32_t L = (uint32_t)m.L;
-    unsigned g = (L + 255) / 256;
-    if (c->prec == AXS_FP64)
-        hipLaunchKernelGGL(phi2_kernel<PhiF64>, dim3(g), dim3(256), 0, c->stream, L,
-                           PhiF64{(const dvec2 *)fv.phi}, plan->in);
-    else
-        hipLaunchKernelGGL(phi2_kernel<PhiF32>, dim3(g), dim3(256), 0, c->stream, L,
-                           PhiF32{(const fvec4 *)fv.phi}, plan->in);
-    AXS_HIP_TRY(hipGetLastError());
-    if (rocfft_execution_info_set_stream(plan->info, c->stream) != rocfft_status_success) {
-        set_error("rocfft_execution_info_set_stream failed");
-        return AXS_ERR_FFT;
-    }
-    void *ib[1] = {plan->in};
-    void *ob[1] = {plan->out};
-    if (rocfft_execute(plan->plan, ib, ob, plan->info) != rocfft_status_success) {
-        set_error("rocfft_execute failed");
-        return AXS_ERR_FFT;
-    }
-
-    // bin ranges in |k|^2 (integers)
-    std::vector<int64_t> lo(nbins), hi(nbins);
+    B.nbins = nbins;
+    B.lo.assign(nbins, 0);
+    B.hi.assign(nbins, 0);
+    std::vector<int64_t> &lo = B.lo, &hi = B.hi;
     int64_t top = (int64_t)(N / 2) * (N / 2);      // |k| < N/2  (pk_estimation.cpp:93)
     if (binning == AXS_BIN_LINEAR) {
         // floor(|k|) - 1 == b  <=>  (b+1)^2 <= |k|^2 < (b+2)^2    (:95)
@@ -314,9 +315,7 @@ int run_powerspec(Context *c, Member &m, int binning, int nbins, double *ks, dou
             if (lo[b] > hi[b]) lo[b] = hi[b];
         }
     }
-    // work items: each bin's bounding-box rows in chunks of kPkRows
-    size_t nb = (size_t)nbins;
-    std::vector<uint32_t> items(nb + 1, 0);
+    B.items.assign(nbins + 1, 0);
     for (int b = 0; b < nbins; ++b) {
         int64_t rows = 0;
         if (hi[b] > lo[b]) {
@@ -325,19 +324,46 @@ int run_powerspec(Context *c, Member &m, int binning, int nbins, double *ks, dou
             while (R > 0 && (R - 1) * (R - 1) >= hi[b]) --R;
             if (R > N / 2) R = N / 2;
             int64_t side = 2 * R + 1;
-            rows = (c->nd == 3) ? side * side : side;
+            rows = (nd == 3) ? side * side : side;
         }
-        items[b + 1] = items[b] + (uint32_t)((rows + kPkRows - 1) / kPkRows);
+        B.items[b + 1] = B.items[b] + (uint32_t)((rows + kPkRows - 1) / kPkRows);
     }
-    const uint32_t nitems = items[nb];
+    return AXS_OK;
+}
+
+// Spectra of the n members idx[0..n), all of side N, in one batched
+// transform: member idx[q]'s bins go to ks / pk / count + q nbins.
+static int pk_batch(Context *c, const int *idx, int n, int binning, int nbins_in, double *ks,
+                    double *pk, int32_t *count)
+{
+    const int N = c->members[idx[0]].N;
+    BinLayout B;
+    AXS_TRY(bin_layout(N, c->nd, binning, nbins_in, B));
+    const int nbins = B.nbins;
+    for (int q = 0; q < n; ++q)
+        if (!c->members[idx[q]].has_state) {
+            set_error("member has no state: call axs_set_member_state first");
+            return AXS_ERR_STATE;
+        }
+    FftPlan *plan = nullptr;
+    AXS_TRY(get_plan(c, N, n, &plan));
+    const uint32_t L = (uint32_t)c->members[idx[0]].L;
+
+    const size_t nb = (size_t)nbins;
+    const uint32_t nitems = B.items[nb];
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t o_lo = 0, o_hi = o_lo + align(nb * 8), o_ib = o_hi + align(nb * 8);
-    const size_t o_ppk = o_ib + align((nb + 1) * 4), o_pks = o_ppk + align((size_t)nitems * 16);
-    const size_t o_pcnt = o_pks + align((size_t)nitems * 16);
-    const size_t o_pk = o_pcnt + align((size_t)nitems * 8), o_ks = o_pk + align(nb * 8);
-    const size_t o_cnt = o_ks + align(nb * 8), need = o_cnt + align(nb * 8);
+    const size_t o_src = 0, o_lo = align((size_t)n * 8), o_hi = o_lo + align(nb * 8);
+    const size_t o_ib = o_hi + align(nb * 8);
+    const size_t o_ppk = o_ib + align((nb + 1) * 4);
+    const size_t o_pks = o_ppk + align((size_t)n * nitems * 16);
+    const size_t o_pcnt = o_pks + align((size_t)n * nitems * 16);
+    const size_t o_pk = o_pcnt + align((size_t)n * nitems * 8);
+    const size_t o_ks = o_pk + align((size_t)n * nb * 8);
+    const size_t o_cnt = o_ks + align((size_t)n * nb * 8);
+    const size_t need = o_cnt + align((size_t)n * nb * 8);
     AXS_TRY(ensure_scratch(c, need));
     char *sp = (char *)c->scratch;
+    const void **dsrc = (const void **)(sp + o_src);
     int64_t *dlo = (int64_t *)(sp + o_lo);
     int64_t *dhi = (int64_t *)(sp + o_hi);
     uint32_t *dib = (uint32_t *)(sp + o_ib);
@@ -347,45 +373,119 @@ int run_powerspec(Context *c, Member &m, int binning, int nbins, double *ks, dou
     double *dpk = (double *)(sp + o_pk);
     double *dks = (double *)(sp + o_ks);
     long long *dcnt = (long long *)(sp + o_cnt);
-    AXS_HIP_TRY(hipMemcpyAsync(dlo, lo.data(), nb * 8, hipMemcpyHostToDevice, c->stream));
-    AXS_HIP_TRY(hipMemcpyAsync(dhi, hi.data(), nb * 8, hipMemcpyHostToDevice, c->stream));
-    AXS_HIP_TRY(hipMemcpyAsync(dib, items.data(), (nb + 1) * 4, hipMemcpyHostToDevice,
+
+    // pack phi2 of every member (phi only: no FP32 velocity reconstruction)
+    std::vector<const void *> src(n);
+    for (int q = 0; q < n; ++q) src[q] = c->members[idx[q]].state[c->cur][PHI];
+    AXS_HIP_TRY(hipMemcpyAsync(dsrc, src.data(), (size_t)n * 8, hipMemcpyHostToDevice,
+                               c->stream));
+    const dim3 gp((L + 255) / 256, (unsigned)n);
+    if (c->prec == AXS_FP64)
+        hipLaunchKernelGGL(phi2_kernel<PhiF64>, gp, dim3(256), 0, c->stream, L, dsrc, plan->in);
+    else
+        hipLaunchKernelGGL(phi2_kernel<PhiF32>, gp, dim3(256), 0, c->stream, L, dsrc, plan->in);
+    AXS_HIP_TRY(hipGetLastError());
+    if (rocfft_execution_info_set_stream(plan->info, c->stream) != rocfft_status_success) {
+        set_error("rocfft_execution_info_set_stream failed");
+        return AXS_ERR_FFT;
+    }
+    void *ib[1] = {plan->in};
+    void *ob[1] = {plan->out};
+    if (rocfft_execute(plan->plan, ib, ob, plan->info) != rocfft_status_success) {
+        set_error("rocfft_execute failed");
+        return AXS_ERR_FFT;
+    }
+
+    AXS_HIP_TRY(hipMemcpyAsync(dlo, B.lo.data(), nb * 8, hipMemcpyHostToDevice, c->stream));
+    AXS_HIP_TRY(hipMemcpyAsync(dhi, B.hi.data(), nb * 8, hipMemcpyHostToDevice, c->stream));
+    AXS_HIP_TRY(hipMemcpyAsync(dib, B.items.data(), (nb + 1) * 4, hipMemcpyHostToDevice,
                                c->stream));
     if (nitems > 0) {
         auto k = (c->nd == 3) ? bin_kernel<3> : bin_kernel<2>;
-        hipLaunchKernelGGL(k, dim3(nitems), dim3(kPkRows), 0, c->stream, N,
+        hipLaunchKernelGGL(k, dim3(nitems, (unsigned)n), dim3(kPkRows), 0, c->stream, N,
                            (const double2 *)plan->out, nbins, dib, dlo, dhi, ppk, pks, pcnt);
     }
-    hipLaunchKernelGGL(bin_fold_kernel, dim3((nbins + 63) / 64), dim3(64), 0, c->stream, nbins,
-                       dib, ppk, pks, pcnt, dpk, dks, dcnt);
+    hipLaunchKernelGGL(bin_fold_kernel, dim3((nbins + 63) / 64, (unsigned)n), dim3(64), 0,
+                       c->stream, nbins, nitems, dib, ppk, pks, pcnt, dpk, dks, dcnt);
     AXS_HIP_TRY(hipGetLastError());
-    std::vector<double> hpk(nb), hks(nb);
-    std::vector<long long> hcnt(nb);
-    AXS_HIP_TRY(hipMemcpyAsync(hpk.data(), dpk, nb * 8, hipMemcpyDeviceToHost, c->stream));
-    AXS_HIP_TRY(hipMemcpyAsync(hks.data(), dks, nb * 8, hipMemcpyDeviceToHost, c->stream));
-    AXS_HIP_TRY(hipMemcpyAsync(hcnt.data(), dcnt, nb * 8, hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> hpk((size_t)n * nb), hks((size_t)n * nb);
+    std::vector<long long> hcnt((size_t)n * nb);
+    AXS_HIP_TRY(hipMemcpyAsync(hpk.data(), dpk, hpk.size() * 8, hipMemcpyDeviceToHost,
+                               c->stream));
+    AXS_HIP_TRY(hipMemcpyAsync(hks.data(), dks, hks.size() * 8, hipMemcpyDeviceToHost,
+                               c->stream));
+    AXS_HIP_TRY(hipMemcpyAsync(hcnt.data(), dcnt, hcnt.size() * 8, hipMemcpyDeviceToHost,
+                               c->stream));
     AXS_HIP_TRY(hipStreamSynchronize(c->stream));
 
     float Lbox = N * c->p.space_step;                           // pk_estimation.cpp:41
-    for (int b = 0; b < nbins; ++b) {
-        count[b] = (int32_t)hcnt[b];
-        if (hcnt[b] == 0) {
-            ks[b] = 0.0;
-            pk[b] = 0.0;
+    for (size_t e = 0; e < (size_t)n * nb; ++e) {
+        count[e] = (int32_t)hcnt[e];
+        if (hcnt[e] == 0) {
+            ks[e] = 0.0;
+            pk[e] = 0.0;
             continue;
         }
-        double kbar = hks[b] / (double)hcnt[b];                 // :118 / :196
-        double p = hpk[b];
+        double kbar = hks[e] / (double)hcnt[e];                 // :118 / :196
+        double p = hpk[e];
         if (binning == AXS_BIN_LINEAR) {
             if (c->nd == 2)
                 p /= 2.0f * M_PI * kbar;                        // :119
             else
                 p /= 4.0f * M_PI * (kbar * kbar);               // :197
         } else {
-            p /= (double)hcnt[b];
+            p /= (double)hcnt[e];
         }
-        ks[b] = kbar * (2.0f * M_PI / Lbox);                    // :120 / :198
-        pk[b] = p;
+        ks[e] = kbar * (2.0f * M_PI / Lbox);                    // :120 / :198
+        pk[e] = p;
+    }
+    return AXS_OK;
+}
+
+int run_powerspec(Context *c, Member &m, int binning, int nbins, double *ks, double *pk,
+                  int32_t *count)
+{
+    const int member = (int)(&m - c->members.data());
+    return pk_batch(c, &member, 1, binning, nbins, ks, pk, count);
+}
+
+// Every member's spectrum: members of one side in batched transforms of at
+// most kPkBatchBytes of input each, in member order within a side; member
+// r's bins at offset sum_{q<r} nbins_q.
+constexpr size_t kPkBatchBytes = (size_t)4 << 30;
+
+int run_powerspec_all(Context *c, int binning, int nbins, double *ks, double *pk, int32_t *count)
+{
+    const int nm = (int)c->members.size();
+    std::vector<size_t> first(nm + 1, 0);
+    for (int r = 0; r < nm; ++r) {
+        const int nb = (binning == AXS_BIN_LINEAR) ? c->members[r].N / 2 : nbins;
+        first[r + 1] = first[r] + (size_t)(nb > 0 ? nb : 0);
+    }
+    std::vector<char> done(nm, 0);
+    for (int r = 0; r < nm; ++r) {
+        if (done[r]) continue;
+        const int N = c->members[r].N;
+        const size_t Lb = (size_t)c->members[r].L * 8;
+        const int cap = (int)std::max<size_t>(1, kPkBatchBytes / Lb);
+        std::vector<int> group;
+        for (int q = r; q < nm; ++q)
+            if (!done[q] && c->members[q].N == N) group.push_back(q);
+        for (size_t g0 = 0; g0 < group.size(); g0 += (size_t)cap) {
+            const int n = (int)std::min<size_t>((size_t)cap, group.size() - g0);
+            const int nb = (binning == AXS_BIN_LINEAR) ? N / 2 : nbins;
+            std::vector<double> bks((size_t)n * nb), bpk((size_t)n * nb);
+            std::vector<int32_t> bcnt((size_t)n * nb);
+            AXS_TRY(pk_batch(c, group.data() + g0, n, binning, nbins, bks.data(), bpk.data(),
+                             bcnt.data()));
+            for (int q = 0; q < n; ++q) {
+                const size_t dst = first[group[g0 + q]], srco = (size_t)q * nb;
+                std::copy(bks.begin() + srco, bks.begin() + srco + nb, ks + dst);
+                std::copy(bpk.begin() + srco, bpk.begin() + srco + nb, pk + dst);
+                std::copy(bcnt.begin() + srco, bcnt.begin() + srco + nb, count + dst);
+            }
+        }
+        for (int q : group) done[q] = 1;
     }
     return AXS_OK;
 }

@@ -431,6 +431,54 @@ adjacent_pairs_kernel(const int32_t *__restrict__ coords, uint64_t n,
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(out, (unsigned long long)c);
 }
 
+// Flagged sites of one member from its ballot words, and in 2D the
+// list-adjacent pairs of s_run.cpp:101-109 without building the list: in
+// ascending site order (m = i + N j) the entry after a flagged site m is the
+// next flagged site, and the pair counts when that is m + 1 in the same row
+// (di = 1, dj = 0) or m + N (di = 0, dj = 1).  Adds to cnt[0] and cnt[1].
+template <int ND>
+__global__ void __launch_bounds__(256)
+member_count_kernel(const uint64_t *__restrict__ words, uint32_t L, int N,
+                    unsigned long long *__restrict__ cnt)
+{
+    const uint64_t nwords = ((uint64_t)L + 63) / 64;
+    const uint64_t w = blockIdx.x * 256ull + threadIdx.x;
+    unsigned long long n = 0, pairs = 0;
+    if (w < nwords) {
+        uint64_t bits = words[w];
+        n = (unsigned long long)__popcll(bits);
+        if (ND == 2) {
+            while (bits) {
+                const uint64_t m = w * 64 + (uint64_t)(__ffsll((unsigned long long)bits) - 1);
+                bits &= bits - 1;
+                uint64_t next = ~0ull;                   // next flagged site after m
+                if (bits) {
+                    next = w * 64 + (uint64_t)(__ffsll((unsigned long long)bits) - 1);
+                } else {
+                    for (uint64_t v = w + 1; v < nwords && v * 64 <= m + (uint64_t)N; ++v) {
+                        const uint64_t x = words[v];
+                        if (x) {
+                            next = v * 64 + (uint64_t)(__ffsll((unsigned long long)x) - 1);
+                            break;
+                        }
+                    }
+                }
+                if ((next == m + 1 && m % (uint64_t)N != (uint64_t)(N - 1)) ||
+                    next == m + (uint64_t)N)
+                    ++pairs;
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        n += __shfl_down(n, o, 64);
+        pairs += __shfl_down(pairs, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (n) atomicAdd(&cnt[0], n);
+        if (pairs) atomicAdd(&cnt[1], pairs);
+    }
+}
+
 static double accept_threshold(int thr)
 {
     float f = 0.5f * (1.0f - (float)thr / 100.0f);   // string_finding.cpp:8, float
@@ -477,6 +525,51 @@ static int launch_strings_march(Context *c, int N, uint32_t L, const void *phi, 
     return AXS_OK;
 }
 
+// Layout of the finder's device buffer for lattices of up to L sites:
+// ballot words, block counts, block offsets (+ total), a 256-byte slot for
+// the 2D pair count, the scan's tile sums, then two counters per member for
+// the ensemble-wide count.
+struct StrLayout {
+    uint32_t nblocks, ntiles;
+    size_t words_b, cnt_b, off_b, tail_b, counters_b;
+    StrLayout(uint32_t L, size_t nmembers)
+    {
+        auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+        nblocks = (L + kStrSpan - 1) / kStrSpan;
+        ntiles = (nblocks + kScanTile - 1) / kScanTile;
+        words_b = ((size_t)nblocks * kStrSpan / 64) * 8;
+        cnt_b = align((size_t)nblocks * 4);
+        off_b = align((size_t)(nblocks + 1) * 8);
+        tail_b = 256 + align((size_t)ntiles * 8);
+        counters_b = align(nmembers * 16);
+    }
+    size_t bytes() const { return words_b + cnt_b + off_b + tail_b + counters_b; }
+};
+
+// ballot words and block counts of member m (the search itself)
+static int search_member(Context *c, Member &m, uint64_t *words, uint32_t *bcount)
+{
+    const uint32_t L = (uint32_t)m.L;
+    const uint32_t nblocks = (L + kStrSpan - 1) / kStrSpan;
+    // the angles need phi only (no FP32 velocity reconstruction)
+    const void *phi = m.state[c->cur][PHI];
+    const double accept = accept_threshold(c->p.thr);
+    if (c->nd == 3) return launch_strings_march(c, m.N, L, phi, accept, words, bcount);
+    AXS_TRY(ensure_scratch(c, (size_t)L * sizeof(double)));
+    double *ang = (double *)c->scratch;
+    const unsigned ga = (L + 255) / 256;
+    if (c->prec == AXS_FP64)
+        hipLaunchKernelGGL(angle_kernel<PhiF64>, dim3(ga), dim3(256), 0, c->stream, L,
+                           PhiF64{(const dvec2 *)phi}, ang);
+    else
+        hipLaunchKernelGGL(angle_kernel<PhiF32>, dim3(ga), dim3(256), 0, c->stream, L,
+                           PhiF32{(const fvec4 *)phi}, ang);
+    hipLaunchKernelGGL(strings_flag_kernel<2>, dim3(nblocks), dim3(kStrBlock), 0, c->stream,
+                       m.N, L, ang, accept, words, bcount);
+    AXS_HIP_TRY(hipGetLastError());
+    return AXS_OK;
+}
+
 int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *count,
                 int64_t *num_cores)
 {
@@ -485,48 +578,27 @@ int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *co
         return AXS_ERR_STATE;
     }
     const int member = (int)(&m - c->members.data());
-    uint32_t L = (uint32_t)m.L;
-    uint32_t nblocks = (L + kStrSpan - 1) / kStrSpan;
-    size_t words_b = ((size_t)nblocks * kStrSpan / 64) * 8;
-    size_t cnt_b = ((size_t)nblocks * 4 + 255) & ~(size_t)255;
-    size_t off_b = ((size_t)(nblocks + 1) * 8 + 255) & ~(size_t)255;
-    const uint32_t ntiles = (nblocks + kScanTile - 1) / kScanTile;
-    size_t tail_b = 256 + (((size_t)ntiles * 8 + 255) & ~(size_t)255);
+    const uint32_t L = (uint32_t)m.L;
+    const StrLayout lay(L, c->members.size());
+    const uint32_t nblocks = lay.nblocks, ntiles = lay.ntiles;
     const bool cached = c->str_member == member && c->str_version == m.version && c->strbuf &&
-                        c->strbuf_bytes >= words_b + cnt_b + off_b + tail_b;
+                        c->strbuf_bytes >= lay.bytes();
     if (!cached) {
         c->str_member = -1;
-        AXS_TRY(ensure_device(c, &c->strbuf, &c->strbuf_bytes, words_b + cnt_b + off_b + tail_b));
+        AXS_TRY(ensure_device(c, &c->strbuf, &c->strbuf_bytes, lay.bytes()));
     }
     char *sp = (char *)c->strbuf;
     uint64_t *words = (uint64_t *)sp;
-    uint32_t *bcount = (uint32_t *)(sp + words_b);
-    uint64_t *off = (uint64_t *)(sp + words_b + cnt_b);
-    unsigned long long *pairs = (unsigned long long *)(sp + words_b + cnt_b + off_b);
-    uint64_t *tsum = (uint64_t *)(sp + words_b + cnt_b + off_b + 256);
+    uint32_t *bcount = (uint32_t *)(sp + lay.words_b);
+    uint64_t *off = (uint64_t *)(sp + lay.words_b + lay.cnt_b);
+    unsigned long long *pairs = (unsigned long long *)(sp + lay.words_b + lay.cnt_b + lay.off_b);
+    uint64_t *tsum = (uint64_t *)(sp + lay.words_b + lay.cnt_b + lay.off_b + 256);
 
     uint64_t total = 0;
     if (cached) {
         total = c->str_total;
     } else {
-        // the angles need phi only (no FP32 velocity reconstruction)
-        const void *phi = m.state[c->cur][PHI];
-        const double accept = accept_threshold(c->p.thr);
-        if (c->nd == 3) {
-            AXS_TRY(launch_strings_march(c, m.N, L, phi, accept, words, bcount));
-        } else {
-            AXS_TRY(ensure_scratch(c, (size_t)L * sizeof(double)));
-            double *ang = (double *)c->scratch;
-            const unsigned ga = (L + 255) / 256;
-            if (c->prec == AXS_FP64)
-                hipLaunchKernelGGL(angle_kernel<PhiF64>, dim3(ga), dim3(256), 0, c->stream, L,
-                                   PhiF64{(const dvec2 *)phi}, ang);
-            else
-                hipLaunchKernelGGL(angle_kernel<PhiF32>, dim3(ga), dim3(256), 0, c->stream, L,
-                                   PhiF32{(const fvec4 *)phi}, ang);
-            hipLaunchKernelGGL(strings_flag_kernel<2>, dim3(nblocks), dim3(kStrBlock), 0,
-                               c->stream, m.N, L, ang, accept, words, bcount);
-        }
+        AXS_TRY(search_member(c, m, words, bcount));
         hipLaunchKernelGGL(tile_scan_kernel, dim3(ntiles), dim3(1024), 0, c->stream, bcount, off,
                            tsum, nblocks);
         hipLaunchKernelGGL(top_scan_kernel, dim3(1), dim3(1024), 0, c->stream, tsum, ntiles,
@@ -584,6 +656,50 @@ int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *co
         AXS_HIP_TRY(hipStreamSynchronize(c->stream));
     }
     return (coords == nullptr || cap >= (int64_t)total) ? AXS_OK : AXS_ERR_CAPACITY;
+}
+
+// axs_string_counts: every member searched in turn on the stream, counted
+// from its ballot words into its own counters, one copy and one
+// synchronisation for the ensemble (the shared buffers are sized for the
+// largest member first, so nothing is re-allocated between the launches).
+int run_string_counts(Context *c, int64_t *n_plaquettes, int64_t *num_cores)
+{
+    uint32_t Lmax = 0;
+    for (const Member &m : c->members) {
+        if (!m.has_state) {
+            set_error("member has no state: call axs_set_member_state first");
+            return AXS_ERR_STATE;
+        }
+        Lmax = (uint32_t)m.L > Lmax ? (uint32_t)m.L : Lmax;
+    }
+    const size_t nm = c->members.size();
+    const StrLayout lay(Lmax, nm);
+    c->str_member = -1;
+    AXS_TRY(ensure_device(c, &c->strbuf, &c->strbuf_bytes, lay.bytes()));
+    if (c->nd == 2) AXS_TRY(ensure_scratch(c, (size_t)Lmax * sizeof(double)));
+    char *sp = (char *)c->strbuf;
+    uint64_t *words = (uint64_t *)sp;
+    uint32_t *bcount = (uint32_t *)(sp + lay.words_b);
+    unsigned long long *counters =
+        (unsigned long long *)(sp + lay.words_b + lay.cnt_b + lay.off_b + lay.tail_b);
+    AXS_HIP_TRY(hipMemsetAsync(counters, 0, nm * 16, c->stream));
+    for (size_t r = 0; r < nm; ++r) {
+        Member &m = c->members[r];
+        AXS_TRY(search_member(c, m, words, bcount));
+        const uint32_t L = (uint32_t)m.L;
+        const unsigned g = (unsigned)(((L + 63) / 64 + 255) / 256);
+        auto k = (c->nd == 2) ? member_count_kernel<2> : member_count_kernel<3>;
+        hipLaunchKernelGGL(k, dim3(g), dim3(256), 0, c->stream, words, L, m.N, counters + 2 * r);
+        AXS_HIP_TRY(hipGetLastError());
+    }
+    std::vector<unsigned long long> h(2 * nm);
+    AXS_HIP_TRY(hipMemcpyAsync(h.data(), counters, nm * 16, hipMemcpyDeviceToHost, c->stream));
+    AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+    for (size_t r = 0; r < nm; ++r) {
+        if (n_plaquettes) n_plaquettes[r] = (int64_t)h[2 * r];
+        if (num_cores) num_cores[r] = (int64_t)(h[2 * r] - h[2 * r + 1]);
+    }
+    return AXS_OK;
 }
 
 }  // namespace axs

@@ -183,6 +183,15 @@ int axs_member_powerspec(axs_ctx *ctx, int member, int binning, int nbins,
 int axs_powerspec(axs_ctx *ctx, int binning, int nbins,
                   double *ks, double *pk, int32_t *count);
 
+/* The spectra of every member in one call (ensembles have no reference
+ * counterpart; each member's spectrum is the axs_member_powerspec one).
+ * Members of one side share batched rocFFT transforms and one binning
+ * launch, with one synchronisation per batch.  Member r's bins start at
+ * entry sum_{q<r} nb_q of ks, pk and count, nb_q = N_q/2 (AXS_BIN_LINEAR)
+ * or nbins (AXS_BIN_LOG).                                                  */
+int axs_powerspec_all(axs_ctx *ctx, int binning, int nbins,
+                      double *ks, double *pk, int32_t *count);
+
 /* ---- kernel configuration ---------------------------------------------- */
 
 #define AXS_KERNEL_AUTO   0   /* march kernel where it applies, else simple   */

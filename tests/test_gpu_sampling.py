@@ -129,6 +129,62 @@ def test_strings_3d_straight_lines(axis, N):
     np.testing.assert_array_equal(c, c_ref)
 
 
+@pytest.mark.parametrize("nd,Ns", [(2, [64, 66, 130, 256, 64]), (2, [8]), (3, [64, 34, 96])])
+def test_string_counts_ensemble(nd, Ns):
+    """axs_string_counts over an ensemble of uneven sides, dense (random
+    phases) and smooth members: flagged sites and, in 2D, the core count
+    n - (list-adjacent pairs) taken from the ballot words, against the
+    oracle's list of each member."""
+    lat = axs.Lattice(nd, members=Ns)
+    fields = []
+    for r, N in enumerate(Ns):
+        if r % 2 == 0:
+            rng = np.random.default_rng(100 + r)
+            p1, p2 = rng.standard_normal(N ** nd), rng.standard_normal(N ** nd)
+        else:
+            p1, p2 = kinv_field(nd, N, seed=200 + r)
+        z = np.zeros(N ** nd)
+        lat.set_state(p1, p2, z, z, member=r)
+        fields.append((p1, p2))
+    npl, ncores = lat.string_counts()
+    for r, (N, (p1, p2)) in enumerate(zip(Ns, fields)):
+        n_ref, c_ref = O.cores(O.axion(p1, p2), nd, N, thr=1)
+        assert npl[r] == n_ref, (r, N)
+        assert ncores[r] == (O.num_cores2(c_ref) if nd == 2 else n_ref), (r, N)
+    assert npl.sum() > 0
+
+
+@pytest.mark.parametrize("nd,Ns,binning,prec", [(2, [64, 128, 64, 96, 128], "log", "fp64"),
+                                                (2, [64, 128, 64], "linear", "fp32"),
+                                                (3, [32, 48, 32], "log", "fp32"),
+                                                (3, [32, 48, 32, 32], "linear", "fp64")])
+def test_powerspec_all(nd, Ns, binning, prec):
+    """Every member's spectrum from one call (batched transforms per side)
+    against the oracle on the member's own state, and against the
+    one-member call."""
+    lat = axs.Lattice(nd, members=Ns, precision=prec)
+    for r, N in enumerate(Ns):
+        p1, p2 = kinv_field(nd, N, seed=300 + r)
+        z = np.zeros(N ** nd)
+        lat.set_state(p1, p2, z, z, member=r)
+    out = lat.powerspec_all(binning=binning, nbins=20)
+    assert len(out) == len(Ns)
+    for r, N in enumerate(Ns):
+        ks, pk, cnt = out[r]
+        phi2 = lat.get_state(member=r)["phi2"]
+        if binning == "log":
+            ks_r, pk_r, cnt_r = O.powerspec_log(phi2, nd, N, 0.5, 20)
+        else:
+            ks_r, pk_r, cnt_r = O.powerspec(phi2, nd, N, 0.5)
+        np.testing.assert_array_equal(cnt, cnt_r)
+        sel = cnt > 0
+        np.testing.assert_allclose(ks[sel], ks_r[sel], rtol=1e-13)
+        np.testing.assert_allclose(pk[sel], pk_r[sel], rtol=1e-9, atol=1e-12 * pk_r[sel].max())
+        ks1, pk1, cnt1 = lat.powerspec(member=r, binning=binning, nbins=20)
+        np.testing.assert_array_equal(cnt, cnt1)
+        np.testing.assert_allclose(pk, pk1, rtol=1e-12, atol=1e-14 * pk1.max())
+
+
 def test_strings_hand_built_vortices():
     N = 64
     p1, p2 = vortex2d(N, [(10.5, 12.5), (40.5, 30.5), (20.5, 50.5)], [1, -1, 1])
