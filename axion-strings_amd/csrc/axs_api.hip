@@ -850,7 +850,8 @@ int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
         // several steps per pass, 3D, on every member
         const int R = cfg->rows ? cfg->rows
                                 : (RAD > 1 ? high_order_rows(c->prec, RAD)
-                                           : (W == 4 ? 4 : (W == 8 ? 3 : 2)));
+                                           : (W == 4 ? 4
+                                                     : ((W == 8 && c->prec == AXS_FP32) ? 3 : 2)));
         rows_ok = wg != (AXS_MARCH_WG512 | AXS_MARCH_WG1024) &&
                   multi_shape_built(depth, W, R, c->prec, RAD);
         bool fits = cfg->kernel != AXS_KERNEL_SIMPLE && depth <= (RAD > 1 ? 2 : 4);
