@@ -14,8 +14,9 @@
 // the bin are one contiguous run of fz.  The rows of each bin's bounding
 // box are cut into chunks of kPkRows; one workgroup sums one (bin, chunk)
 // item in compensated FP64 (a work-item per row, then a fixed-order tree),
-// and a fold kernel merges each bin's items in chunk order -- balanced over
-// the bins (the outer log bins hold most modes), deterministic, no atomics.
+// and a fold kernel merges each bin's items (a work-group per bin: strided
+// runs in chunk order, then a fixed pairwise tree) -- balanced over the
+// bins (the outer log bins hold most modes), deterministic, no atomics.
 #include <rocfft/rocfft.h>
 
 #include <math.h>
@@ -245,32 +246,43 @@ bin_kernel(int N, const double2 *__restrict__ F, int nbins, const uint32_t *__re
     }
 }
 
-// bin b of lattice blockIdx.y: its items' partials merged in item order
-__global__ void __launch_bounds__(64)
+// bin blockIdx.x of lattice blockIdx.y: work-item t merges the bin's items
+// t, t + 256, ... in order, then a fixed pairwise tree over the work-items
+// (deterministic for a given item count)
+__global__ void __launch_bounds__(256)
 bin_fold_kernel(int nbins, uint32_t nitems, const uint32_t *__restrict__ item_begin,
                 const KahanP *__restrict__ part_pk, const KahanP *__restrict__ part_ks,
                 const long long *__restrict__ part_cnt, double *__restrict__ out_pk,
                 double *__restrict__ out_ks, long long *__restrict__ out_cnt)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nbins) return;
+    __shared__ KahanP sa[256], sk[256];
+    __shared__ long long sn[256];
+    const int b = blockIdx.x;
     const size_t pofs = (size_t)blockIdx.y * nitems, oofs = (size_t)blockIdx.y * nbins;
-    part_pk += pofs;
-    part_ks += pofs;
-    part_cnt += pofs;
-    out_pk += oofs;
-    out_ks += oofs;
-    out_cnt += oofs;
     KahanP a{0.0, 0.0}, k{0.0, 0.0};
     long long n = 0;
-    for (uint32_t t = item_begin[b]; t < item_begin[b + 1]; ++t) {
-        a = kmergep(a, part_pk[t]);
-        k = kmergep(k, part_ks[t]);
-        n += part_cnt[t];
+    for (uint32_t t = item_begin[b] + threadIdx.x; t < item_begin[b + 1]; t += 256) {
+        a = kmergep(a, part_pk[pofs + t]);
+        k = kmergep(k, part_ks[pofs + t]);
+        n += part_cnt[pofs + t];
     }
-    out_pk[b] = a.s + a.c;
-    out_ks[b] = k.s + k.c;
-    out_cnt[b] = n;
+    sa[threadIdx.x] = a;
+    sk[threadIdx.x] = k;
+    sn[threadIdx.x] = n;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            sa[threadIdx.x] = kmergep(sa[threadIdx.x], sa[threadIdx.x + h]);
+            sk[threadIdx.x] = kmergep(sk[threadIdx.x], sk[threadIdx.x + h]);
+            sn[threadIdx.x] += sn[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out_pk[oofs + b] = sa[0].s + sa[0].c;
+        out_ks[oofs + b] = sk[0].s + sk[0].c;
+        out_cnt[oofs + b] = sn[0];
+    }
 }
 
 // bins of a lattice of side N as ranges [lo, hi) of the integer |k|^2, and
@@ -405,7 +417,7 @@ static int pk_batch(Context *c, const int *idx, int n, int binning, int nbins_in
         hipLaunchKernelGGL(k, dim3(nitems, (unsigned)n), dim3(kPkRows), 0, c->stream, N,
                            (const double2 *)plan->out, nbins, dib, dlo, dhi, ppk, pks, pcnt);
     }
-    hipLaunchKernelGGL(bin_fold_kernel, dim3((nbins + 63) / 64, (unsigned)n), dim3(64), 0,
+    hipLaunchKernelGGL(bin_fold_kernel, dim3((unsigned)nbins, (unsigned)n), dim3(256), 0,
                        c->stream, nbins, nitems, dib, ppk, pks, pcnt, dpk, dks, dcnt);
     AXS_HIP_TRY(hipGetLastError());
     std::vector<double> hpk((size_t)n * nb), hks((size_t)n * nb);

@@ -19,6 +19,8 @@
 // re-reads the ballot words and writes the coordinates at their ordered
 // positions.  The search is skipped when the same member's unchanged state
 // was searched by the previous call.
+#include <algorithm>
+
 #include "axs_internal.hpp"
 #include "axs_lattice.hpp"
 
@@ -441,12 +443,14 @@ __global__ void __launch_bounds__(256)
 member_count_kernel(const uint64_t *__restrict__ words, uint32_t L, int N,
                     unsigned long long *__restrict__ cnt)
 {
+    __shared__ unsigned long long part[2][4];
     const uint64_t nwords = ((uint64_t)L + 63) / 64;
-    const uint64_t w = blockIdx.x * 256ull + threadIdx.x;
     unsigned long long n = 0, pairs = 0;
-    if (w < nwords) {
+    // grid-stride over the words, one pair of atomics per work-group
+    for (uint64_t w = blockIdx.x * 256ull + threadIdx.x; w < nwords;
+         w += (uint64_t)gridDim.x * 256ull) {
         uint64_t bits = words[w];
-        n = (unsigned long long)__popcll(bits);
+        n += (unsigned long long)__popcll(bits);
         if (ND == 2) {
             while (bits) {
                 const uint64_t m = w * 64 + (uint64_t)(__ffsll((unsigned long long)bits) - 1);
@@ -473,9 +477,17 @@ member_count_kernel(const uint64_t *__restrict__ words, uint32_t L, int N,
         n += __shfl_down(n, o, 64);
         pairs += __shfl_down(pairs, o, 64);
     }
-    if ((threadIdx.x & 63) == 0) {
-        if (n) atomicAdd(&cnt[0], n);
-        if (pairs) atomicAdd(&cnt[1], pairs);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        part[0][wave] = n;
+        part[1][wave] = pairs;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long tn = part[0][0] + part[0][1] + part[0][2] + part[0][3];
+        const unsigned long long tp = part[1][0] + part[1][1] + part[1][2] + part[1][3];
+        if (tn) atomicAdd(&cnt[0], tn);
+        if (tp) atomicAdd(&cnt[1], tp);
     }
 }
 
@@ -687,7 +699,8 @@ int run_string_counts(Context *c, int64_t *n_plaquettes, int64_t *num_cores)
         Member &m = c->members[r];
         AXS_TRY(search_member(c, m, words, bcount));
         const uint32_t L = (uint32_t)m.L;
-        const unsigned g = (unsigned)(((L + 63) / 64 + 255) / 256);
+        const uint64_t nw = ((uint64_t)L + 63) / 64;
+        const unsigned g = (unsigned)std::min<uint64_t>((nw + 255) / 256, 4ull * c->num_cu);
         auto k = (c->nd == 2) ? member_count_kernel<2> : member_count_kernel<3>;
         hipLaunchKernelGGL(k, dim3(g), dim3(256), 0, c->stream, words, L, m.N, counters + 2 * r);
         AXS_HIP_TRY(hipGetLastError());
