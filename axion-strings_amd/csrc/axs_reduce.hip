@@ -7,6 +7,8 @@
 // block writes one pair per quantity, and a single-block pass folds the
 // block pairs in block order.  The partition depends only on the lattice
 // size, so results are identical from run to run.
+#include <algorithm>
+
 #include "axs_internal.hpp"
 #include "axs_lattice.hpp"
 
@@ -89,34 +91,68 @@ background_kernel(uint32_t L, P phi, V vel, Kahan *__restrict__ part)
         for (int q = 0; q < 6; ++q) part[blockIdx.x * 6 + q] = k[q];
 }
 
+// Energy densities summed over the lattice.  A work-item owns one site q
+// of the plane orthogonal to the slowest axis (3D: i, 2D: j) and marches
+// along that axis over a chunk of CI planes: the forward neighbour along
+// the slow axis is the next iteration's centre, kept in registers, so phi
+// is read from HBM once per site (the in-plane neighbours +1 are cache
+// hits).  The gradient terms are added in axis order (3D: i, j, k; 2D: i,
+// j), as the reference's axis order of neighbours.
 template <int ND, class P, class V>
 __global__ void __launch_bounds__(kRedBlock)
-energy_kernel(int N, uint32_t L, P phi, V vel, double tau2, double T03, double lam_4tau2,
-              Kahan *__restrict__ part)
+energy_kernel(int N, int CI, uint32_t ncols, P phi, V vel, double tau2, double T03,
+              double lam_4tau2, Kahan *__restrict__ part)
 {
     __shared__ Kahan sh[3][kRedBlock / 64];
     Kahan k[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) k[q] = Kahan{0.0, 0.0};
-    uint32_t stride = gridDim.x * kRedBlock;
-    for (uint32_t m = blockIdx.x * kRedBlock + threadIdx.x; m < L; m += stride) {
-        Site<ND> s = decompose<ND>(m, N);
-        double a, b, va, vb;
-        phi.get(m, a, b);
-        vel.get(m, va, vb);
-        double g = 0.0;
-#pragma unroll
-        for (int d = 0; d < ND; ++d) {
-            uint32_t q = shifted<ND>(s, d, 1, N);
-            double n1, n2;
-            phi.get(q, n1, n2);
-            double d1 = n1 - a, d2 = n2 - b;
-            g += d1 * d1 + d2 * d2;
+    const uint32_t plane = (ND == 3) ? (uint32_t)N * (uint32_t)N : (uint32_t)N;
+    const uint32_t stride = gridDim.x * kRedBlock;
+    for (uint32_t col = blockIdx.x * kRedBlock + threadIdx.x; col < ncols; col += stride) {
+        const uint32_t q = col % plane, ch = col / plane;
+        // in-plane forward neighbours: 3D (j, k+1) and (j+1, k); 2D i+1
+        uint32_t qk, qj = 0;
+        if (ND == 3) {
+            const uint32_t j = q / (uint32_t)N, kk = q - j * (uint32_t)N;
+            qk = j * (uint32_t)N + (kk + 1 == (uint32_t)N ? 0u : kk + 1);
+            qj = (j + 1 == (uint32_t)N ? 0u : j + 1) * (uint32_t)N + kk;
+        } else {
+            qk = (q + 1 == (uint32_t)N) ? 0u : q + 1;
         }
-        double S = a * a + b * b - tau2 + T03;
-        kadd(k[0], 0.5 * (va * va + vb * vb));
-        kadd(k[1], 0.5 * g);
-        kadd(k[2], lam_4tau2 * S * S);
+        const int s0 = (int)ch * CI, s1 = (s0 + CI < N) ? s0 + CI : N;
+        double a, b;
+        phi.get((uint32_t)s0 * plane + q, a, b);
+        for (int sl = s0; sl < s1; ++sl) {
+            const uint32_t base = (uint32_t)sl * plane;
+            const uint32_t nbase = (sl + 1 == N) ? 0u : base + plane;
+            double sa, sb, ka, kb, va, vb;
+            phi.get(nbase + q, sa, sb);              // slow-axis neighbour, next centre
+            phi.get(base + qk, ka, kb);
+            vel.get(base + q, va, vb);
+            double g = 0.0;
+            if (ND == 3) {
+                double ja, jb;
+                phi.get(base + qj, ja, jb);
+                double d1 = sa - a, d2 = sb - b;
+                g += d1 * d1 + d2 * d2;
+                d1 = ja - a, d2 = jb - b;
+                g += d1 * d1 + d2 * d2;
+                d1 = ka - a, d2 = kb - b;
+                g += d1 * d1 + d2 * d2;
+            } else {
+                double d1 = ka - a, d2 = kb - b;
+                g += d1 * d1 + d2 * d2;
+                d1 = sa - a, d2 = sb - b;
+                g += d1 * d1 + d2 * d2;
+            }
+            const double S = a * a + b * b - tau2 + T03;
+            kadd(k[0], 0.5 * (va * va + vb * vb));
+            kadd(k[1], 0.5 * g);
+            kadd(k[2], lam_4tau2 * S * S);
+            a = sa;
+            b = sb;
+        }
     }
     block_reduce<3>(k, sh);
     if (threadIdx.x == 0)
@@ -168,8 +204,15 @@ int run_energy(Context *c, Member &m, double out[3])
 {
     FieldView fv;
     AXS_TRY(current_fields(c, m, &fv));
-    uint32_t L = (uint32_t)m.L;
-    int nb = red_blocks(L);
+    const int N = m.N;
+    const uint32_t plane = (c->nd == 3) ? (uint32_t)N * (uint32_t)N : (uint32_t)N;
+    // chunks along the slow axis: as few as still give a work-item to every
+    // lane of four work-groups per CU
+    const uint64_t want = (uint64_t)c->num_cu * 4 * kRedBlock;
+    const uint32_t nch = (uint32_t)std::min<uint64_t>((uint64_t)N, (want + plane - 1) / plane);
+    const int CI = (int)((N + nch - 1) / nch);
+    const uint32_t ncols = plane * (uint32_t)((N + CI - 1) / CI);
+    int nb = red_blocks(ncols);
     size_t part_b = (size_t)nb * 3 * sizeof(Kahan);
     AXS_TRY(ensure_scratch(c, part_b + 64));
     Kahan *part = (Kahan *)c->scratch;
@@ -180,12 +223,12 @@ int run_energy(Context *c, Member &m, double out[3])
     double lam_4tau2 = lam / (4.0 * tau2);
     if (c->prec == AXS_FP64) {
         auto k = (c->nd == 2) ? energy_kernel<2, PhiF64, VelF64> : energy_kernel<3, PhiF64, VelF64>;
-        hipLaunchKernelGGL(k, dim3(nb), dim3(kRedBlock), 0, c->stream, m.N, L,
+        hipLaunchKernelGGL(k, dim3(nb), dim3(kRedBlock), 0, c->stream, N, CI, ncols,
                            PhiF64{(const dvec2 *)fv.phi}, VelF64{(const dvec2 *)fv.vel}, tau2, T03,
                            lam_4tau2, part);
     } else {
         auto k = (c->nd == 2) ? energy_kernel<2, PhiF32, VelF32> : energy_kernel<3, PhiF32, VelF32>;
-        hipLaunchKernelGGL(k, dim3(nb), dim3(kRedBlock), 0, c->stream, m.N, L,
+        hipLaunchKernelGGL(k, dim3(nb), dim3(kRedBlock), 0, c->stream, N, CI, ncols,
                            PhiF32{(const fvec4 *)fv.phi}, VelF32{(const fvec2 *)fv.vel}, tau2, T03,
                            lam_4tau2, part);
     }
