@@ -109,7 +109,11 @@ energy_kernel(int N, int CI, uint32_t ncols, P phi, V vel, double tau2, double T
     for (int q = 0; q < 3; ++q) k[q] = Kahan{0.0, 0.0};
     const uint32_t plane = (ND == 3) ? (uint32_t)N * (uint32_t)N : (uint32_t)N;
     const uint32_t stride = gridDim.x * kRedBlock;
-    for (uint32_t col = blockIdx.x * kRedBlock + threadIdx.x; col < ncols; col += stride) {
+    // XCD-aware order (work-groups go round-robin to the 8 XCDs): each XCD
+    // takes a contiguous run of columns, so row j + 1 is read on the same L2
+    uint32_t blk = blockIdx.x;
+    if ((gridDim.x & 7u) == 0) blk = (blk & 7u) * (gridDim.x >> 3) + (blk >> 3);
+    for (uint32_t col = blk * kRedBlock + threadIdx.x; col < ncols; col += stride) {
         const uint32_t q = col % plane, ch = col / plane;
         // in-plane forward neighbours: 3D (j, k+1) and (j+1, k); 2D i+1
         uint32_t qk, qj = 0;

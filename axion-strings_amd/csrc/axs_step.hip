@@ -18,6 +18,8 @@
 // FP32 form: phi and u = phidot + dt/2 ker are stored in FP32; one step is
 //   phi' = phi + dt u;  ker' = F(phi', tau');  u' = u + dt ker'
 // which is the same velocity-Verlet map written in half-step velocities.
+#include <algorithm>
+
 #include "axs_internal.hpp"
 #include "axs_lattice.hpp"
 
@@ -225,42 +227,148 @@ int prepare_batch(Context *c)
     return AXS_OK;
 }
 
-// FP32: ker = F(phi, tau) at the current state and phidot = u - dt/2 ker
-template <int ND>
+// FP32: ker = F(phi, tau) at the current state and phidot = u - dt/2 ker.
+// A work-item owns one site q of the plane orthogonal to the slowest axis
+// (3D: i, 2D: j) and marches along that axis over a chunk of CI planes,
+// keeping phi of planes sl-R .. sl+R in a register ring: every plane is
+// read from HBM once per chunk (the in-plane neighbours, whose offsets are
+// fixed along the march, are cache hits).  The neighbour terms are added in
+// the one-step kernels' order (distance l, then axis, +l before -l) with
+// the same F32Math expressions, so ker is theirs bit for bit.
+template <int ND, int R>
 __global__ void __launch_bounds__(256)
-canonical_fp32_kernel(int N, uint32_t L, const fvec4 *__restrict__ in, fvec2 *__restrict__ ker,
-                      fvec2 *__restrict__ vel, StepScalarsF s, int recompute)
+canonical_fp32_kernel(int N, int CI, uint32_t ncols, const fvec4 *__restrict__ in,
+                      fvec2 *__restrict__ ker, fvec2 *__restrict__ vel, StepScalarsF s,
+                      int recompute)
 {
-    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= L) return;
-    fvec4 c = in[m];
-    fvec2 k;
-    if (recompute) {
-        Site<ND> site = decompose<ND>(m, N);
-        float s1 = -0.0f, s2 = -0.0f;
-        for (int l = 1; l < s.ncoef; ++l) {
-            float cl = s.c[l];
+    // XCD-aware order: hardware deals work-groups round-robin to the 8 XCDs;
+    // give each XCD a contiguous run of columns, so that the rows j +- l a
+    // work-group reads are being read by work-groups on the same L2
+    uint32_t b = blockIdx.x;
+    if ((gridDim.x & 7u) == 0) b = (b & 7u) * (gridDim.x >> 3) + (b >> 3);
+    const uint32_t col = b * 256u + threadIdx.x;
+    if (col >= ncols) return;
+    const uint32_t plane = (ND == 3) ? (uint32_t)N * (uint32_t)N : (uint32_t)N;
+    const uint32_t q = col % plane;
+    const int s0 = (int)(col / plane) * CI, s1 = (s0 + CI < N) ? s0 + CI : N;
+    if (!recompute) {
+        for (int sl = s0; sl < s1; ++sl) {
+            const uint32_t m = (uint32_t)sl * plane + q;
+            const fvec4 c = in[m];
+            const fvec2 k = ker[m];
+            fvec2 v;
+            v.x = c.z - s.half_dt * k.x;
+            v.y = c.w - s.half_dt * k.y;
+            vel[m] = v;
+        }
+        return;
+    }
+    auto wrap = [N](int x) {               // periodic index, |x| within a few N
+        while (x < 0) x += N;
+        while (x >= N) x -= N;
+        return x;
+    };
+    // in-plane neighbour offsets at distance l (+l, -l) along the in-plane
+    // axes: 3D axis 1 (j) and axis 2 (k); 2D axis 0 (i)
+    uint32_t ap[R], am[R], bp[R], bm[R];
+    if (ND == 3) {
+        const int j = (int)(q / (uint32_t)N), k = (int)(q - (uint32_t)j * (uint32_t)N);
 #pragma unroll
-            for (int a = 0; a < ND; ++a) {
-                fvec4 p = in[shifted<ND>(site, a, l, N)];
-                fvec4 q = in[shifted<ND>(site, a, -l, N)];
-                s1 = F32Math::acc(s1, cl, p.x);
-                s2 = F32Math::acc(s2, cl, p.y);
-                s1 = F32Math::acc(s1, cl, q.x);
-                s2 = F32Math::acc(s2, cl, q.y);
+        for (int l = 1; l <= R; ++l) {
+            ap[l - 1] = (uint32_t)wrap(j + l) * (uint32_t)N + (uint32_t)k;
+            am[l - 1] = (uint32_t)wrap(j - l) * (uint32_t)N + (uint32_t)k;
+            bp[l - 1] = (uint32_t)j * (uint32_t)N + (uint32_t)wrap(k + l);
+            bm[l - 1] = (uint32_t)j * (uint32_t)N + (uint32_t)wrap(k - l);
+        }
+    } else {
+#pragma unroll
+        for (int l = 1; l <= R; ++l) {
+            ap[l - 1] = (uint32_t)wrap((int)q + l);
+            am[l - 1] = (uint32_t)wrap((int)q - l);
+            bp[l - 1] = bm[l - 1] = 0;
+        }
+    }
+    // phi (x, y) of a site as one 8-byte load from the (phi, u) array
+    const fvec2 *__restrict__ in2 = reinterpret_cast<const fvec2 *>(in);
+    auto xy = [&](uint32_t m) { return in2[2 * (size_t)m]; };
+    // one plane's operands: centre (phi, u) and the in-plane neighbours;
+    // the next plane's are loaded while the current one is computed
+    struct Plane {
+        fvec4 c;
+        fvec2 a[2 * R], b[2 * R];
+    };
+    auto load = [&](int sl, Plane &P) {
+        const uint32_t base = (uint32_t)sl * plane;
+        P.c = in[base + q];
+#pragma unroll
+        for (int l = 0; l < R; ++l) {
+            P.a[2 * l] = xy(base + ap[l]);
+            P.a[2 * l + 1] = xy(base + am[l]);
+            if (ND == 3) {
+                P.b[2 * l] = xy(base + bp[l]);
+                P.b[2 * l + 1] = xy(base + bm[l]);
             }
         }
-        float t = F32Math::own(s.cl, F32Math::pot(c.x, c.y, s.pot0), s.ac0);
-        k.x = F32Math::force(s.alpha, s1, c.x, t);
-        k.y = F32Math::force(s.alpha, s2, c.y, t);
-        ker[m] = k;
-    } else {
-        k = ker[m];
+    };
+    fvec2 ring[2 * R + 1];                 // phi of slow index sl - R + t
+#pragma unroll
+    for (int t = 0; t < 2 * R + 1; ++t) ring[t] = xy((uint32_t)wrap(s0 - R + t) * plane + q);
+    Plane cur, nxt;
+    load(s0, cur);
+    for (int sl = s0; sl < s1; ++sl) {
+        const uint32_t base = (uint32_t)sl * plane;
+        fvec2 ring_next = ring[2 * R];
+        if (sl + 1 < s1) {
+            load(sl + 1, nxt);
+            ring_next = xy((uint32_t)wrap(sl + 1 + R) * plane + q);
+        }
+        const fvec4 c = cur.c;
+        float s1x = -0.0f, s2x = -0.0f;
+#pragma unroll
+        for (int l = 1; l <= R; ++l) {
+            const float cl = s.c[l];
+            const fvec2 p1 = cur.a[2 * l - 2], q1 = cur.a[2 * l - 1];
+            if (ND == 3) {
+                // axis 0: the slow axis, from the ring; then j, then k
+                s1x = F32Math::acc(s1x, cl, ring[R + l].x);
+                s2x = F32Math::acc(s2x, cl, ring[R + l].y);
+                s1x = F32Math::acc(s1x, cl, ring[R - l].x);
+                s2x = F32Math::acc(s2x, cl, ring[R - l].y);
+                s1x = F32Math::acc(s1x, cl, p1.x);
+                s2x = F32Math::acc(s2x, cl, p1.y);
+                s1x = F32Math::acc(s1x, cl, q1.x);
+                s2x = F32Math::acc(s2x, cl, q1.y);
+                const fvec2 p2 = cur.b[2 * l - 2], q2 = cur.b[2 * l - 1];
+                s1x = F32Math::acc(s1x, cl, p2.x);
+                s2x = F32Math::acc(s2x, cl, p2.y);
+                s1x = F32Math::acc(s1x, cl, q2.x);
+                s2x = F32Math::acc(s2x, cl, q2.y);
+            } else {
+                // axis 0: in-plane i; axis 1: the slow axis j, from the ring
+                s1x = F32Math::acc(s1x, cl, p1.x);
+                s2x = F32Math::acc(s2x, cl, p1.y);
+                s1x = F32Math::acc(s1x, cl, q1.x);
+                s2x = F32Math::acc(s2x, cl, q1.y);
+                s1x = F32Math::acc(s1x, cl, ring[R + l].x);
+                s2x = F32Math::acc(s2x, cl, ring[R + l].y);
+                s1x = F32Math::acc(s1x, cl, ring[R - l].x);
+                s2x = F32Math::acc(s2x, cl, ring[R - l].y);
+            }
+        }
+        const float t = F32Math::own(s.cl, F32Math::pot(c.x, c.y, s.pot0), s.ac0);
+        fvec2 k;
+        k.x = F32Math::force(s.alpha, s1x, c.x, t);
+        k.y = F32Math::force(s.alpha, s2x, c.y, t);
+        ker[base + q] = k;
+        fvec2 v;
+        v.x = c.z - s.half_dt * k.x;
+        v.y = c.w - s.half_dt * k.y;
+        vel[base + q] = v;
+#pragma unroll
+        for (int t2 = 0; t2 < 2 * R; ++t2) ring[t2] = ring[t2 + 1];
+        ring[2 * R] = ring_next;
+        cur = nxt;
     }
-    fvec2 v;
-    v.x = c.z - s.half_dt * k.x;
-    v.y = c.w - s.half_dt * k.y;
-    vel[m] = v;
 }
 
 static inline unsigned grid_for(uint32_t L, unsigned block) { return (L + block - 1) / block; }
@@ -330,15 +438,42 @@ int launch_step(Context *c, const StepScalars &s)
     return AXS_OK;
 }
 
+template <int ND>
+static void launch_canonical_nd(Context *c, int R, unsigned grid, int N, int CI, uint32_t ncols,
+                                const fvec4 *in, fvec2 *ker, fvec2 *vel, const StepScalarsF &f,
+                                int recompute)
+{
+    auto k = (R == 1)   ? canonical_fp32_kernel<ND, 1>
+             : (R == 2) ? canonical_fp32_kernel<ND, 2>
+             : (R == 3) ? canonical_fp32_kernel<ND, 3>
+                        : canonical_fp32_kernel<ND, 4>;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, c->stream, N, CI, ncols, in, ker, vel, f,
+                       recompute);
+}
+
 int launch_fp32_canonical(Context *c, Member &m, const StepScalars &s, bool recompute_ker)
 {
     StepScalarsF f = to_float(s);
-    uint32_t L = (uint32_t)m.L;
-    unsigned grid = grid_for(L, 256);
-    auto kern = (c->nd == 2) ? canonical_fp32_kernel<2> : canonical_fp32_kernel<3>;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, c->stream, m.N, L,
-                       (const fvec4 *)m.state[c->cur][PHI], (fvec2 *)m.aux_ker,
-                       (fvec2 *)m.aux_vel, f, recompute_ker ? 1 : 0);
+    const int N = m.N, R = f.ncoef - 1;
+    if (R < 1 || R > 4) {
+        set_error("stencil radius %d is not supported", R);
+        return AXS_ERR_UNSUPPORTED;
+    }
+    const uint32_t plane = (c->nd == 3) ? (uint32_t)N * (uint32_t)N : (uint32_t)N;
+    // chunks along the slow axis: as few as still give a work-item to every
+    // lane of four work-groups per CU
+    const uint64_t want = (uint64_t)c->num_cu * 4 * 256;
+    const uint32_t nch = (uint32_t)std::min<uint64_t>((uint64_t)N, (want + plane - 1) / plane);
+    const int CI = (int)((N + nch - 1) / nch);
+    const uint32_t ncols = plane * (uint32_t)((N + CI - 1) / CI);
+    const unsigned grid = grid_for(ncols, 256);
+    const fvec4 *in = (const fvec4 *)m.state[c->cur][PHI];
+    if (c->nd == 2)
+        launch_canonical_nd<2>(c, R, grid, N, CI, ncols, in, (fvec2 *)m.aux_ker,
+                               (fvec2 *)m.aux_vel, f, recompute_ker ? 1 : 0);
+    else
+        launch_canonical_nd<3>(c, R, grid, N, CI, ncols, in, (fvec2 *)m.aux_ker,
+                               (fvec2 *)m.aux_vel, f, recompute_ker ? 1 : 0);
     AXS_HIP_TRY(hipGetLastError());
     return AXS_OK;
 }
