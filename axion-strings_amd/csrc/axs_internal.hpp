@@ -190,6 +190,7 @@ struct Context {
     int str_member = -1;
     uint64_t str_version = 0;
     uint64_t str_total = 0;
+    bool str_scanned = false;   // block offsets of the cached words computed
 
     ~Context();
 };

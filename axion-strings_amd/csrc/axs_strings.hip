@@ -606,23 +606,27 @@ int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *co
     unsigned long long *pairs = (unsigned long long *)(sp + lay.words_b + lay.cnt_b + lay.off_b);
     uint64_t *tsum = (uint64_t *)(sp + lay.words_b + lay.cnt_b + lay.off_b + 256);
 
-    uint64_t total = 0;
-    if (cached) {
-        total = c->str_total;
-    } else {
-        AXS_TRY(search_member(c, m, words, bcount));
+    auto scan = [&]() {
         hipLaunchKernelGGL(tile_scan_kernel, dim3(ntiles), dim3(1024), 0, c->stream, bcount, off,
                            tsum, nblocks);
         hipLaunchKernelGGL(top_scan_kernel, dim3(1), dim3(1024), 0, c->stream, tsum, ntiles,
                            off + nblocks);
         hipLaunchKernelGGL(add_tile_offsets_kernel, dim3((nblocks + 255) / 256), dim3(256), 0,
                            c->stream, off, tsum, nblocks);
-        AXS_HIP_TRY(hipGetLastError());
+        return hipGetLastError();
+    };
+    uint64_t total = 0;
+    if (cached) {
+        total = c->str_total;
+    } else {
+        AXS_TRY(search_member(c, m, words, bcount));
+        AXS_HIP_TRY(scan());
         AXS_HIP_TRY(hipMemcpyAsync(&total, off + nblocks, 8, hipMemcpyDeviceToHost, c->stream));
         AXS_HIP_TRY(hipStreamSynchronize(c->stream));
         c->str_member = member;
         c->str_version = m.version;
         c->str_total = total;
+        c->str_scanned = true;
     }
     *count = (int64_t)total;
 
@@ -630,6 +634,10 @@ int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *co
     if (!need_list) {
         if (num_cores) *num_cores = (int64_t)total;
         return (coords == nullptr || cap >= (int64_t)total) ? AXS_OK : AXS_ERR_CAPACITY;
+    }
+    if (!c->str_scanned) {         // words cached by a count: offsets for the writer
+        AXS_HIP_TRY(scan());
+        c->str_scanned = true;
     }
 
     // coordinate list in its own buffer (scratch keeps words / offsets)
@@ -691,13 +699,16 @@ int run_string_counts(Context *c, int64_t *n_plaquettes, int64_t *num_cores)
     if (c->nd == 2) AXS_TRY(ensure_scratch(c, (size_t)Lmax * sizeof(double)));
     char *sp = (char *)c->strbuf;
     uint64_t *words = (uint64_t *)sp;
-    uint32_t *bcount = (uint32_t *)(sp + lay.words_b);
     unsigned long long *counters =
         (unsigned long long *)(sp + lay.words_b + lay.cnt_b + lay.off_b + lay.tail_b);
     AXS_HIP_TRY(hipMemsetAsync(counters, 0, nm * 16, c->stream));
     for (size_t r = 0; r < nm; ++r) {
         Member &m = c->members[r];
-        AXS_TRY(search_member(c, m, words, bcount));
+        // block counts where the member's own layout keeps them (below the
+        // counters of the largest layout), so the last member's search can
+        // be reused by a coordinate list
+        const StrLayout lr((uint32_t)m.L, nm);
+        AXS_TRY(search_member(c, m, words, (uint32_t *)(sp + lr.words_b)));
         const uint32_t L = (uint32_t)m.L;
         const uint64_t nw = ((uint64_t)L + 63) / 64;
         const unsigned g = (unsigned)std::min<uint64_t>((nw + 255) / 256, 4ull * c->num_cu);
@@ -712,6 +723,12 @@ int run_string_counts(Context *c, int64_t *n_plaquettes, int64_t *num_cores)
         if (n_plaquettes) n_plaquettes[r] = (int64_t)h[2 * r];
         if (num_cores) num_cores[r] = (int64_t)(h[2 * r] - h[2 * r + 1]);
     }
+    // the words and block counts of the last member stay valid: a
+    // coordinate list of it next needs only the scan
+    c->str_member = (int)nm - 1;
+    c->str_version = c->members[nm - 1].version;
+    c->str_total = h[2 * (nm - 1)];
+    c->str_scanned = false;
     return AXS_OK;
 }
 

@@ -185,6 +185,28 @@ def test_powerspec_all(nd, Ns, binning, prec):
         np.testing.assert_allclose(pk, pk1, rtol=1e-12, atol=1e-14 * pk1.max())
 
 
+@pytest.mark.parametrize("nd,Ns", [(3, [96, 64]), (3, [64, 130]), (2, [256, 64]), (2, [66])])
+def test_string_list_after_counts(nd, Ns):
+    """A coordinate list right after axs_string_counts reuses the last
+    member's search (only the offsets are computed) and searches any other
+    member afresh: every list is the oracle's."""
+    lat = axs.Lattice(nd, members=Ns)
+    fields = []
+    for r, N in enumerate(Ns):
+        rng = np.random.default_rng(40 + r)
+        p1, p2 = rng.standard_normal(N ** nd), rng.standard_normal(N ** nd)
+        z = np.zeros(N ** nd)
+        lat.set_state(p1, p2, z, z, member=r)
+        fields.append((p1, p2))
+    npl, _ = lat.string_counts()
+    for r in [len(Ns) - 1] + list(range(len(Ns) - 1)):
+        N = Ns[r]
+        n_ref, c_ref = O.cores(O.axion(*fields[r]), nd, N, thr=1)
+        n, c = lat.strings(member=r)
+        assert n == n_ref == npl[r]
+        np.testing.assert_array_equal(c, c_ref)
+
+
 def test_strings_hand_built_vortices():
     N = 64
     p1, p2 = vortex2d(N, [(10.5, 12.5), (40.5, 30.5), (20.5, 50.5)], [1, -1, 1])
