@@ -435,6 +435,8 @@ def load_host_library(path: str = HOST_LIB_PATH) -> ctypes.CDLL:
     h.axs_host_init_fields.argtypes = [c_int, c_int, c_int, c_float, c_float, c_float,
                                        ctypes.c_uint64] + [_DP] * 4
     h.axs_host_init_fields.restype = c_int
+    h.axs_host_philox4x32_10.argtypes = [POINTER(ctypes.c_uint32)] * 3
+    h.axs_host_philox4x32_10.restype = None
     h.axs_host_fft.argtypes = [_DP, c_int, c_int, c_int]
     h.axs_host_fft.restype = c_int
     h.axs_host_final_step.argtypes = [c_int, c_float, c_float]
@@ -468,6 +470,16 @@ def host_init(mode: str, ndims: int, N: int, space_step: float = 0.5, lam: float
     if rc != 0:
         raise ValueError(f"axs_host_init_fields: {rc}")
     return out
+
+
+def host_philox(ctr, key):
+    """one Philox4x32-10 block of the initial-condition generators"""
+    h = load_host_library()
+    c = (ctypes.c_uint32 * 4)(*ctr)
+    k = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    h.axs_host_philox4x32_10(c, k, o)
+    return [int(x) for x in o]
 
 
 def host_fft(data, ndims: int, N: int, sign: int = -1) -> np.ndarray:

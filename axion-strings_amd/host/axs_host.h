@@ -25,6 +25,10 @@ int axs_host_init_fields(int mode, int ndims, int N, float space_step, float lam
                          float T_initial, uint64_t seed, double *phi1, double *phi2,
                          double *phidot1, double *phidot2);
 
+/* One Philox4x32-10 block (Salmon et al., SC'11): the counter-based
+ * generator behind both initial-condition generators (host and device).  */
+void axs_host_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
 /* In-place complex FFT of an N^ndims (ndims 1..3) interleaved array. */
 int axs_host_fft(double *data, int ndims, int N, int sign);
 

@@ -56,6 +56,16 @@ void normal_pair(uint64_t seed, uint32_t stream, uint64_t idx, double &g1, doubl
 
 }  // namespace
 
+extern "C" void axs_host_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
+                                       uint32_t out[4])
+{
+    U4 r = philox4x32_10(U4{ctr[0], ctr[1], ctr[2], ctr[3]}, key[0], key[1]);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+    out[3] = r.w;
+}
+
 extern "C" int axs_host_init_fields(int mode, int ndims, int N, float space_step, float lambda,
                                     float T_initial, uint64_t seed, double *phi1, double *phi2,
                                     double *phidot1, double *phidot2)

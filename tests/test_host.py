@@ -137,3 +137,17 @@ def test_host_init_thermal_spectrum():
         r_phi = np.mean(np.abs(F[sel]) ** 2 / (2 * nb[sel] / w[sel]))
         r_vel = np.mean(np.abs(V[sel]) ** 2 / (2 * nb[sel] * w[sel]))
         assert 0.85 < r_phi < 1.15 and 0.85 < r_vel < 1.15, (lo, r_phi, r_vel)
+
+
+def test_philox4x32_10_known_answers():
+    # the published Philox4x32-10 known-answer vectors (Random123's
+    # kat_vectors: counter, key -> output): all-zero, all-ones, and the
+    # digits-of-pi case
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+        ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+        ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+         [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+    ]
+    for ctr, key, out in kat:
+        assert axs.host_philox(ctr, key) == out, [hex(x) for x in axs.host_philox(ctr, key)]
