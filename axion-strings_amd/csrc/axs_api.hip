@@ -250,6 +250,7 @@ Context::~Context()
     if (scratch) (void)hipFree(scratch);
     if (listbuf) (void)hipFree(listbuf);
     if (strbuf) (void)hipFree(strbuf);
+    if (strtab) (void)hipFree(strtab);
     for (int h = 0; h < 2; ++h)
         if (batch_tab[h]) (void)hipFree(batch_tab[h]);
     for (MultiTables &t : mtab)

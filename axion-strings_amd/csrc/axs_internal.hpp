@@ -191,6 +191,9 @@ struct Context {
     uint64_t str_version = 0;
     uint64_t str_total = 0;
     bool str_scanned = false;   // block offsets of the cached words computed
+    // ensemble-wide 2D count: member table and block -> member map (built once)
+    void *strtab = nullptr;
+    uint32_t strtab_blocks = 0;
 
     ~Context();
 };

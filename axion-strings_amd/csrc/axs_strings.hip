@@ -100,15 +100,17 @@ __device__ __forceinline__ bool plaquette_flag(const double *__restrict__ ang, u
     }
 }
 
+// plaquette test of block `blk` (kStrSpan sites) of one lattice from its
+// angles: one ballot word per wave and pass, the block's count in bcount
 template <int ND>
-__global__ void __launch_bounds__(kStrBlock)
-strings_flag_kernel(int N, uint32_t L, const double *__restrict__ ang, double accept,
-                    uint64_t *__restrict__ words, uint32_t *__restrict__ bcount)
+__device__ __forceinline__ void flag_block(int N, uint32_t L, const double *__restrict__ ang,
+                                           double accept, uint64_t *__restrict__ words,
+                                           uint32_t *__restrict__ bcount, uint32_t blk)
 {
     __shared__ uint32_t wave_tot[kStrBlock / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t total = 0;
-    uint32_t base = blockIdx.x * (uint32_t)kStrSpan;
+    uint32_t base = blk * (uint32_t)kStrSpan;
     for (int it = 0; it < kStrIters; ++it) {
         uint32_t m = base + it * kStrBlock + threadIdx.x;
         bool f = (m < L) && plaquette_flag<ND>(ang, m, N, accept);
@@ -122,8 +124,57 @@ strings_flag_kernel(int N, uint32_t L, const double *__restrict__ ang, double ac
     if (threadIdx.x == 0) {
         uint32_t t = 0;
         for (int w = 0; w < kStrBlock / 64; ++w) t += wave_tot[w];
-        bcount[blockIdx.x] = t;
+        bcount[blk] = t;
     }
+}
+
+template <int ND>
+__global__ void __launch_bounds__(kStrBlock)
+strings_flag_kernel(int N, uint32_t L, const double *__restrict__ ang, double accept,
+                    uint64_t *__restrict__ words, uint32_t *__restrict__ bcount)
+{
+    flag_block<ND>(N, L, ang, accept, words, bcount, blockIdx.x);
+}
+
+// Ensemble-wide 2D finder: member r owns blocks [first_block, first_block +
+// ceil(L / kStrSpan)) of one concatenated index space (its words start at
+// word first_block * kStrSpan / 64, its angles at site_base); blk2mem maps
+// a block to its member.
+struct StrMember {
+    const void *phi[2];      // the field of either ping-pong half
+    uint64_t site_base;
+    uint32_t L, first_block;
+    int N, pad;
+};
+
+template <class P>
+__global__ void __launch_bounds__(kStrBlock)
+angle_batch_kernel(const StrMember *__restrict__ mem, const uint32_t *__restrict__ blk2mem,
+                   int half, double *__restrict__ ang)
+{
+    const StrMember M = mem[blk2mem[blockIdx.x]];
+    P phi;
+    phi.p = (decltype(phi.p))M.phi[half];
+    const uint32_t base = (blockIdx.x - M.first_block) * (uint32_t)kStrSpan;
+    for (int it = 0; it < kStrIters; ++it) {
+        const uint32_t m = base + it * kStrBlock + threadIdx.x;
+        if (m < M.L) {
+            double a1, a2;
+            phi.get(m, a1, a2);
+            ang[M.site_base + m] = scaled_angle(a1, a2);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kStrBlock)
+strings_flag_batch_kernel(const StrMember *__restrict__ mem, const uint32_t *__restrict__ blk2mem,
+                          const double *__restrict__ ang, double accept,
+                          uint64_t *__restrict__ words, uint32_t *__restrict__ bcount)
+{
+    const StrMember M = mem[blk2mem[blockIdx.x]];
+    const size_t wbase = (size_t)M.first_block * (kStrSpan / 64);
+    flag_block<2>(M.N, M.L, ang + M.site_base, accept, words + wbase, bcount + M.first_block,
+                  blockIdx.x - M.first_block);
 }
 
 // 3D finder in one pass over phi.  A work-group of kSfW waves owns a tile
@@ -438,6 +489,38 @@ adjacent_pairs_kernel(const int32_t *__restrict__ coords, uint64_t n,
 // ascending site order (m = i + N j) the entry after a flagged site m is the
 // next flagged site, and the pair counts when that is m + 1 in the same row
 // (di = 1, dj = 0) or m + N (di = 0, dj = 1).  Adds to cnt[0] and cnt[1].
+// flagged sites in word w of one lattice's words, and (2D) the pairs of
+// list-adjacent flagged sites whose first site lies in w
+template <int ND>
+__device__ __forceinline__ void count_word(const uint64_t *__restrict__ words, uint64_t nwords,
+                                           int N, uint64_t w, unsigned long long &n,
+                                           unsigned long long &pairs)
+{
+    uint64_t bits = words[w];
+    n += (unsigned long long)__popcll(bits);
+    if (ND == 2) {
+        while (bits) {
+            const uint64_t m = w * 64 + (uint64_t)(__ffsll((unsigned long long)bits) - 1);
+            bits &= bits - 1;
+            uint64_t next = ~0ull;                   // next flagged site after m
+            if (bits) {
+                next = w * 64 + (uint64_t)(__ffsll((unsigned long long)bits) - 1);
+            } else {
+                for (uint64_t v = w + 1; v < nwords && v * 64 <= m + (uint64_t)N; ++v) {
+                    const uint64_t x = words[v];
+                    if (x) {
+                        next = v * 64 + (uint64_t)(__ffsll((unsigned long long)x) - 1);
+                        break;
+                    }
+                }
+            }
+            if ((next == m + 1 && m % (uint64_t)N != (uint64_t)(N - 1)) ||
+                next == m + (uint64_t)N)
+                ++pairs;
+        }
+    }
+}
+
 template <int ND>
 __global__ void __launch_bounds__(256)
 member_count_kernel(const uint64_t *__restrict__ words, uint32_t L, int N,
@@ -448,31 +531,8 @@ member_count_kernel(const uint64_t *__restrict__ words, uint32_t L, int N,
     unsigned long long n = 0, pairs = 0;
     // grid-stride over the words, one pair of atomics per work-group
     for (uint64_t w = blockIdx.x * 256ull + threadIdx.x; w < nwords;
-         w += (uint64_t)gridDim.x * 256ull) {
-        uint64_t bits = words[w];
-        n += (unsigned long long)__popcll(bits);
-        if (ND == 2) {
-            while (bits) {
-                const uint64_t m = w * 64 + (uint64_t)(__ffsll((unsigned long long)bits) - 1);
-                bits &= bits - 1;
-                uint64_t next = ~0ull;                   // next flagged site after m
-                if (bits) {
-                    next = w * 64 + (uint64_t)(__ffsll((unsigned long long)bits) - 1);
-                } else {
-                    for (uint64_t v = w + 1; v < nwords && v * 64 <= m + (uint64_t)N; ++v) {
-                        const uint64_t x = words[v];
-                        if (x) {
-                            next = v * 64 + (uint64_t)(__ffsll((unsigned long long)x) - 1);
-                            break;
-                        }
-                    }
-                }
-                if ((next == m + 1 && m % (uint64_t)N != (uint64_t)(N - 1)) ||
-                    next == m + (uint64_t)N)
-                    ++pairs;
-            }
-        }
-    }
+         w += (uint64_t)gridDim.x * 256ull)
+        count_word<ND>(words, nwords, N, w, n, pairs);
     for (int o = 32; o > 0; o >>= 1) {
         n += __shfl_down(n, o, 64);
         pairs += __shfl_down(pairs, o, 64);
@@ -488,6 +548,35 @@ member_count_kernel(const uint64_t *__restrict__ words, uint32_t L, int N,
         const unsigned long long tp = part[1][0] + part[1][1] + part[1][2] + part[1][3];
         if (tn) atomicAdd(&cnt[0], tn);
         if (tp) atomicAdd(&cnt[1], tp);
+    }
+}
+
+// the ensemble's 2D counts: eight blocks per work-group, each block's 32
+// words on 32 lanes (one member per block), two atomics per block
+__global__ void __launch_bounds__(256)
+count_batch_kernel(const StrMember *__restrict__ mem, const uint32_t *__restrict__ blk2mem,
+                   uint32_t nblocks, const uint64_t *__restrict__ words,
+                   unsigned long long *__restrict__ cnt)
+{
+    const uint32_t blk = blockIdx.x * 8u + (threadIdx.x >> 5);
+    unsigned long long n = 0, pairs = 0;
+    uint32_t r = 0;
+    if (blk < nblocks) {
+        r = blk2mem[blk];
+        const StrMember M = mem[r];
+        const uint64_t nwords = ((uint64_t)M.L + 63) / 64;
+        const uint64_t w = (uint64_t)(blk - M.first_block) * (kStrSpan / 64) + (threadIdx.x & 31);
+        if (w < nwords)
+            count_word<2>(words + (size_t)M.first_block * (kStrSpan / 64), nwords, M.N, w, n,
+                          pairs);
+    }
+    for (int o = 16; o > 0; o >>= 1) {
+        n += __shfl_xor(n, o, 64);
+        pairs += __shfl_xor(pairs, o, 64);
+    }
+    if ((threadIdx.x & 31) == 0 && blk < nblocks) {
+        if (n) atomicAdd(&cnt[2 * r], n);
+        if (pairs) atomicAdd(&cnt[2 * r + 1], pairs);
     }
 }
 
@@ -678,6 +767,70 @@ int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *co
     return (coords == nullptr || cap >= (int64_t)total) ? AXS_OK : AXS_ERR_CAPACITY;
 }
 
+// axs_string_counts of a 2D ensemble: three launches for all members (the
+// angles, the plaquette test and the counts over the concatenated block
+// space), one copy, one synchronisation
+static int run_string_counts_2d_batch(Context *c, int64_t *n_plaquettes, int64_t *num_cores)
+{
+    const size_t nm = c->members.size();
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    std::vector<StrMember> tab(nm);
+    uint32_t nblocks = 0;
+    uint64_t sites = 0;
+    for (size_t r = 0; r < nm; ++r) {
+        const Member &m = c->members[r];
+        tab[r] = StrMember{{m.state[0][PHI], m.state[1][PHI]}, sites, (uint32_t)m.L, nblocks,
+                           m.N, 0};
+        nblocks += ((uint32_t)m.L + kStrSpan - 1) / kStrSpan;
+        sites += (uint64_t)m.L;
+    }
+    const size_t tab_b = align(nm * sizeof(StrMember));
+    if (!c->strtab) {          // members and their buffers are fixed: built once
+        std::vector<uint32_t> blk(nblocks);
+        for (size_t r = 0; r < nm; ++r) {
+            const uint32_t nb = (tab[r].L + kStrSpan - 1) / kStrSpan;
+            for (uint32_t b = 0; b < nb; ++b) blk[tab[r].first_block + b] = (uint32_t)r;
+        }
+        AXS_HIP_TRY(hipMalloc(&c->strtab, tab_b + (size_t)nblocks * 4));
+        AXS_HIP_TRY(hipMemcpy(c->strtab, tab.data(), nm * sizeof(StrMember),
+                              hipMemcpyHostToDevice));
+        AXS_HIP_TRY(hipMemcpy((char *)c->strtab + tab_b, blk.data(), (size_t)nblocks * 4,
+                              hipMemcpyHostToDevice));
+        c->strtab_blocks = nblocks;
+    }
+    const StrMember *dmem = (const StrMember *)c->strtab;
+    const uint32_t *dblk = (const uint32_t *)((char *)c->strtab + tab_b);
+    const size_t words_b = (size_t)nblocks * (kStrSpan / 64) * 8, cnt_b = align((size_t)nblocks * 4);
+    c->str_member = -1;
+    AXS_TRY(ensure_device(c, &c->strbuf, &c->strbuf_bytes, words_b + cnt_b + align(nm * 16)));
+    AXS_TRY(ensure_scratch(c, (size_t)sites * sizeof(double)));
+    char *sp = (char *)c->strbuf;
+    uint64_t *words = (uint64_t *)sp;
+    uint32_t *bcount = (uint32_t *)(sp + words_b);
+    unsigned long long *counters = (unsigned long long *)(sp + words_b + cnt_b);
+    double *ang = (double *)c->scratch;
+    AXS_HIP_TRY(hipMemsetAsync(counters, 0, nm * 16, c->stream));
+    if (c->prec == AXS_FP64)
+        hipLaunchKernelGGL(angle_batch_kernel<PhiF64>, dim3(nblocks), dim3(kStrBlock), 0,
+                           c->stream, dmem, dblk, c->cur, ang);
+    else
+        hipLaunchKernelGGL(angle_batch_kernel<PhiF32>, dim3(nblocks), dim3(kStrBlock), 0,
+                           c->stream, dmem, dblk, c->cur, ang);
+    hipLaunchKernelGGL(strings_flag_batch_kernel, dim3(nblocks), dim3(kStrBlock), 0, c->stream,
+                       dmem, dblk, ang, accept_threshold(c->p.thr), words, bcount);
+    hipLaunchKernelGGL(count_batch_kernel, dim3((nblocks + 7) / 8), dim3(256), 0, c->stream, dmem,
+                       dblk, nblocks, words, counters);
+    AXS_HIP_TRY(hipGetLastError());
+    std::vector<unsigned long long> h(2 * nm);
+    AXS_HIP_TRY(hipMemcpyAsync(h.data(), counters, nm * 16, hipMemcpyDeviceToHost, c->stream));
+    AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+    for (size_t r = 0; r < nm; ++r) {
+        if (n_plaquettes) n_plaquettes[r] = (int64_t)h[2 * r];
+        if (num_cores) num_cores[r] = (int64_t)(h[2 * r] - h[2 * r + 1]);
+    }
+    return AXS_OK;
+}
+
 // axs_string_counts: every member searched in turn on the stream, counted
 // from its ballot words into its own counters, one copy and one
 // synchronisation for the ensemble (the shared buffers are sized for the
@@ -693,6 +846,7 @@ int run_string_counts(Context *c, int64_t *n_plaquettes, int64_t *num_cores)
         Lmax = (uint32_t)m.L > Lmax ? (uint32_t)m.L : Lmax;
     }
     const size_t nm = c->members.size();
+    if (c->nd == 2 && nm > 1) return run_string_counts_2d_batch(c, n_plaquettes, num_cores);
     const StrLayout lay(Lmax, nm);
     c->str_member = -1;
     AXS_TRY(ensure_device(c, &c->strbuf, &c->strbuf_bytes, lay.bytes()));

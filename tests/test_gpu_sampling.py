@@ -65,9 +65,9 @@ def test_strings_3d_threshold_straddle(N, prec):
     accept (float 0.495, thr = 1) of a circle per site along k, plus noise
     of 1e-9 turns per site, so plaquettes b and c of every site sum to
     accept within 1e-8 (with a slight upward bias from the two short edges):
-    most sites are flagged and some are not.  The one-pass finder's float screen cannot
-    decide any of them: each site goes through its FP64 fallback, which
-    must give the oracle's set."""
+    most sites are flagged and some are not.  The one-pass finder's float
+    screen cannot decide any of them: each site goes through its FP64
+    fallback, which must give the oracle's set."""
     rng = np.random.default_rng(5)
     k = np.arange(N ** 3) % N
     accept = float(np.float32(0.495))
@@ -129,7 +129,8 @@ def test_strings_3d_straight_lines(axis, N):
     np.testing.assert_array_equal(c, c_ref)
 
 
-@pytest.mark.parametrize("nd,Ns", [(2, [64, 66, 130, 256, 64]), (2, [8]), (3, [64, 34, 96])])
+@pytest.mark.parametrize("nd,Ns", [(2, [64, 66, 130, 256, 64]), (2, [8]), (2, [8, 66, 4, 2048]),
+                                   (3, [64, 34, 96])])
 def test_string_counts_ensemble(nd, Ns):
     """axs_string_counts over an ensemble of uneven sides, dense (random
     phases) and smooth members: flagged sites and, in 2D, the core count
