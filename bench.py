@@ -299,8 +299,9 @@ WORKFLOWS = {
 
 def workflow_leg(np, axs, T0, cpu_rate_3d=None):
     """Wall time of each configuration run on the GPU (steps plus the
-    in-loop sampling, as the driver makes the calls), and of config 0 on
-    the CPU oracle; config 1 on the CPU is estimated from the measured
+    in-loop sampling, as the driver makes the calls): configs 0 and 1 in
+    the FP64 form, config 3 in the FP32 form; config 0 also on the CPU
+    oracle, configs 1 and 3 on the CPU estimated from the measured
     reference-structured step rate (steps only, no sampling)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -343,6 +344,34 @@ def workflow_leg(np, axs, T0, cpu_rate_3d=None):
         out[name] = rec
         log(f"[bench] {name}: GPU {gpu_s:.3f} s" +
             (f", CPU {rec['cpu_s']:.2f} s" if "cpu_s" in rec else ""))
+    # config 3 as a whole run in the FP32 form: 3D 1024^3, 1000 steps, the
+    # string count and the 64-bin log spectrum every 50 steps, from the
+    # device k^-1 field (the host generator would take minutes at 1024^3);
+    # the sampling path's one-time set-up (FFT plans) runs before the clock
+    N, steps, every = 1024, 1000, 50
+    lat = axs.Lattice(3, N, precision="fp32", T_initial=T0)
+    lat.init("kinv", seed=1998)
+    lat.string_counts()
+    lat.powerspec(binning="log", nbins=64)
+    lat.init("kinv", seed=1998)
+    lat.tau = 1.0
+    t0 = time.perf_counter()
+    nstr = 0
+    for done in range(0, steps, every):
+        nstr = int(lat.string_counts()[0][0])
+        lat.powerspec(binning="log", nbins=64)
+        lat.step(every)
+    gpu_s = time.perf_counter() - t0
+    lat.close()
+    rec = {"ndims": 3, "N": N, "steps": steps, "sample_every": every, "precision": "fp32",
+           "samples": "string count + 64-bin log power spectrum", "gpu_s": gpu_s,
+           "gpu_site_updates_per_s": N ** 3 * steps / gpu_s, "last_string_count": nstr}
+    if cpu_rate_3d:
+        rec["cpu_s"] = N ** 3 * steps / cpu_rate_3d
+        rec["cpu_kind"] = "oracle FP64 step rate at 256^3 x steps (sampling not included)"
+        rec["speedup"] = rec["cpu_s"] / gpu_s
+    out["config3_3d_1024_fp32"] = rec
+    log(f"[bench] config3_3d_1024_fp32: GPU {gpu_s:.3f} s")
     return out
 
 
