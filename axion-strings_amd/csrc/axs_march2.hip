@@ -39,28 +39,39 @@ int multi_waves(int flags)
 
 int stencil_radius_of(const Context *c) { return c->p.stencil_setting + 1; }
 
-// Longest chunk (FP32 up to 256 planes, FP64 up to 64, at most half the
-// smallest side: fewer halo planes per chunk) that still deals the chip
-// enough work items over all members: three per four CUs for FP32, four
-// per CU for FP64.  Measured: FP32 256 at 512^3 and 1024^3 (7% faster than
-// 64 at 1024^3), 64 at 256^3 and for the 48-lattice ensemble, 32 at 192^3,
-// 8 at 128^3; FP64 64 at 512^3 and up and for the ensemble (128 slower),
-// 32 at 256^3, 8 at 128^3.
+// Chunk of planes per work item (FP32 up to 256, FP64 up to 64, at most
+// half the smallest side) by a cost model: one work-group per CU, every item
+// marches chunk + 2H planes (H = halo), so a pass takes about
+// ceil(items / CUs) * (chunk + 2H) plane steps of one work-group; the
+// cheapest cap wins (ties: the longer chunk).  It reproduces the measured
+// best chunk at every side swept (FP32 depth 4: 256 at 1024^3, 64 at 384^3
+// and 256^3, 32 at 192^3, 8 at 128^3 (depth 3); FP64 depth 3: 64 at 1024^3,
+// 320^3 and 192^3, 32 at 256^3 and 128^3; 64 for both forms' 48-lattice
+// ensemble), where the former rule (fill the chip with at least 3/4
+// (FP32) or 4 (FP64) items per CU) took 8 at FP64 192^3 (1.6x slower)
+// and 128 at FP32 384^3 (1.2x slower).
 int multi_chunk_cap(const Context *c, int TJ, int TK)
 {
     const bool f32 = c->prec == AXS_FP32;
-    const uint64_t target = f32 ? (uint64_t)c->num_cu * 3 / 4 : (uint64_t)c->num_cu * 4;
+    const int H = (kTBW - TK) / 2;
+    const uint64_t cus = (uint64_t)(c->num_cu > 0 ? c->num_cu : 1);
     int min_side = 1 << 30;
     for (const Member &m : c->members) min_side = m.N < min_side ? m.N : min_side;
-    for (int cap = f32 ? 256 : 64; cap > 8; cap >>= 1) {
-        if (2 * cap > min_side) continue;
+    int best = 8;
+    uint64_t best_cost = ~0ull;
+    for (int cap = f32 ? 256 : 64; cap >= 8; cap >>= 1) {
+        if (cap > 8 && 2 * cap > min_side) continue;
         uint64_t items = 0;
         for (const Member &m : c->members)
             items += (uint64_t)((m.N + TJ - 1) / TJ) * (uint64_t)((m.N + TK - 1) / TK) *
                      (uint64_t)(m.N / divisor_below(m.N, cap));
-        if (items >= target) return cap;
+        const uint64_t cost = ((items + cus - 1) / cus) * (uint64_t)(cap + 2 * H);
+        if (cost < best_cost) {
+            best_cost = cost;
+            best = cap;
+        }
     }
-    return 8;
+    return best;
 }
 
 // waves per work-group: the configured size; the higher orders, built in
