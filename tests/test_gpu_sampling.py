@@ -129,14 +129,16 @@ def test_strings_3d_straight_lines(axis, N):
     np.testing.assert_array_equal(c, c_ref)
 
 
-@pytest.mark.parametrize("nd,Ns", [(2, [64, 66, 130, 256, 64]), (2, [8]), (2, [8, 66, 4, 2048]),
-                                   (3, [64, 34, 96])])
-def test_string_counts_ensemble(nd, Ns):
+@pytest.mark.parametrize("nd,Ns,prec", [(2, [64, 66, 130, 256, 64], "fp64"), (2, [8], "fp64"),
+                                        (2, [8, 66, 4, 2048], "fp64"), (2, [130, 64, 96], "fp32"),
+                                        (3, [64, 34, 96], "fp64"), (3, [66, 64], "fp32")])
+def test_string_counts_ensemble(nd, Ns, prec):
     """axs_string_counts over an ensemble of uneven sides, dense (random
     phases) and smooth members: flagged sites and, in 2D, the core count
     n - (list-adjacent pairs) taken from the ballot words, against the
-    oracle's list of each member."""
-    lat = axs.Lattice(nd, members=Ns)
+    oracle's list of each member (on the member's own, possibly
+    float-rounded, state)."""
+    lat = axs.Lattice(nd, members=Ns, precision=prec)
     fields = []
     for r, N in enumerate(Ns):
         if r % 2 == 0:
@@ -146,7 +148,8 @@ def test_string_counts_ensemble(nd, Ns):
             p1, p2 = kinv_field(nd, N, seed=200 + r)
         z = np.zeros(N ** nd)
         lat.set_state(p1, p2, z, z, member=r)
-        fields.append((p1, p2))
+        g = lat.get_state(member=r, fields=("phi1", "phi2"))
+        fields.append((g["phi1"], g["phi2"]))
     npl, ncores = lat.string_counts()
     for r, (N, (p1, p2)) in enumerate(zip(Ns, fields)):
         n_ref, c_ref = O.cores(O.axion(p1, p2), nd, N, thr=1)
