@@ -3,10 +3,12 @@
 //   energy components (kinetic, gradient, potential).
 //
 // Each work-item keeps Neumaier (sum, compensation) pairs over a fixed
-// strided subset of sites; waves combine pairs with cross-lane shuffles, the
-// block writes one pair per quantity, and a single-block pass folds the
-// block pairs in block order.  The partition depends only on the lattice
-// size, so results are identical from run to run.
+// subset of sites (background: a strided set; energy: columns along the
+// slowest axis, marched so that the slow-axis neighbour stays in
+// registers); waves combine pairs with cross-lane shuffles, the block
+// writes one pair per quantity, and a single-block pass folds the block
+// pairs in block order.  The partition depends only on the lattice size,
+// so results are identical from run to run.
 #include <algorithm>
 
 #include "axs_internal.hpp"

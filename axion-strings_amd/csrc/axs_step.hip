@@ -18,6 +18,8 @@
 // FP32 form: phi and u = phidot + dt/2 ker are stored in FP32; one step is
 //   phi' = phi + dt u;  ker' = F(phi', tau');  u' = u + dt ker'
 // which is the same velocity-Verlet map written in half-step velocities.
+// The FP32 form's read-back of phidot and ker (canonical_fp32_kernel) is
+// here too.
 #include <algorithm>
 
 #include "axs_internal.hpp"
