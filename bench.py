@@ -455,10 +455,20 @@ def cpu_baseline(np, seconds: float):
     s5 = O.State(rng.standard_normal(N5 ** 3), rng.standard_normal(N5 ** 3), z5, z5)
     f512, _, _ = timed(lambda k: s5.run_fused(O.params(3, N5), k), 0.2 * seconds, N5 ** 3)
     del s5
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {
         "value": rate,
         "unit": "site-updates/s",
         "cores": threads,
+        "host": {"cpu_model": model, "logical_cpus": os.cpu_count()},
         "kind": "port",
         "sample": f"3D {Ns}^3 FP64, {n} reference-structured steps (drift, 2x CSR SpMV "
                   f"Laplacian, potential, kick, copy) of the CPU oracle, {t:.1f} s, "
