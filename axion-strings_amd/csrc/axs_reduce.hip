@@ -216,7 +216,9 @@ int run_energy(Context *c, Member &m, double out[3])
     // lane of four work-groups per CU
     const uint64_t want = (uint64_t)c->num_cu * 4 * kRedBlock;
     const uint32_t nch = (uint32_t)std::min<uint64_t>((uint64_t)N, (want + plane - 1) / plane);
-    const int CI = (int)((N + nch - 1) / nch);
+    // at most 64 planes per chunk: work-items of one round march in step,
+    // so the row j + 1 they share is still in L2
+    const int CI = std::min(64, (int)((N + nch - 1) / nch));
     const uint32_t ncols = plane * (uint32_t)((N + CI - 1) / CI);
     int nb = red_blocks(ncols);
     size_t part_b = (size_t)nb * 3 * sizeof(Kahan);

@@ -466,7 +466,9 @@ int launch_fp32_canonical(Context *c, Member &m, const StepScalars &s, bool reco
     // lane of four work-groups per CU
     const uint64_t want = (uint64_t)c->num_cu * 4 * 256;
     const uint32_t nch = (uint32_t)std::min<uint64_t>((uint64_t)N, (want + plane - 1) / plane);
-    const int CI = (int)((N + nch - 1) / nch);
+    // at most 64 planes per chunk: work-groups of one round then march in
+    // step, so the rows j +- l they share are still in L2
+    const int CI = std::min(64, (int)((N + nch - 1) / nch));
     const uint32_t ncols = plane * (uint32_t)((N + CI - 1) / CI);
     const unsigned grid = grid_for(ncols, 256);
     const fvec4 *in = (const fvec4 *)m.state[c->cur][PHI];
