@@ -269,7 +269,11 @@ strings_march_kernel(SfGeom g, P phi, double accept, unsigned long long *__restr
     __shared__ float A[2][kSfTJ + 1][65];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int N = g.N;
+    // XCD-aware order (work-groups go round-robin to the 8 XCDs): each XCD
+    // takes a contiguous run of tiles, k fastest, so the 65th column and the
+    // 31st row a tile reads from its neighbours are read on the same L2
     uint32_t b = blockIdx.x;
+    if ((gridDim.x & 7u) == 0) b = (b & 7u) * (gridDim.x >> 3) + (b >> 3);
     const int tk = (int)(b % (uint32_t)g.nTk);
     b /= (uint32_t)g.nTk;
     const int tj = (int)(b % (uint32_t)g.nTj);
