@@ -156,8 +156,8 @@ int axs_strings(axs_ctx *ctx, int32_t *coords, int64_t cap, int64_t *count);
 /* Flagged-site count and, in 2D, num_cores = n - (list-adjacent pairs)
  * (s_run.cpp:99-109); in 3D num_cores == n_plaquettes.  One value per
  * member, arrays of length n_members, from one synchronisation for the
- * whole ensemble.  The search of the last member (of a single lattice:
- * its only one) stays cached while its state is unchanged, so an
+ * whole ensemble.  For a single lattice (and the last member of a 3D
+ * ensemble) the search stays cached while the state is unchanged, so an
  * axs_member_strings / axs_strings call for it next only writes the list. */
 int axs_string_counts(axs_ctx *ctx, int64_t *n_plaquettes, int64_t *num_cores);
 
