@@ -70,39 +70,23 @@ angle_kernel(uint32_t L, P phi, double *__restrict__ ang)
     }
 }
 
-template <int ND>
-__device__ __forceinline__ bool plaquette_flag(const double *__restrict__ ang, uint32_t m, int N,
-                                               double accept)
+// 2D plaquette of site m (string_finding.cpp:17-20): corners (i,j),
+// (i+1,j), (i+1,j+1), (i,j+1), three edges
+__device__ __forceinline__ bool plaquette_flag2(const double *__restrict__ ang, uint32_t m, int N,
+                                                double accept)
 {
-    Site<ND> s = decompose<ND>(m, N);
-    if (ND == 2) {
-        // corners (i,j), (i+1,j), (i+1,j+1), (i,j+1): string_finding.cpp:17-20
-        uint32_t q2 = shifted<ND>(s, 0, 1, N);
-        Site<ND> s2 = decompose<ND>(q2, N);
-        uint32_t q3 = shifted<ND>(s2, 1, 1, N);
-        uint32_t q4 = shifted<ND>(s, 1, 1, N);
-        double n1 = ang[m];
-        double n2 = ang[q2];
-        double n3 = ang[q3];
-        double n4 = ang[q4];
-        double sum = edge_theta(n1, n2) + edge_theta(n2, n3) + edge_theta(n3, n4);
-        return sum > accept;
-    } else {
-        // string_finding.cpp:49-62; plaquette b uses corner (i, j+1, k+1)
-        uint32_t qi = shifted<ND>(s, 0, 1, N);                     // (i+1, j,   k)
-        Site<ND> si = decompose<ND>(qi, N);
-        uint32_t qij = shifted<ND>(si, 1, 1, N);                   // (i+1, j+1, k)
-        uint32_t qj = shifted<ND>(s, 1, 1, N);                     // (i,   j+1, k)
-        Site<ND> sj = decompose<ND>(qj, N);
-        uint32_t qjk = shifted<ND>(sj, 2, 1, N);                   // (i,   j+1, k+1)
-        uint32_t qk = shifted<ND>(s, 2, 1, N);                     // (i,   j,   k+1)
-        return plaquettes3(ang[m], ang[qi], ang[qij], ang[qj], ang[qjk], ang[qk], accept);
-    }
+    const Site<2> s = decompose<2>(m, N);
+    const uint32_t q2 = shifted<2>(s, 0, 1, N);
+    const Site<2> s2 = decompose<2>(q2, N);
+    const uint32_t q3 = shifted<2>(s2, 1, 1, N);
+    const uint32_t q4 = shifted<2>(s, 1, 1, N);
+    const double n1 = ang[m], n2 = ang[q2], n3 = ang[q3], n4 = ang[q4];
+    const double sum = edge_theta(n1, n2) + edge_theta(n2, n3) + edge_theta(n3, n4);
+    return sum > accept;
 }
 
-// plaquette test of block `blk` (kStrSpan sites) of one lattice from its
-// angles: one ballot word per wave and pass, the block's count in bcount
-template <int ND>
+// 2D plaquette test of block `blk` (kStrSpan sites) of one lattice from
+// its angles: one ballot word per wave and pass, the block's count in bcount
 __device__ __forceinline__ void flag_block(int N, uint32_t L, const double *__restrict__ ang,
                                            double accept, uint64_t *__restrict__ words,
                                            uint32_t *__restrict__ bcount, uint32_t blk)
@@ -113,7 +97,7 @@ __device__ __forceinline__ void flag_block(int N, uint32_t L, const double *__re
     uint32_t base = blk * (uint32_t)kStrSpan;
     for (int it = 0; it < kStrIters; ++it) {
         uint32_t m = base + it * kStrBlock + threadIdx.x;
-        bool f = (m < L) && plaquette_flag<ND>(ang, m, N, accept);
+        bool f = (m < L) && plaquette_flag2(ang, m, N, accept);
         uint64_t w = __ballot(f);
         uint32_t wi = (base + it * kStrBlock) / 64 + wave;
         if (lane == 0 && (uint64_t)wi * 64 < L) words[wi] = w;
@@ -128,12 +112,11 @@ __device__ __forceinline__ void flag_block(int N, uint32_t L, const double *__re
     }
 }
 
-template <int ND>
 __global__ void __launch_bounds__(kStrBlock)
 strings_flag_kernel(int N, uint32_t L, const double *__restrict__ ang, double accept,
                     uint64_t *__restrict__ words, uint32_t *__restrict__ bcount)
 {
-    flag_block<ND>(N, L, ang, accept, words, bcount, blockIdx.x);
+    flag_block(N, L, ang, accept, words, bcount, blockIdx.x);
 }
 
 // Ensemble-wide 2D finder: member r owns blocks [first_block, first_block +
@@ -173,7 +156,7 @@ strings_flag_batch_kernel(const StrMember *__restrict__ mem, const uint32_t *__r
 {
     const StrMember M = mem[blk2mem[blockIdx.x]];
     const size_t wbase = (size_t)M.first_block * (kStrSpan / 64);
-    flag_block<2>(M.N, M.L, ang + M.site_base, accept, words + wbase, bcount + M.first_block,
+    flag_block(M.N, M.L, ang + M.site_base, accept, words + wbase, bcount + M.first_block,
                   blockIdx.x - M.first_block);
 }
 
@@ -696,7 +679,7 @@ static int search_member(Context *c, Member &m, uint64_t *words, uint32_t *bcoun
     else
         hipLaunchKernelGGL(angle_kernel<PhiF32>, dim3(ga), dim3(256), 0, c->stream, L,
                            PhiF32{(const fvec4 *)phi}, ang);
-    hipLaunchKernelGGL(strings_flag_kernel<2>, dim3(nblocks), dim3(kStrBlock), 0, c->stream,
+    hipLaunchKernelGGL(strings_flag_kernel, dim3(nblocks), dim3(kStrBlock), 0, c->stream,
                        m.N, L, ang, accept, words, bcount);
     AXS_HIP_TRY(hipGetLastError());
     return AXS_OK;
