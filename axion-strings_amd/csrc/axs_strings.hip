@@ -157,7 +157,7 @@ strings_flag_batch_kernel(const StrMember *__restrict__ mem, const uint32_t *__r
     const StrMember M = mem[blk2mem[blockIdx.x]];
     const size_t wbase = (size_t)M.first_block * (kStrSpan / 64);
     flag_block(M.N, M.L, ang + M.site_base, accept, words + wbase, bcount + M.first_block,
-                  blockIdx.x - M.first_block);
+               blockIdx.x - M.first_block);
 }
 
 // 3D finder in one pass over phi.  A work-group of kSfW waves owns a tile
@@ -181,9 +181,10 @@ strings_flag_batch_kernel(const StrMember *__restrict__ mem, const uint32_t *__r
 // is not; a site whose plaquettes are all undecided that way (or NaN), or
 // that touches an input outside [1e-30, 1e30] in magnitude (where the
 // float angle could be off: zeros, underflow, overflow, NaN), is decided
-// by the FP64 angles of its six corners with the two-pass finder's
-// expressions.  The flags are therefore those of the FP64 test, bit for
-// bit; the FP64 path runs for a handful of sites per lattice.
+// by the FP64 angles of its six corners (scaled_angle, plaquettes3: the
+// expressions of string_finding.cpp).  The flags are therefore those of
+// the FP64 test, bit for bit; the FP64 path runs for a handful of sites
+// per lattice.
 constexpr int kSfW = 8;
 constexpr int kSfTJ = 30;
 constexpr float kSfMargin = 1.0f / 16384.0f;
