@@ -90,6 +90,12 @@ struct Member {
     bool ker_valid = true;
     bool vel_valid = false;   // FP32: aux_vel holds phidot of the current state
     uint64_t version = 0;     // bumped whenever the state changes
+    // FP32: background sums and energy components of state `samp_version`
+    // at tau `samp_tau` (one pass computes both)
+    bool samp_valid = false;
+    uint64_t samp_version = 0;
+    double samp_tau = 0.0;
+    double samp[9] = {0.0};
 };
 
 struct FftPlan;

@@ -1,6 +1,9 @@
 // axs_reduce.hip -- whole-lattice reductions in compensated FP64:
 //   background sums of s_run.cpp:135-150 (six quantities);
 //   energy components (kinetic, gradient, potential).
+// The FP64 form sums its stored fields (background_kernel, energy_kernel);
+// the FP32 form gets both sets from one pass over its (phi, u) state that
+// rebuilds phidot per site (sample_fp32_kernel), kept per state and tau.
 //
 // Each work-item keeps Neumaier (sum, compensation) pairs over a fixed
 // subset of sites (background: a strided set; energy: columns along the
@@ -165,6 +168,170 @@ energy_kernel(int N, int CI, uint32_t ncols, P phi, V vel, double tau2, double T
         for (int q = 0; q < 3; ++q) part[blockIdx.x * 3 + q] = k[q];
 }
 
+// FP32 form: the six background sums and the three energy components in one
+// pass over the stored (phi, u) state, nothing written but the block sums.
+// phidot = u - dt/2 ker is rebuilt per site with canonical_fp32_kernel's
+// float expressions -- ker recomputed from phi with the step kernels'
+// neighbour order, or read from kin while that holds the current state's
+// -- so the sums run over the fields a read-back gives.  A work-item
+// marches columns along the slowest axis (as energy_kernel), keeping phi of
+// planes sl-R .. sl+R in a register ring and loading the next plane's
+// operands while the current one is summed.
+constexpr int kSampQ = 9;     // background (6), then kinetic, gradient, potential
+
+template <int ND, int R>
+__global__ void __launch_bounds__(kRedBlock)
+sample_fp32_kernel(int N, int CI, uint32_t ncols, const fvec4 *__restrict__ in,
+                   const fvec2 *__restrict__ kin, StepScalarsF s, int recompute, double tau2,
+                   double T03, double lam_4tau2, Kahan *__restrict__ part)
+{
+    __shared__ Kahan sh[kSampQ][kRedBlock / 64];
+    Kahan k[kSampQ];
+#pragma unroll
+    for (int q = 0; q < kSampQ; ++q) k[q] = Kahan{0.0, 0.0};
+    const uint32_t plane = (ND == 3) ? (uint32_t)N * (uint32_t)N : (uint32_t)N;
+    const uint32_t stride = gridDim.x * kRedBlock;
+    uint32_t blk = blockIdx.x;
+    if ((gridDim.x & 7u) == 0) blk = (blk & 7u) * (gridDim.x >> 3) + (blk >> 3);
+    auto wrap = [N](int x) {               // periodic index, |x| within a few N
+        while (x < 0) x += N;
+        while (x >= N) x -= N;
+        return x;
+    };
+    // phi (x, y) of a site as one 8-byte load from the (phi, u) array
+    const fvec2 *__restrict__ in2 = reinterpret_cast<const fvec2 *>(in);
+    auto xy = [&](uint32_t m) { return in2[2 * (size_t)m]; };
+    struct Plane {
+        fvec4 c;
+        fvec2 a[2 * R], b[2 * R];
+    };
+    for (uint32_t col = blk * kRedBlock + threadIdx.x; col < ncols; col += stride) {
+        const uint32_t q = col % plane;
+        const int s0 = (int)(col / plane) * CI, s1 = (s0 + CI < N) ? s0 + CI : N;
+        // in-plane neighbours at distance l (+l, -l): 3D axis 1 (j) and
+        // axis 2 (k); 2D axis 0 (i).  Index 0 is the forward neighbour.
+        uint32_t ap[R], am[R], bp[R], bm[R];
+        if (ND == 3) {
+            const int j = (int)(q / (uint32_t)N), kk = (int)(q - (uint32_t)j * (uint32_t)N);
+#pragma unroll
+            for (int l = 1; l <= R; ++l) {
+                ap[l - 1] = (uint32_t)wrap(j + l) * (uint32_t)N + (uint32_t)kk;
+                am[l - 1] = (uint32_t)wrap(j - l) * (uint32_t)N + (uint32_t)kk;
+                bp[l - 1] = (uint32_t)j * (uint32_t)N + (uint32_t)wrap(kk + l);
+                bm[l - 1] = (uint32_t)j * (uint32_t)N + (uint32_t)wrap(kk - l);
+            }
+        } else {
+#pragma unroll
+            for (int l = 1; l <= R; ++l) {
+                ap[l - 1] = (uint32_t)wrap((int)q + l);
+                am[l - 1] = (uint32_t)wrap((int)q - l);
+                bp[l - 1] = bm[l - 1] = 0;
+            }
+        }
+        auto load = [&](int sl, Plane &P) {
+            const uint32_t base = (uint32_t)sl * plane;
+            P.c = in[base + q];
+#pragma unroll
+            for (int l = 0; l < R; ++l) {
+                P.a[2 * l] = xy(base + ap[l]);
+                P.a[2 * l + 1] = xy(base + am[l]);
+                if (ND == 3) {
+                    P.b[2 * l] = xy(base + bp[l]);
+                    P.b[2 * l + 1] = xy(base + bm[l]);
+                }
+            }
+        };
+        fvec2 ring[2 * R + 1];             // phi of slow index sl - R + t
+#pragma unroll
+        for (int t = 0; t < 2 * R + 1; ++t) ring[t] = xy((uint32_t)wrap(s0 - R + t) * plane + q);
+        Plane cur, nxt;
+        load(s0, cur);
+        for (int sl = s0; sl < s1; ++sl) {
+            const uint32_t base = (uint32_t)sl * plane;
+            fvec2 ring_next = ring[2 * R];
+            if (sl + 1 < s1) {
+                load(sl + 1, nxt);
+                ring_next = xy((uint32_t)wrap(sl + 1 + R) * plane + q);
+            }
+            const fvec4 c = cur.c;
+            fvec2 kf;
+            if (recompute) {
+                float s1x = -0.0f, s2x = -0.0f;
+#pragma unroll
+                for (int l = 1; l <= R; ++l) {
+                    const float cl = s.c[l];
+                    const fvec2 p1 = cur.a[2 * l - 2], q1 = cur.a[2 * l - 1];
+                    if (ND == 3) {
+                        s1x = F32Math::acc(s1x, cl, ring[R + l].x);
+                        s2x = F32Math::acc(s2x, cl, ring[R + l].y);
+                        s1x = F32Math::acc(s1x, cl, ring[R - l].x);
+                        s2x = F32Math::acc(s2x, cl, ring[R - l].y);
+                        s1x = F32Math::acc(s1x, cl, p1.x);
+                        s2x = F32Math::acc(s2x, cl, p1.y);
+                        s1x = F32Math::acc(s1x, cl, q1.x);
+                        s2x = F32Math::acc(s2x, cl, q1.y);
+                        const fvec2 p2 = cur.b[2 * l - 2], q2 = cur.b[2 * l - 1];
+                        s1x = F32Math::acc(s1x, cl, p2.x);
+                        s2x = F32Math::acc(s2x, cl, p2.y);
+                        s1x = F32Math::acc(s1x, cl, q2.x);
+                        s2x = F32Math::acc(s2x, cl, q2.y);
+                    } else {
+                        s1x = F32Math::acc(s1x, cl, p1.x);
+                        s2x = F32Math::acc(s2x, cl, p1.y);
+                        s1x = F32Math::acc(s1x, cl, q1.x);
+                        s2x = F32Math::acc(s2x, cl, q1.y);
+                        s1x = F32Math::acc(s1x, cl, ring[R + l].x);
+                        s2x = F32Math::acc(s2x, cl, ring[R + l].y);
+                        s1x = F32Math::acc(s1x, cl, ring[R - l].x);
+                        s2x = F32Math::acc(s2x, cl, ring[R - l].y);
+                    }
+                }
+                const float t = F32Math::own(s.cl, F32Math::pot(c.x, c.y, s.pot0), s.ac0);
+                kf.x = F32Math::force(s.alpha, s1x, c.x, t);
+                kf.y = F32Math::force(s.alpha, s2x, c.y, t);
+            } else {
+                kf = kin[base + q];
+            }
+            const float vx = c.z - s.half_dt * kf.x, vy = c.w - s.half_dt * kf.y;
+            const double a = c.x, b = c.y, va = vx, vb = vy;
+            kadd(k[0], a);                               // s_run.cpp:141-146
+            kadd(k[1], b);
+            kadd(k[2], va);
+            kadd(k[3], vb);
+            kadd(k[4], fabs(atan2(a, b)));
+            kadd(k[5], sqrt(a * a + b * b));
+            // forward differences in energy_kernel's axis order
+            const fvec2 fs = ring[R + 1], fa = cur.a[0];
+            double g = 0.0, d1, d2;
+            if (ND == 3) {
+                const fvec2 fb = cur.b[0];
+                d1 = (double)fs.x - a, d2 = (double)fs.y - b;
+                g += d1 * d1 + d2 * d2;
+                d1 = (double)fa.x - a, d2 = (double)fa.y - b;
+                g += d1 * d1 + d2 * d2;
+                d1 = (double)fb.x - a, d2 = (double)fb.y - b;
+                g += d1 * d1 + d2 * d2;
+            } else {
+                d1 = (double)fa.x - a, d2 = (double)fa.y - b;
+                g += d1 * d1 + d2 * d2;
+                d1 = (double)fs.x - a, d2 = (double)fs.y - b;
+                g += d1 * d1 + d2 * d2;
+            }
+            const double S = a * a + b * b - tau2 + T03;
+            kadd(k[6], 0.5 * (va * va + vb * vb));
+            kadd(k[7], 0.5 * g);
+            kadd(k[8], lam_4tau2 * S * S);
+#pragma unroll
+            for (int t2 = 0; t2 < 2 * R; ++t2) ring[t2] = ring[t2 + 1];
+            ring[2 * R] = ring_next;
+            cur = nxt;
+        }
+    }
+    block_reduce<kSampQ>(k, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < kSampQ; ++q) part[blockIdx.x * kSampQ + q] = k[q];
+}
+
 __global__ void fold_kernel(const Kahan *__restrict__ part, int nblocks, int Q,
                             double *__restrict__ out)
 {
@@ -181,8 +348,79 @@ static int red_blocks(uint32_t L)
     return (int)(need < (uint32_t)kRedMaxBlocks ? need : (uint32_t)kRedMaxBlocks);
 }
 
+// FP32 form: the nine sums of sample_fp32_kernel for the member's current
+// state, computed once per (state, tau) and kept for the other call
+template <int ND>
+static auto sample_kernel_for(int R)
+{
+    return (R == 1)   ? sample_fp32_kernel<ND, 1>
+           : (R == 2) ? sample_fp32_kernel<ND, 2>
+           : (R == 3) ? sample_fp32_kernel<ND, 3>
+                      : sample_fp32_kernel<ND, 4>;
+}
+
+static int fp32_sums(Context *c, Member &m, double out[kSampQ])
+{
+    if (!m.has_state) {
+        set_error("member has no state: call axs_set_member_state first");
+        return AXS_ERR_STATE;
+    }
+    if (m.samp_valid && m.samp_version == m.version && m.samp_tau == c->tau) {
+        std::copy(m.samp, m.samp + kSampQ, out);
+        return AXS_OK;
+    }
+    const StepScalarsF f = to_float(make_scalars(c->p, c->tau));
+    const int N = m.N, R = f.ncoef - 1;
+    if (R < 1 || R > 4) {
+        set_error("stencil radius %d is not supported", R);
+        return AXS_ERR_UNSUPPORTED;
+    }
+    const uint32_t plane = (c->nd == 3) ? (uint32_t)N * (uint32_t)N : (uint32_t)N;
+    // slow-axis chunks as in run_energy
+    const uint64_t want = (uint64_t)c->num_cu * 4 * kRedBlock;
+    const uint32_t nch = (uint32_t)std::min<uint64_t>((uint64_t)N, (want + plane - 1) / plane);
+    const int CI = std::min(64, (int)((N + nch - 1) / nch));
+    const uint32_t ncols = plane * (uint32_t)((N + CI - 1) / CI);
+    const int nb = red_blocks(ncols);
+    const size_t part_b = (size_t)nb * kSampQ * sizeof(Kahan);
+    AXS_TRY(ensure_scratch(c, part_b + kSampQ * sizeof(double)));
+    Kahan *part = (Kahan *)c->scratch;
+    double *dout = (double *)((char *)c->scratch + part_b);
+    const double lam = (double)c->p.lambda, T0 = (double)c->p.T_initial;
+    const double tau2 = c->tau * c->tau, T03 = (T0 * T0) / 3.0, lam_4tau2 = lam / (4.0 * tau2);
+    auto k = (c->nd == 2) ? sample_kernel_for<2>(R) : sample_kernel_for<3>(R);
+    hipLaunchKernelGGL(k, dim3(nb), dim3(kRedBlock), 0, c->stream, N, CI, ncols,
+                       (const fvec4 *)m.state[c->cur][PHI], (const fvec2 *)m.aux_ker, f,
+                       m.ker_valid ? 0 : 1, tau2, T03, lam_4tau2, part);
+    hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(64), 0, c->stream, part, nb, kSampQ, dout);
+    AXS_HIP_TRY(hipGetLastError());
+    AXS_HIP_TRY(hipMemcpyAsync(m.samp, dout, kSampQ * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+    AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+    m.samp_valid = true;
+    m.samp_version = m.version;
+    m.samp_tau = c->tau;
+    std::copy(m.samp, m.samp + kSampQ, out);
+    return AXS_OK;
+}
+
+static void energy_scale(const Context *c, const double raw[3], double out[3])
+{
+    const double dx = (double)c->p.space_step;
+    const double vol = (c->nd == 3) ? dx * dx * dx : dx * dx;
+    out[0] = raw[0] * vol;
+    out[1] = raw[1] / (dx * dx) * vol;
+    out[2] = raw[2] * vol;
+}
+
 int run_background(Context *c, Member &m, double out[6])
 {
+    if (c->prec == AXS_FP32) {
+        double sums[kSampQ];
+        AXS_TRY(fp32_sums(c, m, sums));
+        std::copy(sums, sums + 6, out);
+        return AXS_OK;
+    }
     FieldView fv;
     AXS_TRY(current_fields(c, m, &fv));
     uint32_t L = (uint32_t)m.L;
@@ -208,6 +446,12 @@ int run_background(Context *c, Member &m, double out[6])
 
 int run_energy(Context *c, Member &m, double out[3])
 {
+    if (c->prec == AXS_FP32) {
+        double sums[kSampQ];
+        AXS_TRY(fp32_sums(c, m, sums));
+        energy_scale(c, sums + 6, out);
+        return AXS_OK;
+    }
     FieldView fv;
     AXS_TRY(current_fields(c, m, &fv));
     const int N = m.N;
@@ -245,11 +489,7 @@ int run_energy(Context *c, Member &m, double out[3])
     double raw[3];
     AXS_HIP_TRY(hipMemcpyAsync(raw, dout, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     AXS_HIP_TRY(hipStreamSynchronize(c->stream));
-    double dx = (double)c->p.space_step;
-    double vol = (c->nd == 3) ? dx * dx * dx : dx * dx;
-    out[0] = raw[0] * vol;
-    out[1] = raw[1] / (dx * dx) * vol;
-    out[2] = raw[2] * vol;
+    energy_scale(c, raw, out);
     return AXS_OK;
 }
 

@@ -170,7 +170,10 @@ int axs_member_background(axs_ctx *ctx, int member, double sums[6]);
 int axs_background(axs_ctx *ctx, double sums[6]);
 
 /* Energy components (kinetic, gradient, potential) times the cell volume;
- * no reference counterpart.  Compensated FP64.                            */
+ * no reference counterpart.  Compensated FP64.
+ * FP32 form: one pass computes the background sums and the energy of a
+ * member's state together; the second of the two calls on the same state
+ * and tau returns the kept result without touching the device.           */
 int axs_member_energy(axs_ctx *ctx, int member, double out[3]);
 int axs_energy(axs_ctx *ctx, double out[3]);
 

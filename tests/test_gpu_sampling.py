@@ -269,6 +269,60 @@ def test_energy(nd, N, nsteps, prec):
     np.testing.assert_allclose(got, ref, rtol=tol)
 
 
+@pytest.mark.parametrize("nd,N,stencil", [(3, 48, 0), (3, 40, 2), (3, 34, 3), (2, 128, 0),
+                                          (2, 96, 1), (2, 66, 3)])
+def test_fp32_sums_before_read_back(nd, N, stencil):
+    """FP32 form: background and energy taken straight after steps rebuild
+    phidot inside the summing pass (ker recomputed from phi); taken after a
+    read-back they read ker from the read-back buffer.  Both must be bit for
+    bit the same, match the oracle's sums over the read-back fields, and a
+    second call (served from the per-state cache) must return them again."""
+    def evolved():
+        lat = axs.Lattice(nd, N, precision="fp32", stencil_setting=stencil)
+        p1, p2, v1, v2, _, _ = random_state(nd, N, seed=5)
+        lat.set_state(p1, p2, v1, v2, tau=1.0)
+        lat.step(7)
+        return lat
+    a = evolved()
+    bg_a, e_a = a.background(), a.energy()
+    np.testing.assert_array_equal(a.background(), bg_a)
+    np.testing.assert_array_equal(a.energy(), e_a)
+    b = evolved()
+    g = b.get_state()
+    np.testing.assert_array_equal(b.background(), bg_a)
+    np.testing.assert_array_equal(b.energy(), e_a)
+    ref = O.background(g["phi1"], g["phi2"], g["phidot1"], g["phidot2"])
+    scale = np.array([np.abs(g[k]).sum() for k in ("phi1", "phi2", "phidot1", "phidot2")] +
+                     [ref[4], ref[5]])
+    assert np.all(np.abs(bg_a - ref) <= 1e-12 * scale + 1e-300), (bg_a, ref)
+    e_ref = O.energy(O.params(nd, N, stencil_setting=stencil), a.tau, g["phi1"], g["phi2"],
+                     g["phidot1"], g["phidot2"])
+    np.testing.assert_allclose(e_a, e_ref, rtol=1e-11)
+
+
+def test_fp32_sums_follow_tau_and_state():
+    # the cached sums are keyed by state and tau: a new tau (set_tau) or new
+    # state (set_state with an uploaded ker) gives the oracle's new values
+    nd, N = 3, 32
+    lat = axs.Lattice(nd, N, precision="fp32")
+    p1, p2, v1, v2, k1, k2 = random_state(nd, N, seed=8, ker=True)
+    lat.set_state(p1, p2, v1, v2, tau=1.0)
+    lat.step(3)
+    e0 = lat.energy()
+    lat.tau = 2.5
+    e1 = lat.energy()
+    g = lat.get_state()
+    ref = O.energy(O.params(nd, N), 2.5, g["phi1"], g["phi2"], g["phidot1"], g["phidot2"])
+    np.testing.assert_allclose(e1, ref, rtol=1e-11)
+    assert e1[2] != e0[2]
+    lat.set_state(p1, p2, v1, v2, ker1=k1, ker2=k2, tau=1.0)
+    g = lat.get_state()
+    ref = O.background(g["phi1"], g["phi2"], g["phidot1"], g["phidot2"])
+    np.testing.assert_allclose(lat.background(), ref, rtol=1e-12, atol=1e-9)
+    ref = O.energy(O.params(nd, N), 1.0, g["phi1"], g["phi2"], g["phidot1"], g["phidot2"])
+    np.testing.assert_allclose(lat.energy(), ref, rtol=1e-11)
+
+
 def test_energy_free_field_conserved():
     # lambda = 0, T0 = 0: H = K + G is conserved by velocity-Verlet up to a
     # bounded O((w dt)^2) oscillation.  With smooth data (modes |n| < N/8,
