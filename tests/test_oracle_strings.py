@@ -162,3 +162,155 @@ def test_xi_known_values():
     assert O.xi3(300, 1.0, 256, 0.5) == np.float32(np.float32(200.0) * 0.25 / 128 ** 3)
     # tau = 2: t = 2, a = 2; 2D area = (256*0.5*2)^2 = 65536 -> xi = n*4/65536
     assert O.xi2(64, 2.0, 256, 0.5) == np.float32(64 * 4.0 / 65536)
+
+
+# ---------------------------------------------------------------------------
+# 3D plaquette geometry (string_finding.cpp:49-62) and the threshold
+# percentage (string_finding.cpp:7-8, 39-40), pinned by flag sets derived by
+# hand.  Corner offsets of site s = (i, j, k), in the order the edges run:
+#   a: s, s+(1,0,0), s+(1,1,0), s+(0,1,0)
+#   b: s, s+(1,0,0), s+(0,1,1), s+(0,0,1)     (not planar: Q8)
+#   c: s, s+(0,1,0), s+(0,1,1), s+(0,0,1)
+# Three edges per plaquette: corner 1 and corner 4 each touch one counted
+# edge, corners 2 and 3 two.
+
+def _flag_set_3d(turns, N, thr=1):
+    """flagged sites of a field given as angles in turns, [i, j, k]"""
+    n, c = O.cores(2 * np.pi * np.asarray(turns, np.float64).reshape(-1), 3, N, thr=thr)
+    return {tuple(int(v) for v in x) for x in c}
+
+
+def _shift(p, d, N):
+    return tuple((a - b) % N for a, b in zip(p, d))
+
+
+@pytest.mark.parametrize("delta", [0.3, 0.6, -0.3])
+@pytest.mark.parametrize("P", [(2, 3, 1), (0, 0, 0), (5, 5, 5)])
+def test_single_raised_corner_3d(delta, P):
+    # One site turned by delta (circular distance 0.3 or 0.4 to the rest),
+    # the rest at angle 0.  A plaquette with P as corner 2 or 3 sums to
+    # 2 * 0.3 (0.4) > 0.495; with P as corner 1 or 4 to 0.3 (0.4) < 0.495.
+    # P is corner 2/3 of: a at s = P-(1,0,0), P-(1,1,0); b at P-(1,0,0),
+    # P-(0,1,1); c at P-(0,1,0), P-(0,1,1).
+    N = 6
+    t = np.zeros((N, N, N))
+    t[P] = delta
+    want = {_shift(P, d, N) for d in ((1, 0, 0), (1, 1, 0), (0, 1, 1), (0, 1, 0))}
+    assert _flag_set_3d(t, N) == want
+
+
+def test_plaquette_a_alone_decides():
+    # P = s+(1,1,0) is corner 3 of a at s (sum 0.6) and no corner of b or c
+    # at s, so s is flagged by a alone; s+(0,1,0) = P-(1,0,0) by a and b.
+    N = 6
+    s = (2, 2, 2)
+    t = np.zeros((N, N, N))
+    t[3, 3, 2] = 0.3
+    got = _flag_set_3d(t, N)
+    assert s in got
+    assert got == {(2, 3, 2), (2, 2, 2), (3, 2, 1), (3, 2, 2)}
+
+
+def test_plaquette_c_alone_decides():
+    # P = s+(0,1,0) is corner 2 of c at s (0.6), corner 4 of a at s (0.3),
+    # no corner of b at s: s is flagged by c alone.
+    N = 6
+    t = np.zeros((N, N, N))
+    t[2, 3, 2] = 0.3
+    got = _flag_set_3d(t, N)
+    assert (2, 2, 2) in got
+    assert got == {(1, 3, 2), (1, 2, 2), (2, 2, 1), (2, 2, 2)}
+
+
+@pytest.mark.parametrize("s", [(2, 2, 2), (5, 0, 5)])
+def test_plaquette_b_alone_decides(s):
+    # s+(1,0,0) at +0.2 turns and s+(0,1,1) at -0.2, everything else at 0.
+    # At s: a = 0.2 + 0.2 + 0 = 0.4, c = 0 + 0.2 + 0.2 = 0.4, but
+    # b = 0.2 + min(0.4, 0.6) + 0.2 = 0.8 > 0.495.  Any other plaquette
+    # holds at most one of the two sites (only b pairs corners that differ
+    # by (-1, 1, 1)) and sums to at most 0.4, so s is the only flagged site.
+    # A planar b through (i+1, j, k+1) would flag nothing here.
+    N = 6
+    t = np.zeros((N, N, N))
+    t[(s[0] + 1) % N, s[1], s[2]] = 0.2
+    t[s[0], (s[1] + 1) % N, (s[2] + 1) % N] = -0.2
+    assert _flag_set_3d(t, N) == {s}
+
+
+def test_straight_string_along_j_exact_set():
+    # Angles in quarter turns q(i, k), the same for every j: one winding
+    # cell (corners (1,1), (2,1), (2,2), (1,2) at 0, 1, 2, 3 quarters).
+    #       k: 0 1 2 3
+    #   i=0:   0 0 0 0
+    #   i=1:   0 0 3 0
+    #   i=2:   0 1 2 0
+    #   i=3:   0 0 0 0
+    # Along j the plaquettes of site (i, j, k) reduce to
+    #   a = 2 d(q(i,k), q(i+1,k)),
+    #   b = d(q(i,k), q(i+1,k)) + d(q(i+1,k), q(i,k+1)),
+    #   c = d(q(i,k), q(i,k+1)),
+    # with d in quarters 0, .25, .5, .25 for |dq| = 0, 1, 2, 3.  a flags every
+    # (i, k) whose i-neighbour differs: (0,2), (1,1), (1,2), (2,1), (2,2); b
+    # and c add nothing (b is at most .25 where a is 0, c never reaches .5).
+    # A planar b would also flag (1, 0): .25 + .25 through (2,0)-(2,1)-(1,1).
+    N = 4
+    q = np.array([[0, 0, 0, 0], [0, 0, 3, 0], [0, 1, 2, 0], [0, 0, 0, 0]], np.float64)
+    t = np.repeat((q / 4.0)[:, None, :], N, axis=1)        # [i, j, k]
+    cells = {(0, 2), (1, 1), (1, 2), (2, 1), (2, 2)}
+    want = {(i, j, k) for (i, k) in cells for j in range(N)}
+    assert _flag_set_3d(t, N) == want
+
+
+# float thresholds 0.5f * (1.0f - thr/100.0f): thr 0 -> 0.5 exactly,
+# thr 1 -> 0.4950000047683716 (above 0.495), thr 5 -> 0.4749999940395355
+# (below 0.475), thr 30 -> 0.3499999940395355 (below 0.35)
+FLOAT_ACCEPT = {0: 0.5, 1: 0.4950000047683716, 5: 0.4749999940395355, 30: 0.3499999940395355}
+
+
+@pytest.mark.parametrize("thr", [0, 1, 5, 30])
+def test_threshold_percent_2d(thr):
+    # three equal edges of s/3 turns around plaquette (0,0): flagged iff the
+    # sum exceeds the float threshold; sums a few 1e-9 either side of it,
+    # and of the decimal value when the two differ by more than that
+    N = 4
+    acc = FLOAT_ACCEPT[thr]
+
+    def flagged(s):
+        a = s / 3
+        th = np.zeros((N, N))
+        th[0, 1], th[1, 1], th[1, 0] = 2 * np.pi * a, 2 * np.pi * 2 * a, 2 * np.pi * 3 * a
+        n, c = O.cores(th.reshape(-1), 2, N, thr=thr)
+        return (0, 0) in {tuple(x) for x in c}
+
+    assert flagged(acc + 2e-9)
+    assert not flagged(acc - 2e-9)
+    dec = 0.5 * (1 - thr / 100)
+    if thr == 1:
+        assert not flagged(dec + 2e-9)      # the float threshold is above 0.495
+    elif thr in (5, 30):
+        assert flagged(dec - 2e-9)          # ... and below 0.475, 0.35
+
+
+@pytest.mark.parametrize("thr,delta,n_want", [
+    (1, 0.2, 0), (30, 0.2, 4),              # 2 * 0.2 = 0.4 against 0.495 / 0.35
+    (1, 0.4, 4), (30, 0.4, 6),              # one edge of 0.4 passes 0.35 too
+    (5, 0.24, 4), (1, 0.24, 0),             # 0.48 against 0.475 / 0.495
+    (0, 0.2499, 0), (0, 0.2501, 4),         # 0.4998 / 0.5002 against 0.5
+    (100, 0.01, 6), (100, 0.0, 0),          # thr 100: accept 0, any edge passes; flat: 0 > 0 fails
+])
+def test_threshold_percent_3d(thr, delta, n_want):
+    # single raised corner: corners 2/3 give 2 * d, corners 1/4 give d
+    # (d = min(delta, 1 - delta)); at thr = 30 one edge of 0.4 flags the two
+    # sites with P as corner 1 or 4 of some plaquette as well: P itself and
+    # P-(0,0,1) (a's corner 4 at P-(0,1,0) is already flagged).
+    N = 6
+    P = (3, 2, 4)
+    t = np.zeros((N, N, N))
+    t[P] = delta
+    got = _flag_set_3d(t, N, thr=thr)
+    assert len(got) == n_want
+    core = {_shift(P, d, N) for d in ((1, 0, 0), (1, 1, 0), (0, 1, 1), (0, 1, 0))}
+    if n_want == 4:
+        assert got == core
+    if n_want == 6:
+        assert got == core | {P, _shift(P, (0, 0, 1), N)}
