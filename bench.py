@@ -11,21 +11,26 @@ steps are followed by exactly K timed steps, with the stream synchronised
 before and after.
 
 The JSON line also carries
-  roofline     : the step kernel's achieved rate: algorithmic bytes per
-                 launch (32 B per site-update in FP32: read + write of phi1,
-                 phi2 and the two velocities, times the site-updates of one
-                 launch: N^3 per time step it advances) over the launch's
-                 average duration, measured with HIP events on the library's
-                 stream.  The default FP32 kernel advances four time steps
-                 per launch, reading and writing the state once, so this
-                 figure exceeds the HBM peak; `traffic` (measured HBM bytes
-                 per launch) and `traffic_rate` give the bytes actually
-                 moved;
+  roofline     : the step kernel against HBM.  One launch of the default
+                 FP32 kernel advances D time steps (D = 4 at 1024^3) and
+                 reads and writes the state once: its compulsory bytes are
+                 32 B per site per launch (read + write of phi1, phi2, u1,
+                 u2 in float).  `achieved` = 32 B x N^3 / average launch
+                 duration (HIP events on the library's stream), `frac` =
+                 achieved / 8 TB/s.  `site_update_rate` is the same bytes
+                 counted once per site-update (32 B x D x N^3 per launch),
+                 which exceeds the peak by design; `traffic` is the HBM
+                 bytes one launch moves, from the committed rocprofv3 PMC
+                 profile of the same kernel configuration (`traffic_source`
+                 names it; null when the run's configuration differs);
   cpu_baseline : the CPU oracle (reference-structured FP64 step: drift,
                  two CSR sparse-matrix Laplacians, potential, kick, copy, with
                  OpenMP) timed on this host over a bounded sample;
-  accuracy     : FP32 vs FP64 field rel-L2, string count, energy and
-                 power-spectrum differences after the same W+K steps.
+  config3      : the 3D 1024^3 configuration as a whole run in both forms
+                 -- always 1000 steps from the same seeded field, the string
+                 count and the 64-bin log power spectrum every 50 steps --
+                 with its accuracy gate: FP32 vs FP64 field rel-L2 <= 1e-4
+                 and string count within 1% after the 1000 steps.
 """
 from __future__ import annotations
 
@@ -40,9 +45,12 @@ sys.path.insert(0, os.path.join(ROOT, "axion-strings_amd"))
 
 HBM_PEAK_GBS = 8000.0
 # measured HBM bytes per launch of the FP32 step kernels at 1024^3, by time
-# steps per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)
+# steps per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated); each file
+# names the kernel configuration it was measured on
 TRAFFIC_FILES = {1: os.path.join(ROOT, "profiles", "traffic_march_fp32_3d_1024.json"),
                  4: os.path.join(ROOT, "profiles", "traffic_multi_step_fp32_3d_1024.json")}
+# compulsory bytes per site and launch: the state read once and written once
+PASS_BYTES = {"fp32": 32.0, "fp64": 96.0}
 
 
 def log(msg: str) -> None:
@@ -130,27 +138,98 @@ def sampling_leg(np, lat, N, reps=3):
     return out
 
 
-def accuracy(np, lat32, lat64):
-    s32 = lat32.get_state(fields=("phi1", "phi2"))
-    s64 = lat64.get_state(fields=("phi1", "phi2"))
-    num = np.sqrt(np.sum((s32["phi1"] - s64["phi1"]) ** 2) + np.sum((s32["phi2"] - s64["phi2"]) ** 2))
-    den = np.sqrt(np.sum(s64["phi1"] ** 2) + np.sum(s64["phi2"] ** 2))
-    del s32, s64
-    n32 = lat32.strings(coords=False)[0]
-    n64 = lat64.strings(coords=False)[0]
-    e32, e64 = lat32.energy(), lat64.energy()
-    _, pk32, c32 = lat32.powerspec(binning="log", nbins=64)
-    _, pk64, c64 = lat64.powerspec(binning="log", nbins=64)
-    sel = c64 > 0
-    pk_err = float(np.max(np.abs(pk32[sel] - pk64[sel]) / np.abs(pk64[sel])))
-    return {
-        "field_rel_l2": float(num / den),
-        "string_count_fp32": int(n32),
-        "string_count_fp64": int(n64),
-        "string_count_rel_diff": float(abs(n32 - n64) / max(n64, 1)),
-        "energy_rel_diff": float(abs(e32.sum() - e64.sum()) / abs(e64.sum())),
-        "pk_log64_max_rel_err": pk_err,
+def _rel_l2_chunked(np, a, b, chunk=1 << 26):
+    num = den = 0.0
+    for lo in range(0, a.size, chunk):
+        d = a[lo:lo + chunk] - b[lo:lo + chunk]
+        num += float(np.dot(d, d))
+        den += float(np.dot(b[lo:lo + chunk], b[lo:lo + chunk]))
+    return num, den
+
+
+def _pk_rel_err(np, a, b):
+    """largest relative difference of a bin holding at least 1e-8 of the
+    peak power (below that a bin holds rounding noise: the initial k^-1
+    field has no power above pi/(4 dx))"""
+    sel = b >= 1e-8 * b.max()
+    return float(np.max(np.abs(a[sel] - b[sel]) / b[sel])) if sel.any() else 0.0
+
+
+def config3_leg(np, lats, steps=1000, every=50, nbins=64):
+    """The 3D configuration as a whole run in each form (the contexts of
+    the timed legs, re-initialised): the device k^-1 field of seed 1998,
+    tau = 1, `steps` steps with the string count and the log-binned power
+    spectrum sampled every `every` steps before the step (s_run.cpp:83-85);
+    the sampling path's one-time set-up (FFT plans) runs before the clock.
+    Then the FP32 run is compared with the FP64 run: field rel-L2 and string
+    count after the last step (the accuracy gate), energy, and the largest
+    differences over the in-loop samples."""
+    out, samples = {}, {}
+    for prec, lat in lats.items():
+        lat.init("kinv", seed=1998)
+        lat.tau = 1.0
+        lat.string_counts()
+        lat.powerspec(binning="log", nbins=nbins)
+        lat.init("kinv", seed=1998)
+        lat.tau = 1.0
+        counts, pks = [], []
+        step_s = samp_s = 0.0
+        t0 = time.perf_counter()
+        for _ in range(0, steps, every):
+            ts = time.perf_counter()
+            counts.append(int(lat.string_counts()[0][0]))
+            pks.append(lat.powerspec(binning="log", nbins=nbins)[1])
+            tm = time.perf_counter()
+            lat.step(every)
+            step_s += time.perf_counter() - tm
+            samp_s += tm - ts
+        total = time.perf_counter() - t0
+        samples[prec] = (counts, pks)
+        L = lat.length()
+        out[prec] = {"run_s": total, "step_s": step_s, "sample_s": samp_s,
+                     "site_updates_per_s": L * steps / total,
+                     "step_site_updates_per_s": L * steps / step_s,
+                     "final_string_count": int(lat.string_counts()[0][0]),
+                     "energy": [float(x) for x in lat.energy()]}
+        log(f"[bench] config3 {prec}: {steps} steps + {len(counts)} samples in {total:.2f} s "
+            f"(steps {step_s:.2f} s), {out[prec]['final_string_count']} strings")
+    if set(lats) != {"fp32", "fp64"}:
+        return out
+    num = den = 0.0
+    for comp in ("phi1", "phi2"):
+        a = lats["fp32"].get_state(fields=(comp,))[comp]
+        b = lats["fp64"].get_state(fields=(comp,))[comp]
+        n_, d_ = _rel_l2_chunked(np, a, b)
+        num, den = num + n_, den + d_
+        del a, b
+    rel_l2 = float(np.sqrt(num / den))
+    n32, n64 = out["fp32"]["final_string_count"], out["fp64"]["final_string_count"]
+    e32, e64 = sum(out["fp32"]["energy"]), sum(out["fp64"]["energy"])
+    c32, p32 = samples["fp32"]
+    c64, p64 = samples["fp64"]
+    cnt_err = max(abs(a - b) / max(b, 1) for a, b in zip(c32, c64))
+    pk_max = pk_pow = 0.0
+    for a, b in zip(p32, p64):
+        pk_max = max(pk_max, _pk_rel_err(np, a, b))
+        pk_pow = max(pk_pow, float(np.sum(np.abs(a - b)) / np.sum(np.abs(b))))
+    count_rel = abs(n32 - n64) / max(n64, 1)
+    out["accuracy_fp32_vs_fp64"] = {
+        "steps": steps,
+        "field_rel_l2": rel_l2,
+        "string_count_fp32": n32,
+        "string_count_fp64": n64,
+        "string_count_rel_diff": count_rel,
+        "energy_rel_diff": abs(e32 - e64) / abs(e64),
+        "samples": len(c32),
+        "sample_string_count_rel_diff_max": cnt_err,
+        "pk_log64_max_rel_err": pk_max,
+        "pk_log64_max_rel_err_floor": "bins holding >= 1e-8 of the sample's peak power",
+        "pk_log64_power_weighted_err_max": pk_pow,
+        "gate": {"field_rel_l2_max": 1e-4, "string_count_rel_diff_max": 0.01,
+                 "pass": bool(rel_l2 <= 1e-4 and count_rel <= 0.01)},
     }
+    out["fp32_speedup_over_fp64_run"] = out["fp64"]["run_s"] / out["fp32"]["run_s"]
+    return out
 
 
 ENSEMBLES = {
@@ -227,12 +306,14 @@ def ensemble_leg(np, axs, name, T0, steps_override=0, autotune=True):
     out["string_count_rel_diff_mean"] = float(rel.mean())
     out["string_count_total_fp32"] = int(c32.sum())
     out["string_count_total_fp64"] = int(c64.sum())
-    pk_err = 0.0
+    pk_err = pk_pow = 0.0
     for a, b in zip(spectra["fp32"], spectra["fp64"]):
-        sel = b != 0
-        if sel.any():
-            pk_err = max(pk_err, float(np.max(np.abs(a[sel] - b[sel]) / np.abs(b[sel]))))
+        if np.any(b != 0):
+            pk_err = max(pk_err, _pk_rel_err(np, a, b))
+            pk_pow = max(pk_pow, float(np.sum(np.abs(a - b)) / np.sum(np.abs(b))))
     out["pk_log64_max_rel_err"] = pk_err
+    out["pk_log64_max_rel_err_floor"] = "bins holding >= 1e-8 of the member's peak power"
+    out["pk_log64_power_weighted_err_max"] = pk_pow
     return out
 
 
@@ -306,16 +387,14 @@ WORKFLOWS = {
 def workflow_leg(np, axs, T0, cpu_rate_3d=None):
     """Wall time of each configuration run on the GPU (steps plus the
     in-loop sampling, as the driver makes the calls): configs 0 and 1 in
-    the FP64 form, config 3 in the FP32 form; config 0 also on the CPU
-    oracle, configs 1 and 3 on the CPU estimated from the measured
+    the FP64 form from the host k^-1 generator's field; config 0 also on
+    the CPU oracle, config 1 on the CPU estimated from the measured
     reference-structured step rate (steps only, no sampling)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle as O
-    from fields import kinv_field
     out = {}
     for name, (nd, N, steps, every, energy) in WORKFLOWS.items():
-        p1, p2 = kinv_field(nd, N, seed=1998)
+        p1, p2, _, _ = axs.host_init("kinv", nd, N, T_initial=T0, seed=1998)
         z = np.zeros(N ** nd)
         lat = axs.Lattice(nd, N, precision="fp64", T_initial=T0)
         lat.set_state(p1, p2, z, z, tau=1.0)
@@ -350,34 +429,6 @@ def workflow_leg(np, axs, T0, cpu_rate_3d=None):
         out[name] = rec
         log(f"[bench] {name}: GPU {gpu_s:.3f} s" +
             (f", CPU {rec['cpu_s']:.2f} s" if "cpu_s" in rec else ""))
-    # config 3 as a whole run in the FP32 form: 3D 1024^3, 1000 steps, the
-    # string count and the 64-bin log spectrum every 50 steps, from the
-    # device k^-1 field (the host generator would take minutes at 1024^3);
-    # the sampling path's one-time set-up (FFT plans) runs before the clock
-    N, steps, every = 1024, 1000, 50
-    lat = axs.Lattice(3, N, precision="fp32", T_initial=T0)
-    lat.init("kinv", seed=1998)
-    lat.string_counts()
-    lat.powerspec(binning="log", nbins=64)
-    lat.init("kinv", seed=1998)
-    lat.tau = 1.0
-    t0 = time.perf_counter()
-    nstr = 0
-    for done in range(0, steps, every):
-        nstr = int(lat.string_counts()[0][0])
-        lat.powerspec(binning="log", nbins=64)
-        lat.step(every)
-    gpu_s = time.perf_counter() - t0
-    lat.close()
-    rec = {"ndims": 3, "N": N, "steps": steps, "sample_every": every, "precision": "fp32",
-           "samples": "string count + 64-bin log power spectrum", "gpu_s": gpu_s,
-           "gpu_site_updates_per_s": N ** 3 * steps / gpu_s, "last_string_count": nstr}
-    if cpu_rate_3d:
-        rec["cpu_s"] = N ** 3 * steps / cpu_rate_3d
-        rec["cpu_kind"] = "oracle FP64 step rate at 256^3 x steps (sampling not included)"
-        rec["speedup"] = rec["cpu_s"] / gpu_s
-    out["config3_3d_1024_fp32"] = rec
-    log(f"[bench] config3_3d_1024_fp32: GPU {gpu_s:.3f} s")
     return out
 
 
@@ -422,9 +473,8 @@ def cpu_baseline(np, seconds: float):
     3D 512^3, the last standing in for 1024^3 per site (its FP64 state and
     the reference's int32 COO matrix do not fit the reference structure)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle as O
-    from fields import kinv_field
+    import axs
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
 
     def timed(run, budget, sites):
@@ -438,7 +488,7 @@ def cpu_baseline(np, seconds: float):
         return sites * n / t, n, t
 
     Ns = 256
-    p1, p2 = kinv_field(3, Ns, seed=1998)
+    p1, p2, _, _ = axs.host_init("kinv", 3, Ns, seed=1998)
     z = np.zeros(Ns ** 3)
     p = O.params(3, Ns)
     A = O.Matrix(3, Ns)
@@ -447,7 +497,7 @@ def cpu_baseline(np, seconds: float):
     del A
     sf = O.State(p1, p2, z, z)
     f256, _, _ = timed(lambda k: sf.run_fused(p, k), 0.15 * seconds, Ns ** 3)
-    q1, q2 = kinv_field(2, 256, seed=1998)
+    q1, q2, _, _ = axs.host_init("kinv", 2, 256, seed=1998)
     z2 = np.zeros(256 ** 2)
     p2d = O.params(2, 256)
     A2 = O.Matrix(2, 256)
@@ -528,6 +578,10 @@ def main() -> int:
                     help="skip the Hamiltonian-form and spectral-Laplacian legs")
     ap.add_argument("--skip-workflows", action="store_true",
                     help="skip the whole-run timings of the 256^2 and 256^3 configurations")
+    ap.add_argument("--skip-config3", action="store_true",
+                    help="skip the whole 1000-step run of both forms and its accuracy gate")
+    ap.add_argument("--config3-steps", type=int, default=1000,
+                    help="steps of the config-3 run (the accuracy gate is stated at 1000)")
     ap.add_argument("--skip-stencils", action="store_true",
                     help="skip the higher-order Laplacian legs")
     ap.add_argument("--skip-ensembles", action="store_true",
@@ -562,15 +616,26 @@ def main() -> int:
         return 0
     lat32, r32 = gpu_leg(np, axs, N, "fp32", W, K, args.T_initial, args.depth)
     out64 = None
-    acc = None
+    config3 = None
     sampling = None
+    lat64 = None
     if not args.skip_fp64 and "fp64" in forms:
         lat64, r64 = gpu_leg(np, axs, N, "fp64", W, K, args.T_initial, args.depth)
-        try:
-            log("[bench] comparing FP32 with FP64")
-            acc = accuracy(np, lat32, lat64)
-        except Exception as e:              # reported, the headline line still prints
-            acc = {"error": str(e)}
+        launch64 = r64["kernel_ms"] * r64["steps_per_launch"]
+        out64 = {
+            "value": r64["site_updates_per_s"],
+            "ms_per_step": r64["ms_per_step"],
+            "kernel_ms": r64["kernel_ms"],
+            "steps_per_launch": r64["steps_per_launch"],
+            "launch_ms": launch64,
+            # compulsory bytes of a pass (phi, phidot, ker read and written
+            # once: 96 B per site) over the launch time
+            "roofline_frac": PASS_BYTES["fp64"] * N ** 3 / (launch64 * 1e-3)
+            / (HBM_PEAK_GBS * 1e9),
+            "scheme_bytes_per_site": r64["scheme_bytes_per_site"],
+            "kernel_config": r64["kernel_config"],
+            "init_s": r64["init_s"],
+        }
         if not args.skip_sampling:
             try:
                 sampling = {"fp32": sampling_leg(np, lat32, N),
@@ -579,29 +644,37 @@ def main() -> int:
                     f"{k} {v['ms']:.1f} ms" for k, v in sampling["fp32"].items()))
             except Exception as e:          # reported, the headline line still prints
                 sampling = {"error": str(e)}
+    if not args.skip_config3:
+        try:
+            lats = {"fp32": lat32} if lat64 is None else {"fp32": lat32, "fp64": lat64}
+            config3 = config3_leg(np, lats, steps=args.config3_steps)
+            a = config3.get("accuracy_fp32_vs_fp64")
+            if a:
+                log(f"[bench] config3 accuracy after {a['steps']} steps: rel-L2 "
+                    f"{a['field_rel_l2']:.3g}, strings {a['string_count_fp32']} vs "
+                    f"{a['string_count_fp64']}, gate {'pass' if a['gate']['pass'] else 'FAIL'}")
+        except Exception as e:              # reported, the headline line still prints
+            config3 = {"error": str(e)}
+    if lat64 is not None:
         lat64.close()
-        out64 = {
-            "value": r64["site_updates_per_s"],
-            "ms_per_step": r64["ms_per_step"],
-            "kernel_ms": r64["kernel_ms"],
-            "roofline_frac_alg": (r64["alg_bytes_per_site"] * N ** 3 / (r64["kernel_ms"] * 1e-3))
-            / (HBM_PEAK_GBS * 1e9),
-            "scheme_bytes_per_site": r64["scheme_bytes_per_site"],
-            "kernel_config": r64["kernel_config"],
-            "init_s": r64["init_s"],
-        }
     lat32.close()
 
     L = N ** 3
     spl = r32["steps_per_launch"]
     launch_ms = r32["kernel_ms"] * spl
-    achieved = r32["alg_bytes_per_site"] * L * spl / (launch_ms * 1e-3) / 1e9
-    traffic = None
+    achieved = PASS_BYTES["fp32"] * L / (launch_ms * 1e-3) / 1e9
+    update_rate = r32["alg_bytes_per_site"] * L * spl / (launch_ms * 1e-3) / 1e9
+    traffic, traffic_src = None, None
     tfile = TRAFFIC_FILES.get(spl)
     if tfile and os.path.exists(tfile):
         try:
             t = json.load(open(tfile))
-            if int(t.get("N", -1)) == N and int(t.get("steps_per_launch", 1)) == spl:
+            same = (int(t.get("N", -1)) == N and int(t.get("steps_per_launch", 1)) == spl and
+                    t.get("kernel_config") == r32["kernel_config"])
+            traffic_src = {"file": os.path.relpath(tfile, ROOT),
+                           "kernel": t.get("kernel"), "kernel_config": t.get("kernel_config"),
+                           "matches_run": bool(same)}
+            if same:
                 traffic = t.get("bytes_per_launch")
         except (ValueError, OSError):
             traffic = None
@@ -677,8 +750,13 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "basis": "compulsory bytes of one launch: 32 B per site (phi1, phi2, u1, u2 "
+                     "read and written once in float) x N^3, over the HIP-event average "
+                     "launch duration",
             "steps_per_launch": spl,
             "launch_ms": launch_ms,
+            "site_update_rate": update_rate,
+            "traffic_source": traffic_src,
             "traffic_rate": (traffic / (launch_ms * 1e-3) / 1e9) if traffic else None,
             "traffic_frac": (traffic / (launch_ms * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
         },
@@ -687,7 +765,8 @@ def main() -> int:
         "init_s": r32["init_s"],
         "fp64": out64,
         "fp32_speedup_over_fp64": (r32["site_updates_per_s"] / out64["value"]) if out64 else None,
-        "accuracy_fp32_vs_fp64": acc,
+        "accuracy_fp32_vs_fp64": (config3 or {}).get("accuracy_fp32_vs_fp64"),
+        "config3": config3,
         "sampling": sampling,
         "ensembles": ens,
         "forms": forms_out,
