@@ -38,6 +38,11 @@ int axs_host_final_step(int N, float space_step, float time_step);
 /* xi of s_run.cpp:111-116 (2D, num_cores) and :123-129 (3D, plaquettes). */
 float axs_host_xi(int ndims, int64_t count, double tau, int N, float space_step);
 
+/* atan2(y[i], x[i]) by the double-double evaluation the device uses for
+ * the string finder's margin cases (correctly rounded but for inputs within
+ * ~2^-100 of a rounding boundary), the same bits as the device build.      */
+void axs_host_atan2_dd(int64_t n, const double *y, const double *x, double *out);
+
 /* run_standard (s_run.cpp:26-232) with the parameters of `path`; returns 0
  * on success.  This is what axs_main runs.                                */
 int axs_host_run(const char *path);

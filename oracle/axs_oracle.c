@@ -21,7 +21,12 @@
  *                      plane wave against the closed-form leapfrog dispersion;
  *                      first step with ker=0 (phi_1 = phi_0 + dt phidot_0).
  *   cores2/cores3    : hand-built vortex / straight-string lattices, the
- *                      winding-number bound (every winding plaquette passes).
+ *                      winding-number bound (every winding plaquette passes),
+ *                      single raised corners and two-corner lattices that
+ *                      fix each plaquette's corners (plaquette b's (i,j+1,k+1)),
+ *                      the float threshold at thr = 0, 1, 5, 30, 100.
+ *   atan2 (axion)    : mpmath's correctly rounded atan2 on random and edge
+ *                      inputs; C99 Annex F special values.
  *   background       : closed forms on constant and antisymmetric fields.
  *   energy           : closed forms (uniform field, single plane wave) and
  *                      its variation equals the step's force (finite diff).
@@ -31,6 +36,7 @@
 #include "axs_oracle.h"
 
 #include <math.h>
+#include <quadmath.h>
 #include <stdlib.h>
 #include <string.h>
 #include <complex.h>
@@ -562,10 +568,38 @@ int orc_vvsl_run_form(const orc_params *p, const orc_matrix *A, int eom, int lap
 /* ---------------------------------------------------------------------- */
 /* strings                                                                 */
 
+/* atan2 rounded correctly to double.  The reference calls libm's atan2
+ * (s_run.cpp:94, 145, 189), whose last bit depends on the library (glibc
+ * 2.35 misrounds about 0.1% of random arguments by one ulp; macOS's libm, the
+ * reference's own build platform, is another implementation); the oracle
+ * fixes the canonical value, the correctly rounded one.  Method: the x87
+ * extended-precision atan2l (64-bit significand, error within an ulp of it)
+ * rounded to double, unless it lies within 2^-60 (relative) of the midpoint
+ * between two doubles; there libquadmath's atan2q (113-bit significand)
+ * decides.  Zeros, infinities and NaN take libm's C99 Annex F values.    */
+double orc_atan2_cr(double y, double x)
+{
+    if (isnan(x) || isnan(y) || isinf(x) || isinf(y) || x == 0.0 || y == 0.0)
+        return atan2(y, x);
+    long double l = atan2l((long double)y, (long double)x);
+    double d = (double)l;
+    if ((long double)d == l) return d;
+    double nb = (l > (long double)d) ? nextafter(d, INFINITY) : nextafter(d, -INFINITY);
+    long double mid = ((long double)d + (long double)nb) * 0.5L;      /* exact */
+    if (fabsl(l - mid) > fabsl(mid) * 0x1p-60L) return d;
+    return (double)atan2q((__float128)y, (__float128)x);
+}
+
+void orc_atan2_cr_array(int64_t n, const double *y, const double *x, double *out)
+{
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) out[i] = orc_atan2_cr(y[i], x[i]);
+}
+
 void orc_axion(int64_t L, const double *phi1, const double *phi2, double *axion)
 {
     #pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < L; i++) axion[i] = atan2(phi1[i], phi2[i]);   /* s_run.cpp:94 */
+    for (int64_t i = 0; i < L; i++) axion[i] = orc_atan2_cr(phi1[i], phi2[i]);   /* s_run.cpp:94 */
 }
 
 /* std::min(a, b) = (b < a) ? b : a ; theta = min(|d|, 1 - |d|)          */
@@ -722,7 +756,7 @@ void orc_background(int64_t L, const double *phi1, const double *phi2,
         s1 += phi2[i];
         s2 += phidot1[i];
         s3 += phidot2[i];
-        s4 += fabs(atan2(phi1[i], phi2[i]));
+        s4 += fabs(orc_atan2_cr(phi1[i], phi2[i]));
         s5 += sqrt(phi1[i] * phi1[i] + phi2[i] * phi2[i]);
     }
     out[0] = s0; out[1] = s1; out[2] = s2; out[3] = s3; out[4] = s4; out[5] = s5;
@@ -929,7 +963,8 @@ void orc_powerspec(int ndims, int N, float dx, const double *data_real,
 /* log-spaced bins: squared edges e2[b] = (N/2)^(2b/nbins), b = 0..nbins;
  * a mode with e2[b] <= |k|^2 < e2[b+1] falls in bin b.                   */
 void orc_powerspec_log(int ndims, int N, float dx, const double *data_real,
-                       int nbins, double *ks, double *pk, int32_t *count)
+                       const double *data_imag, int nbins, double *ks, double *pk,
+                       int32_t *count)
 {
     float L = N * dx;
     int64_t length = orc_length(ndims, N);
@@ -939,7 +974,8 @@ void orc_powerspec_log(int ndims, int N, float dx, const double *data_real,
     for (int b = 0; b < nbins; b++) { ks[b] = 0; pk[b] = 0; count[b] = 0; }
 
     double complex *data = (double complex *)malloc((size_t)length * sizeof(double complex));
-    for (int64_t m = 0; m < length; m++) data[m] = data_real[m];
+    for (int64_t m = 0; m < length; m++)
+        data[m] = data_real[m] + I * (data_imag ? data_imag[m] : 0.0);
     orc_fft(ndims, N, (double *)data, -1);
 
     for (int64_t m = 0; m < length; m++) {

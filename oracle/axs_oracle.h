@@ -73,6 +73,10 @@ void orc_vvsl_run_fused(const orc_params *p, int64_t nsteps,
                         double *phi1, double *phi2, double *phidot1, double *phidot2,
                         double *ker1, double *ker2, double *tau);
 
+/* atan2(y, x) rounded correctly to double (the oracle's libm atan2) */
+double orc_atan2_cr(double y, double x);
+void orc_atan2_cr_array(int64_t n, const double *y, const double *x, double *out);
+
 /* axion = atan2(phi1, phi2), src/standard/s_run.cpp:92-94 */
 void orc_axion(int64_t L, const double *phi1, const double *phi2, double *axion);
 
@@ -132,9 +136,10 @@ void orc_powerspec(int ndims, int N, float dx, const double *data_real,
                    const double *data_imag, double *ks, double *pk, int32_t *count);
 
 /* log-spaced binning form: nbins bins over 1 <= |k| < N/2 (lattice units),
- * pk = mean |F|^2 per mode, ks = mean |k| * 2 pi / L.                   */
+ * pk = mean |F|^2 per mode, ks = mean |k| * 2 pi / L; data_imag may be NULL. */
 void orc_powerspec_log(int ndims, int N, float dx, const double *data_real,
-                       int nbins, double *ks, double *pk, int32_t *count);
+                       const double *data_imag, int nbins, double *ks, double *pk,
+                       int32_t *count);
 
 #ifdef __cplusplus
 }

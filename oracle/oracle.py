@@ -49,6 +49,8 @@ def lib() -> ctypes.CDLL:
         "orc_vvsl_run": ([POINTER(OrcParams), c_void_p, c_int64] + [_DP] * 9, None),
         "orc_vvsl_run_fused": ([POINTER(OrcParams), c_int64] + [_DP] * 7, None),
         "orc_axion": ([c_int64, _DP, _DP, _DP], None),
+        "orc_atan2_cr": ([c_double, c_double], c_double),
+        "orc_atan2_cr_array": ([c_int64, _DP, _DP, _DP], None),
         "orc_cores2": ([_DP, c_int, c_int, _IP, c_int64], c_int64),
         "orc_cores3": ([_DP, c_int, c_int, _IP, c_int64], c_int64),
         "orc_num_cores2": ([_IP, c_int64], c_int64),
@@ -61,7 +63,7 @@ def lib() -> ctypes.CDLL:
         "orc_vvsl_run_form": ([POINTER(OrcParams), c_void_p, c_int, c_int, c_int64] + [_DP] * 9,
                               c_int),
         "orc_powerspec": ([c_int, c_int, c_float, _DP, _DP, _DP, _DP, _IP], None),
-        "orc_powerspec_log": ([c_int, c_int, c_float, _DP, c_int, _DP, _DP, _IP], None),
+        "orc_powerspec_log": ([c_int, c_int, c_float, _DP, _DP, c_int, _DP, _DP, _IP], None),
     }
     for name, (argt, res) in sig.items():
         fn = getattr(L, name)
@@ -176,6 +178,15 @@ def laplacian_fft(field, ndims: int, N: int, dx: float = 0.5) -> np.ndarray:
     return out
 
 
+def atan2_cr(y, x) -> np.ndarray:
+    """atan2 rounded correctly to double, elementwise"""
+    a, b = np.broadcast_arrays(np.asarray(y, np.float64), np.asarray(x, np.float64))
+    a, b = _f64(a), _f64(b)
+    out = np.empty_like(a)
+    lib().orc_atan2_cr_array(a.size, _dp(a), _dp(b), _dp(out))
+    return out
+
+
 def axion(phi1, phi2) -> np.ndarray:
     a, b = _f64(phi1), _f64(phi2)
     out = np.empty_like(a)
@@ -239,9 +250,10 @@ def powerspec(field_real, ndims: int, N: int, dx: float, field_imag=None):
     return ks, pk, cnt
 
 
-def powerspec_log(field_real, ndims: int, N: int, dx: float, nbins: int):
+def powerspec_log(field_real, ndims: int, N: int, dx: float, nbins: int, field_imag=None):
     re = _f64(field_real)
+    im = None if field_imag is None else _f64(field_imag)
     ks, pk, cnt = np.zeros(nbins), np.zeros(nbins), np.zeros(nbins, np.int32)
-    lib().orc_powerspec_log(ndims, N, dx, _dp(re), nbins, _dp(ks), _dp(pk),
+    lib().orc_powerspec_log(ndims, N, dx, _dp(re), _dp(im), nbins, _dp(ks), _dp(pk),
                             cnt.ctypes.data_as(_IP))
     return ks, pk, cnt

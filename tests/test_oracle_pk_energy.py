@@ -139,3 +139,47 @@ def test_energy_variation_is_the_force():
         eb = O.energy(p, tau, b, p2, z, z)
         d = ((ea[1] + ea[2]) - (eb[1] + eb[2])) / (2 * h)
         assert abs(d + vol * ker1[site]) < 1e-6 * max(1.0, abs(vol * ker1[site]))
+
+
+@pytest.mark.parametrize("ndims,N,mode", [(2, 32, (5, 0)), (2, 32, (3, -4)), (3, 16, (1, -2, 2))])
+def test_single_complex_exponential(ndims, N, mode):
+    # phi = exp(i 2 pi n.x/N) (re = cos, im = sin): all power N^(2d) in the
+    # one mode n -- not split between +-n as for a real cosine -- so its bin
+    # holds N^(2d) in linear and in log binning, every other bin nothing
+    dx = 0.5
+    grids = np.meshgrid(*[np.arange(N)] * ndims, indexing="ij")
+    if ndims == 2:
+        theta = 2 * np.pi * (mode[0] * grids[1] + mode[1] * grids[0]) / N
+    else:
+        theta = 2 * np.pi * sum(m * g for m, g in zip(mode, grids)) / N
+    re, im = np.cos(theta).reshape(-1), np.sin(theta).reshape(-1)
+    ks, pk, cnt = O.powerspec(re, ndims, N, dx, field_imag=im)
+    kn = np.sqrt(sum(m * m for m in mode))
+    b = int(np.floor(kn)) - 1
+    kbar = ks[b] / (2 * np.pi / (N * dx))
+    total = pk[b] * (2 * np.pi * kbar if ndims == 2 else 4 * np.pi * kbar ** 2)
+    assert abs(total - N ** (2 * ndims)) < 1e-8 * N ** (2 * ndims)
+    others = [i for i in range(N // 2) if i != b and cnt[i] > 0]
+    assert np.all(np.abs(pk[others]) < 1e-12 * N ** (2 * ndims))
+    ks, pk, cnt = O.powerspec_log(re, ndims, N, dx, 10, field_imag=im)
+    full = pk * cnt
+    bl = int(np.argmax(full))
+    assert abs(full[bl] - N ** (2 * ndims)) < 1e-8 * N ** (2 * ndims)
+    assert np.all(np.abs(np.delete(full, bl)) < 1e-12 * N ** (2 * ndims))
+
+
+@pytest.mark.parametrize("ndims,N", [(2, 32), (3, 16)])
+def test_complex_spectrum_is_the_sum_of_its_parts(ndims, N):
+    # every binned mode has |k| < N/2, so -k is binned with it: for
+    # f = a + i b, |F(k)|^2 + |F(-k)|^2 = 2 (|A(k)|^2 + |B(k)|^2), and each bin
+    # of the complex field's spectrum is the sum of its parts' bins
+    a, b = kinv_field(ndims, N, seed=6)
+    for spec in (lambda re, im=None: O.powerspec(re, ndims, N, 0.5, field_imag=im),
+                 lambda re, im=None: O.powerspec_log(re, ndims, N, 0.5, 9, field_imag=im)):
+        ks, pk, cnt = spec(a, b)
+        _, pa, ca = spec(a)
+        _, pb, cb = spec(b)
+        np.testing.assert_array_equal(cnt, ca)
+        # bins above pi/(4 dx) hold only transform rounding: floor 1e-9 of the
+        # peak (an empty bin is 0/0 in the reference's normalisation: NaN)
+        np.testing.assert_allclose(pk, pa + pb, rtol=1e-12, atol=1e-9 * np.nanmax(pk))
