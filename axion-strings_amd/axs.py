@@ -39,6 +39,14 @@ AXS_INIT_THERMAL = 1
 AXS_SNAP_PHI = 1
 AXS_SNAP_VEL = 2
 AXS_SNAP_KER = 4
+AXS_FIELD_PHI2 = 0
+AXS_FIELD_PHI1 = 1
+AXS_FIELD_PHI = 2
+AXS_FIELD_PHIDOT2 = 3
+AXS_FIELD_PHIDOT1 = 4
+AXS_FIELD_PHIDOT = 5
+FIELDS = {"phi2": AXS_FIELD_PHI2, "phi1": AXS_FIELD_PHI1, "phi": AXS_FIELD_PHI,
+          "phidot2": AXS_FIELD_PHIDOT2, "phidot1": AXS_FIELD_PHIDOT1, "phidot": AXS_FIELD_PHIDOT}
 AXS_EOM_FIELD_RESCALED = 0
 AXS_EOM_HAMILTONIAN = 1
 AXS_LAPLACIAN_STENCIL = 0
@@ -51,9 +59,11 @@ EXPORTS = (
     "axs_create", "axs_create_batch", "axs_destroy", "axs_num_members",
     "axs_set_member_state", "axs_get_member_state", "axs_init_member", "axs_set_tau",
     "axs_get_tau",
-    "axs_set_state", "axs_get_state", "axs_step", "axs_member_strings", "axs_strings",
+    "axs_set_state", "axs_get_state", "axs_step", "axs_member_axion", "axs_member_strings",
+    "axs_strings",
     "axs_string_counts", "axs_member_background", "axs_background",
-    "axs_member_energy", "axs_energy", "axs_member_powerspec", "axs_powerspec",
+    "axs_member_energy", "axs_energy",
+    "axs_member_powerspec", "axs_member_powerspec_field", "axs_powerspec",
     "axs_powerspec_all",
     "axs_last_step_ms", "axs_bytes_per_site", "axs_strerror", "axs_last_error",
     "axs_set_kernel_config", "axs_get_kernel_config", "axs_autotune", "axs_member_laplacian",
@@ -134,6 +144,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "axs_set_state": ([c_void_p] + [_DP] * 6 + [c_double], c_int),
         "axs_get_state": ([c_void_p] + [_DP] * 7, c_int),
         "axs_step": ([c_void_p, c_int64], c_int),
+        "axs_member_axion": ([c_void_p, c_int, _DP], c_int),
         "axs_member_strings": ([c_void_p, c_int, POINTER(c_int32), c_int64, POINTER(c_int64)], c_int),
         "axs_strings": ([c_void_p, POINTER(c_int32), c_int64, POINTER(c_int64)], c_int),
         "axs_string_counts": ([c_void_p, POINTER(c_int64), POINTER(c_int64)], c_int),
@@ -142,6 +153,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "axs_member_energy": ([c_void_p, c_int, _DP], c_int),
         "axs_energy": ([c_void_p, _DP], c_int),
         "axs_member_powerspec": ([c_void_p, c_int, c_int, c_int, _DP, _DP, POINTER(c_int32)], c_int),
+        "axs_member_powerspec_field": ([c_void_p, c_int, c_int, c_int, c_int, _DP, _DP,
+                                        POINTER(c_int32)], c_int),
         "axs_powerspec_all": ([c_void_p, c_int, c_int, _DP, _DP, POINTER(c_int32)], c_int),
         "axs_powerspec": ([c_void_p, c_int, c_int, _DP, _DP, POINTER(c_int32)], c_int),
         "axs_last_step_ms": ([c_void_p, _DP], c_int),
@@ -316,6 +329,13 @@ class Lattice:
         return s.value, a.value
 
     # -- strings -----------------------------------------------------------
+    def axion(self, member: int = 0) -> np.ndarray:
+        """atan2(phi1, phi2) of every site (the reference's axion array),
+        rounded correctly"""
+        out = np.empty(self.length(member), np.float64)
+        _check(self.lib.axs_member_axion(self._ctx, member, _dp(out)), "axs_member_axion")
+        return out
+
     def strings(self, member: int = 0, coords: bool = True):
         """(count, coords[count, ndims] int32 or None), ascending site order."""
         n = c_int64()
@@ -383,16 +403,26 @@ class Lattice:
         _check(self.lib.axs_member_energy(self._ctx, member, _dp(out)), "axs_member_energy")
         return out
 
-    def powerspec(self, member: int = 0, binning: str = "linear", nbins: int = 64):
+    def powerspec(self, member: int = 0, binning: str = "linear", nbins: int = 64,
+                  field: str = "phi2"):
+        """(ks, pk, count) of phi2 (the reference's call) or of another field:
+        phi1, phidot1, phidot2, or the complex phi = phi1 + i phi2 and
+        phidot = phidot1 + i phidot2"""
         N = self.Ns[member]
         b = AXS_BIN_LINEAR if binning == "linear" else AXS_BIN_LOG
         nb = N // 2 if b == AXS_BIN_LINEAR else nbins
         ks = np.zeros(nb, np.float64)
         pk = np.zeros(nb, np.float64)
         cnt = np.zeros(nb, np.int32)
-        _check(self.lib.axs_member_powerspec(self._ctx, member, b, nb, _dp(ks), _dp(pk),
-                                             cnt.ctypes.data_as(POINTER(c_int32))),
-               "axs_member_powerspec")
+        if field == "phi2":
+            _check(self.lib.axs_member_powerspec(self._ctx, member, b, nb, _dp(ks), _dp(pk),
+                                                 cnt.ctypes.data_as(POINTER(c_int32))),
+                   "axs_member_powerspec")
+        else:
+            _check(self.lib.axs_member_powerspec_field(self._ctx, member, FIELDS[field], b, nb,
+                                                       _dp(ks), _dp(pk),
+                                                       cnt.ctypes.data_as(POINTER(c_int32))),
+                   "axs_member_powerspec_field")
         return ks, pk, cnt
 
     def powerspec_all(self, binning: str = "linear", nbins: int = 64):

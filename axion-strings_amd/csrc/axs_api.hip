@@ -132,6 +132,52 @@ int ensure_pinned(Context *c, size_t bytes)
     return AXS_OK;
 }
 
+struct HostCopy {
+    char *dst;
+    const char *src;
+};
+
+static void host_copy_fn(size_t lo, size_t hi, void *arg)
+{
+    HostCopy *h = (HostCopy *)arg;
+    memcpy(h->dst + lo, h->src + lo, hi - lo);
+}
+
+int copy_to_host(Context *c, void *dst, const void *src, size_t bytes)
+{
+    constexpr size_t kChunk = (size_t)64 << 20;
+    if (bytes <= kChunk) {
+        AXS_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+        AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+        return AXS_OK;
+    }
+    AXS_TRY(ensure_pinned(c, 2 * kChunk));
+    for (int h = 0; h < 2; ++h)
+        if (!c->stage_ev[h]) AXS_HIP_TRY(hipEventCreateWithFlags(&c->stage_ev[h],
+                                                                 hipEventDisableTiming));
+    char *stage[2] = {(char *)c->pinned, (char *)c->pinned + kChunk};
+    const size_t n = (bytes + kChunk - 1) / kChunk;
+    auto chunk_bytes = [&](size_t q) { return std::min(kChunk, bytes - q * kChunk); };
+    // chunk q goes to stage q % 2; it is copied out after chunk q+1 is queued
+    AXS_HIP_TRY(hipMemcpyAsync(stage[0], src, chunk_bytes(0), hipMemcpyDeviceToHost, c->stream));
+    AXS_HIP_TRY(hipEventRecord(c->stage_ev[0], c->stream));
+    for (size_t q = 0; q < n; ++q) {
+        const int h = (int)(q & 1);
+        if (q + 1 < n) {
+            const size_t nb = chunk_bytes(q + 1);
+            AXS_HIP_TRY(hipMemcpyAsync(stage[h ^ 1], (const char *)src + (q + 1) * kChunk, nb,
+                                       hipMemcpyDeviceToHost, c->stream));
+            AXS_HIP_TRY(hipEventRecord(c->stage_ev[h ^ 1], c->stream));
+        }
+        AXS_HIP_TRY(hipEventSynchronize(c->stage_ev[h]));
+        HostCopy hc{(char *)dst + q * kChunk, stage[h]};
+        parallel_for(chunk_bytes(q), host_copy_fn, &hc);
+        // stage h is free again only after its copy-out: the transfer of
+        // chunk q+2 into it is queued in the next iteration, after this one
+    }
+    return AXS_OK;
+}
+
 void parallel_for(size_t n, void (*fn)(size_t, size_t, void *), void *arg)
 {
     unsigned hw = std::thread::hardware_concurrency();
@@ -259,6 +305,8 @@ Context::~Context()
     if (pinned) (void)hipHostFree(pinned);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    for (int h = 0; h < 2; ++h)
+        if (stage_ev[h]) (void)hipEventDestroy(stage_ev[h]);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -273,14 +321,37 @@ int current_fields(Context *c, Member &m, FieldView *v)
         v->vel = m.state[c->cur][VEL];
         return AXS_OK;
     }
-    if (!m.vel_valid) {     // once per state: ker (if needed) and phidot
-        StepScalars s = make_scalars(c->p, c->tau);
-        AXS_TRY(launch_fp32_canonical(c, m, s, !m.ker_valid));
-        m.ker_valid = true;
-        m.vel_valid = true;
-    }
+    AXS_TRY(fp32_refresh_aux(c, m));
     v->vel = m.aux_vel;
     return AXS_OK;
+}
+
+int fp32_refresh_aux(Context *c, Member &m)
+{
+    const bool ker_ok = ker_current(c, m);
+    if (ker_ok && m.vel_valid) return AXS_OK;
+    StepScalars s = make_scalars(c->p, c->tau);
+    // a failed launch leaves both buffers unknown
+    m.vel_valid = false;
+    if (!ker_ok) m.ker_valid = false;
+    AXS_TRY(launch_fp32_canonical(c, m, s, !ker_ok));
+    if (!ker_ok) {
+        m.ker_valid = true;
+        m.ker_uploaded = false;
+        m.ker_tau = c->tau;
+    }
+    m.vel_valid = true;
+    return AXS_OK;
+}
+
+void state_changed(Context *c)
+{
+    for (Member &m : c->members) {
+        m.ker_valid = false;
+        m.ker_uploaded = false;
+        m.vel_valid = false;
+        m.version++;
+    }
 }
 
 static int stencil_radius(int stencil) { return (stencil >= 0 && stencil <= 3) ? stencil + 1 : 0; }
@@ -378,7 +449,6 @@ static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ct
                   prop.gcnArchName);
         return AXS_ERR_NO_DEVICE;
     }
-
     axs_ctx *h = new axs_ctx();
     Context *c = &h->impl;
     c->p = *p;
@@ -388,6 +458,7 @@ static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ct
     c->device = p->device;
     c->num_cu = prop.multiProcessorCount;
     c->tau = 1.0;                                              // physics.cpp:32
+    if (const char *f = getenv("AXS_STEP_FAIL_AFTER")) c->step_fail_after = atoi(f);
     int rc = AXS_OK;
     auto fail = [&](int code) {
         delete h;
@@ -540,6 +611,7 @@ int axs_set_member_state(axs_ctx *ctx, int member, const double *phi1, const dou
         parallel_for(L, packpair32_fn, &kp);
         AXS_HIP_TRY(hipMemcpy(m.aux_ker, st, L * 8, hipMemcpyHostToDevice));
         m.ker_valid = true;
+        m.ker_uploaded = true;
     }
     m.has_state = true;
     m.vel_valid = false;
@@ -610,10 +682,10 @@ int axs_set_tau(axs_ctx *ctx, double tau)
         set_error("null context");
         return AXS_ERR_INVALID_ARG;
     }
+    // FP32: a ker recomputed from phi (and the phidot rebuilt from it)
+    // belongs to the tau it was computed at (ker_current); an uploaded one
+    // does not depend on tau
     ctx->impl.tau = tau;
-    // FP32: a ker recomputed from phi depends on tau; an uploaded one does not
-    for (Member &m : ctx->impl.members)
-        if (!m.ker_valid) m.vel_valid = false;
     return AXS_OK;
 }
 
@@ -681,11 +753,18 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
     else
         AXS_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     int rc = AXS_OK;
+    int launches = 0;      // launches that returned AXS_OK (direct loop)
     for (int64_t n = 0; n < nsteps && rc == AXS_OK;) {
         double tau_next = c->tau + dt;                    // s_integrate.cpp:317
         StepScalars s = make_scalars(c->p, tau_next);
         const int64_t left = nsteps - n;
         const int d = (depth < 1) ? 0 : (left >= depth ? depth : remainder_depth(c, (int)left));
+        if (!graph && c->step_fail_after >= 0 && launches >= c->step_fail_after) {
+            set_error("axs_step: failure point after %d launches (AXS_STEP_FAIL_AFTER)",
+                      launches);
+            rc = AXS_ERR_HIP;
+            break;
+        }
         if (d >= 1) {
             StepScalars ss[kMaxDepth];
             double t = c->tau;
@@ -694,11 +773,17 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
                 ss[l] = make_scalars(c->p, t);
             }
             rc = launch_multi_steps(c, ss, d);
-            if (rc == AXS_OK) c->tau = t;
+            if (rc == AXS_OK) {
+                c->tau = t;
+                ++launches;
+            }
             n += d;
         } else {
             rc = launch_step(c, s);
-            if (rc == AXS_OK) c->tau = tau_next;
+            if (rc == AXS_OK) {
+                c->tau = tau_next;
+                ++launches;
+            }
             n += 1;
         }
     }
@@ -711,39 +796,58 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
             rc = AXS_ERR_HIP;
         }
     }
-    if (rc != AXS_OK) {
-        // a captured loop never ran: back to the state before the call; a
-        // direct loop keeps the steps that were launched
+    auto drop_graph = [&]() {
         if (ge) (void)hipGraphExecDestroy(ge);
         if (g) (void)hipGraphDestroy(g);
+        ge = nullptr;
+        g = nullptr;
+    };
+    if (rc != AXS_OK) {
+        // a captured loop never ran: back to the state before the call; a
+        // direct loop keeps the steps that were launched, and what was
+        // derived from the state before them is dropped
+        drop_graph();
         if (graph) {
             c->tau = tau0;
             c->cur = cur0;
+        } else if (launches > 0) {
+            state_changed(c);
         }
         return rc;
     }
     if (graph) {
-        AXS_HIP_TRY(hipEventRecord(c->ev0, c->stream));
-        hipError_t e = hipGraphLaunch(ge, c->stream);
+        hipError_t e = hipEventRecord(c->ev0, c->stream);
+        if (e == hipSuccess && c->step_fail_after >= 0) {
+            set_error("axs_step: failure point at the graph launch (AXS_STEP_FAIL_AFTER)");
+            e = hipErrorLaunchFailure;
+        } else if (e == hipSuccess) {
+            e = hipGraphLaunch(ge, c->stream);
+            if (e != hipSuccess) set_error("hipGraphLaunch failed: %s", hipGetErrorString(e));
+        } else {
+            set_error("hipEventRecord failed: %s", hipGetErrorString(e));
+        }
         if (e != hipSuccess) {
-            (void)hipGraphExecDestroy(ge);
-            (void)hipGraphDestroy(g);
-            set_error("hipGraphLaunch failed: %s", hipGetErrorString(e));
+            // the graph did not run: the capture's tau and ping-pong index
+            // are taken back
+            drop_graph();
+            c->tau = tau0;
+            c->cur = cur0;
             return AXS_ERR_HIP;
         }
     }
-    AXS_HIP_TRY(hipEventRecord(c->ev1, c->stream));
-    AXS_HIP_TRY(hipEventSynchronize(c->ev1));
-    if (ge) (void)hipGraphExecDestroy(ge);
-    if (g) (void)hipGraphDestroy(g);
+    // the launches are queued: the state is the new one from here on, also
+    // if waiting for them fails
+    state_changed(c);
+    hipError_t e = hipEventRecord(c->ev1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+    drop_graph();
     float ms = 0.f;
-    AXS_HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->last_step_ms = (double)ms / (double)nsteps;
-    for (Member &m : c->members) {
-        m.ker_valid = false;
-        m.vel_valid = false;
-        m.version++;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    if (e != hipSuccess) {
+        set_error("axs_step: waiting for the step launches failed: %s", hipGetErrorString(e));
+        return AXS_ERR_HIP;
     }
+    c->last_step_ms = (double)ms / (double)nsteps;
     return AXS_OK;
 }
 
@@ -760,6 +864,16 @@ int axs_member_strings(axs_ctx *ctx, int member, int32_t *coords, int64_t cap, i
 int axs_strings(axs_ctx *ctx, int32_t *coords, int64_t cap, int64_t *count)
 {
     return axs_member_strings(ctx, 0, coords, cap, count);
+}
+
+int axs_member_axion(axs_ctx *ctx, int member, double *axion)
+{
+    AXS_TRY(check_member(ctx, member));
+    if (!axion) {
+        set_error("null output");
+        return AXS_ERR_INVALID_ARG;
+    }
+    return run_axion(&ctx->impl, ctx->impl.members[member], axion);
 }
 
 int axs_string_counts(axs_ctx *ctx, int64_t *n_plaquettes, int64_t *num_cores)
@@ -805,6 +919,18 @@ int axs_member_powerspec(axs_ctx *ctx, int member, int binning, int nbins, doubl
         return AXS_ERR_INVALID_ARG;
     }
     return run_powerspec(&ctx->impl, ctx->impl.members[member], binning, nbins, ks, pk, count);
+}
+
+int axs_member_powerspec_field(axs_ctx *ctx, int member, int field, int binning, int nbins,
+                               double *ks, double *pk, int32_t *count)
+{
+    AXS_TRY(check_member(ctx, member));
+    if (!ks || !pk || !count) {
+        set_error("null output");
+        return AXS_ERR_INVALID_ARG;
+    }
+    return run_powerspec_field(&ctx->impl, ctx->impl.members[member], field, binning, nbins, ks,
+                               pk, count);
 }
 
 int axs_powerspec(axs_ctx *ctx, int binning, int nbins, double *ks, double *pk, int32_t *count)

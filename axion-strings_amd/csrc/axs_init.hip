@@ -339,7 +339,8 @@ int run_init(Context *c, Member &m, int mode, uint64_t seed)
     } else {
         hipLaunchKernelGGL(store_init32_kernel, dim3(g), dim3(256), 0, c->stream, L, phik, velk,
                            scale, (fvec4 *)m.state[h][PHI], (fvec2 *)m.aux_ker);
-        m.ker_valid = true;
+        m.ker_valid = true;      // zero, as after an upload without ker
+        m.ker_uploaded = true;
     }
     AXS_HIP_TRY(hipGetLastError());
     AXS_HIP_TRY(hipStreamSynchronize(c->stream));

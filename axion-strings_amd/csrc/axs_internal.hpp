@@ -87,7 +87,12 @@ struct Member {
     void *aux_ker = nullptr;
     void *aux_vel = nullptr;
     bool has_state = false;
+    // FP32: aux_ker holds ker of the current state.  An uploaded ker (or the
+    // zero of a fresh upload) belongs to the state whatever tau is; a ker
+    // recomputed from phi belongs to the tau it was computed at (ker_tau).
     bool ker_valid = true;
+    bool ker_uploaded = true;
+    double ker_tau = 0.0;
     bool vel_valid = false;   // FP32: aux_vel holds phidot of the current state
     uint64_t version = 0;     // bumped whenever the state changes
     // FP32: background sums and energy components of state `samp_version`
@@ -141,9 +146,14 @@ struct Context {
     Tune tune;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};   // copy_to_host chunk completions
     double last_step_ms = 0.0;
     int device = 0;
     int num_cu = 256;
+    // failure point for tests of the error paths: axs_step reports an error
+    // in place of its (n+1)-th launch (direct loop) or of the graph launch
+    // (AXS_STEP_GRAPH); -1 = off.  Read from AXS_STEP_FAIL_AFTER at creation.
+    int step_fail_after = -1;
 
     // device scratch for reductions / compaction (grown on demand; growing
     // discards the contents)
@@ -207,6 +217,10 @@ struct Context {
 int ensure_scratch(Context *c, size_t bytes);
 int ensure_device(Context *c, void **buf, size_t *have, size_t bytes);
 int ensure_pinned(Context *c, size_t bytes);
+// device-to-host copy of `bytes` into a caller's (pageable) buffer through
+// the pinned staging buffer, in chunks whose transfers overlap the host-side
+// copies of the previous chunk; synchronous on return
+int copy_to_host(Context *c, void *dst, const void *src, size_t bytes);
 
 // host-side parallel loop over [0, n) in contiguous chunks (std::thread)
 void parallel_for(size_t n, void (*fn)(size_t begin, size_t end, void *arg), void *arg);
@@ -281,8 +295,20 @@ int prepare_batch(Context *c);
 
 // FP32: compute ker = F(phi, tau) and phidot = u - dt/2 ker into aux buffers
 int launch_fp32_canonical(Context *c, Member &m, const StepScalars &s, bool recompute_ker);
+// FP32: aux_ker holds ker of the current state at the context's tau
+inline bool ker_current(const Context *c, const Member &m)
+{
+    return m.ker_valid && (m.ker_uploaded || m.ker_tau == c->tau);
+}
+// FP32: aux_ker and aux_vel rebuilt for the current state and tau (once per
+// state and tau; a kept uploaded ker is not recomputed)
+int fp32_refresh_aux(Context *c, Member &m);
+// the state of every member has changed: drop what was derived from it
+void state_changed(Context *c);
 
 int run_string_counts(Context *c, int64_t *n_plaquettes, int64_t *num_cores);
+// atan2(phi1, phi2) of every site of a member, rounded correctly, into a host array
+int run_axion(Context *c, Member &m, double *out);
 int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *count,
                 int64_t *num_cores);
 int run_background(Context *c, Member &m, double out[6]);
@@ -293,6 +319,8 @@ int run_powerspec_all(Context *c, int binning, int nbins, double *ks, double *pk
                       int32_t *count);
 int run_powerspec(Context *c, Member &m, int binning, int nbins, double *ks, double *pk,
                   int32_t *count);
+int run_powerspec_field(Context *c, Member &m, int field, int binning, int nbins, double *ks,
+                        double *pk, int32_t *count);
 
 // device pointers to the current fields of a member: phi (FP64 pairs or
 // FP32 quadruples) and phidot (FP64 pairs or FP32 pairs) -- for the FP32 form

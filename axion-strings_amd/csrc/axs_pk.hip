@@ -1,5 +1,7 @@
-// axs_pk.hip -- power spectrum of phi2 (reference output_powerspec,
-// src/common/pk_estimation.cpp:37-214, called on phi2 at s_run.cpp:206).
+// axs_pk.hip -- power spectra (reference output_powerspec,
+// src/common/pk_estimation.cpp:37-214, called on phi2 at s_run.cpp:206; any
+// stored real component or the complex field phi1 + i phi2 / phidot1 + i
+// phidot2 through axs_member_powerspec_field).
 //
 // The reference multiplies the field by (-1)^(i+j+k) and takes a complex
 // forward FFT of the whole N^d array, so that array index i corresponds to
@@ -104,18 +106,34 @@ static int get_plan(Context *c, int N, int batch, FftPlan **out)
     return AXS_OK;
 }
 
-// phi2 of the current states of a batch of lattices (one per blockIdx.y,
-// field pointers in src) as contiguous double arrays, lattice b at y + b L
-template <class P>
+// One real field per transform of a batch: a component of a stored pair.
+// kind 0: FP64 pairs (phi or phidot), 1: the FP32 (phi1, phi2, u1, u2)
+// quadruple (phi only), 2: FP32 pairs (the phidot read-back); comp 0 or 1.
+struct PkSrc {
+    const void *p;
+    int kind, comp;
+};
+
+// the selected component of every lattice of a batch (one per blockIdx.y)
+// as contiguous double arrays, lattice b at y + b L
 __global__ void __launch_bounds__(256)
-phi2_kernel(uint32_t L, const void *const *__restrict__ src, double *__restrict__ y)
+pack_kernel(uint32_t L, const PkSrc *__restrict__ src, double *__restrict__ y)
 {
     const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m < L) {
-        const P phi{(decltype(P::p))src[blockIdx.y]};
-        double a, b;
-        phi.get(m, a, b);
-        y[(size_t)blockIdx.y * L + m] = b;
+        const PkSrc s = src[blockIdx.y];
+        double v;
+        if (s.kind == 0) {
+            const dvec2 w = ((const dvec2 *)s.p)[m];
+            v = s.comp ? w.y : w.x;
+        } else if (s.kind == 1) {
+            const fvec2 w = ((const fvec2 *)s.p)[2 * (size_t)m];      // (phi1, phi2)
+            v = (double)(s.comp ? w.y : w.x);
+        } else {
+            const fvec2 w = ((const fvec2 *)s.p)[m];
+            v = (double)(s.comp ? w.y : w.x);
+        }
+        y[(size_t)blockIdx.y * L + m] = v;
     }
 }
 
@@ -246,6 +264,116 @@ bin_kernel(int N, const double2 *__restrict__ F, int nbins, const uint32_t *__re
     }
 }
 
+// Row-order binning (nbins <= kRowBinsMax): the half spectrum is read once,
+// in storage order.  Work-group blockIdx.x takes rows [r0, r0 + rows_per);
+// wave w of it rows r0 + w, r0 + w + kRowWaves, ..., lanes along z (64
+// consecutive modes per load).  A row's modes with |k| < N/2 fall in
+// bins that do not decrease along z, so a chunk of 64 spans bins from its
+// first valid lane's to its last's: each of those gets a fixed butterfly
+// sum of its lanes' |F|^2, |k| and counts, which lane 0 adds (compensated)
+// to the wave's accumulator for the bin in LDS.  At the end the waves'
+// accumulators are merged in wave order into the block's partials, stored
+// bin-major (part[(member * nbins + b) * nblk + block]) so that
+// bin_fold_kernel merges each bin's blocks in block order: deterministic.
+constexpr int kRowWaves = 4;
+constexpr int kRowBinsMax = 512;
+
+template <int ND>
+__global__ void __launch_bounds__(64 * kRowWaves)
+bin_rows_kernel(int N, const double2 *__restrict__ F, int nbins, const int64_t *__restrict__ hi2g,
+                uint32_t rows_per, uint32_t nrows, KahanP *__restrict__ part_pk,
+                KahanP *__restrict__ part_ks, long long *__restrict__ part_cnt)
+{
+    __shared__ KahanP apk[kRowWaves][kRowBinsMax], aks[kRowWaves][kRowBinsMax];
+    __shared__ long long acnt[kRowWaves][kRowBinsMax];
+    __shared__ int64_t hi2[kRowBinsMax];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int H = N / 2 + 1;
+    F += (size_t)blockIdx.y * (size_t)nrows * (size_t)H;
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
+        hi2[b] = hi2g[b];
+        for (int w = 0; w < kRowWaves; ++w) {
+            apk[w][b] = KahanP{0.0, 0.0};
+            aks[w][b] = KahanP{0.0, 0.0};
+            acnt[w][b] = 0;
+        }
+    }
+    __syncthreads();
+    const int64_t top = (int64_t)(N / 2) * (N / 2);           // |k| < N/2
+    const uint32_t r0 = blockIdx.x * rows_per;
+    const uint32_t r1 = (r0 + rows_per < nrows) ? r0 + rows_per : nrows;
+    for (uint32_t r = r0 + (uint32_t)wave; r < r1; r += kRowWaves) {
+        int64_t fi = 0, fj;
+        if (ND == 3) {
+            const int64_t ii = r / (uint32_t)N, jj = r % (uint32_t)N;
+            fi = ii < N / 2 ? ii : ii - N;
+            fj = jj < N / 2 ? jj : jj - N;
+        } else {
+            fj = (int64_t)r < N / 2 ? (int64_t)r : (int64_t)r - N;
+        }
+        const int64_t r2 = fi * fi + fj * fj;
+        if (fi == -N / 2 || fj == -N / 2 || r2 >= top) continue;   // uniform: |k| >= N/2
+        int64_t zmax = isqrt_ceil(top - r2);        // z < zmax  <=>  r2 + z^2 < top
+        if (zmax > H) zmax = H;
+        const int64_t zs = (r2 == 0) ? 1 : 0;       // not the k = 0 mode
+        int bcur = 0;
+        while (bcur < nbins - 1 && hi2[bcur] <= r2 + zs * zs) ++bcur;
+        const double2 *row = F + (size_t)r * (size_t)H;
+        for (int64_t z0 = 0; z0 < zmax; z0 += 64) {
+            const int64_t z = z0 + lane;
+            const bool on = z >= zs && z < zmax;
+            double p = 0.0, km = 0.0;
+            long long w = 0;
+            int b = bcur;
+            if (on) {
+                const double2 f = row[z];
+                const int64_t k2 = r2 + z * z;
+                w = (z == 0 || z == N / 2) ? 1 : 2;
+                p = (double)w * (f.x * f.x + f.y * f.y);
+                km = (double)w * sqrt((double)k2);
+                while (hi2[b] <= k2) ++b;            // k2 < top = hi2[nbins - 1]
+            }
+            const uint64_t act = __ballot(on);
+            if (!act) continue;
+            const int first = __builtin_ctzll(act), last = 63 - __builtin_clzll(act);
+            const int bmin = __builtin_amdgcn_readlane(b, first);
+            const int bmax = __builtin_amdgcn_readlane(b, last);
+            for (int bb = bmin; bb <= bmax; ++bb) {
+                const bool mine = on && b == bb;
+                double v = mine ? p : 0.0, u = mine ? km : 0.0;
+                long long c = mine ? w : 0;
+                for (int o = 32; o > 0; o >>= 1) {
+                    v += __shfl_xor(v, o, 64);
+                    u += __shfl_xor(u, o, 64);
+                    c += __shfl_xor(c, o, 64);
+                }
+                if (lane == 0) {
+                    kaddp(apk[wave][bb], v);
+                    kaddp(aks[wave][bb], u);
+                    acnt[wave][bb] += c;
+                }
+            }
+            bcur = bmax;
+        }
+    }
+    __syncthreads();
+    const size_t nblk = gridDim.x;
+    for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
+        KahanP a = apk[0][b], k = aks[0][b];
+        long long n = acnt[0][b];
+        for (int w = 1; w < kRowWaves; ++w) {
+            a = kmergep(a, apk[w][b]);
+            k = kmergep(k, aks[w][b]);
+            n += acnt[w][b];
+        }
+        const size_t slot = ((size_t)blockIdx.y * nbins + b) * nblk + blockIdx.x;
+        part_pk[slot] = a;
+        part_ks[slot] = k;
+        part_cnt[slot] = n;
+    }
+}
+
 // bin blockIdx.x of lattice blockIdx.y: work-item t merges the bin's items
 // t, t + 256, ... in order, then a fixed pairwise tree over the work-items
 // (deterministic for a given item count)
@@ -343,28 +471,34 @@ static int bin_layout(int N, int nd, int binning, int nbins, BinLayout &B)
     return AXS_OK;
 }
 
-// Spectra of the n members idx[0..n), all of side N, in one batched
-// transform: member idx[q]'s bins go to ks / pk / count + q nbins.
-static int pk_batch(Context *c, const int *idx, int n, int binning, int nbins_in, double *ks,
-                    double *pk, int32_t *count)
+// Raw bin sums (|F|^2, |k| and mode counts per bin, before the
+// normalisation) of n real fields of side N in one batched transform:
+// source q's bins go to hpk / hks / hcnt + q nbins.
+static int pk_raw(Context *c, const PkSrc *srcs, int n, int N, const BinLayout &B,
+                  std::vector<double> &hpk, std::vector<double> &hks,
+                  std::vector<long long> &hcnt)
 {
-    const int N = c->members[idx[0]].N;
-    BinLayout B;
-    AXS_TRY(bin_layout(N, c->nd, binning, nbins_in, B));
     const int nbins = B.nbins;
-    for (int q = 0; q < n; ++q)
-        if (!c->members[idx[q]].has_state) {
-            set_error("member has no state: call axs_set_member_state first");
-            return AXS_ERR_STATE;
-        }
     FftPlan *plan = nullptr;
     AXS_TRY(get_plan(c, N, n, &plan));
-    const uint32_t L = (uint32_t)c->members[idx[0]].L;
+    const uint32_t L = (uint32_t)((c->nd == 3) ? (int64_t)N * N * N : (int64_t)N * N);
 
     const size_t nb = (size_t)nbins;
-    const uint32_t nitems = B.items[nb];
+    // row-order binning where the bins' accumulators fit in LDS: nblk
+    // work-groups per lattice, each bin's items are its blocks' partials
+    const bool by_rows = nbins <= kRowBinsMax;
+    const uint32_t nrows = (c->nd == 3) ? (uint32_t)N * (uint32_t)N : (uint32_t)N;
+    const uint32_t nblk = std::min<uint32_t>(2048u, (nrows + 63) / 64);
+    const uint32_t rows_per = (nrows + nblk - 1) / nblk;
+    std::vector<uint32_t> row_items;
+    if (by_rows) {
+        row_items.resize(nb + 1);
+        for (size_t b = 0; b <= nb; ++b) row_items[b] = (uint32_t)b * nblk;
+    }
+    const std::vector<uint32_t> &items = by_rows ? row_items : B.items;
+    const uint32_t nitems = items[nb];
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t o_src = 0, o_lo = align((size_t)n * 8), o_hi = o_lo + align(nb * 8);
+    const size_t o_src = 0, o_lo = align((size_t)n * sizeof(PkSrc)), o_hi = o_lo + align(nb * 8);
     const size_t o_ib = o_hi + align(nb * 8);
     const size_t o_ppk = o_ib + align((nb + 1) * 4);
     const size_t o_pks = o_ppk + align((size_t)n * nitems * 16);
@@ -375,7 +509,7 @@ static int pk_batch(Context *c, const int *idx, int n, int binning, int nbins_in
     const size_t need = o_cnt + align((size_t)n * nb * 8);
     AXS_TRY(ensure_scratch(c, need));
     char *sp = (char *)c->scratch;
-    const void **dsrc = (const void **)(sp + o_src);
+    PkSrc *dsrc = (PkSrc *)(sp + o_src);
     int64_t *dlo = (int64_t *)(sp + o_lo);
     int64_t *dhi = (int64_t *)(sp + o_hi);
     uint32_t *dib = (uint32_t *)(sp + o_ib);
@@ -386,16 +520,10 @@ static int pk_batch(Context *c, const int *idx, int n, int binning, int nbins_in
     double *dks = (double *)(sp + o_ks);
     long long *dcnt = (long long *)(sp + o_cnt);
 
-    // pack phi2 of every member (phi only: no FP32 velocity reconstruction)
-    std::vector<const void *> src(n);
-    for (int q = 0; q < n; ++q) src[q] = c->members[idx[q]].state[c->cur][PHI];
-    AXS_HIP_TRY(hipMemcpyAsync(dsrc, src.data(), (size_t)n * 8, hipMemcpyHostToDevice,
+    AXS_HIP_TRY(hipMemcpyAsync(dsrc, srcs, (size_t)n * sizeof(PkSrc), hipMemcpyHostToDevice,
                                c->stream));
-    const dim3 gp((L + 255) / 256, (unsigned)n);
-    if (c->prec == AXS_FP64)
-        hipLaunchKernelGGL(phi2_kernel<PhiF64>, gp, dim3(256), 0, c->stream, L, dsrc, plan->in);
-    else
-        hipLaunchKernelGGL(phi2_kernel<PhiF32>, gp, dim3(256), 0, c->stream, L, dsrc, plan->in);
+    hipLaunchKernelGGL(pack_kernel, dim3((L + 255) / 256, (unsigned)n), dim3(256), 0, c->stream,
+                       L, dsrc, plan->in);
     AXS_HIP_TRY(hipGetLastError());
     if (rocfft_execution_info_set_stream(plan->info, c->stream) != rocfft_status_success) {
         set_error("rocfft_execution_info_set_stream failed");
@@ -410,9 +538,14 @@ static int pk_batch(Context *c, const int *idx, int n, int binning, int nbins_in
 
     AXS_HIP_TRY(hipMemcpyAsync(dlo, B.lo.data(), nb * 8, hipMemcpyHostToDevice, c->stream));
     AXS_HIP_TRY(hipMemcpyAsync(dhi, B.hi.data(), nb * 8, hipMemcpyHostToDevice, c->stream));
-    AXS_HIP_TRY(hipMemcpyAsync(dib, B.items.data(), (nb + 1) * 4, hipMemcpyHostToDevice,
+    AXS_HIP_TRY(hipMemcpyAsync(dib, items.data(), (nb + 1) * 4, hipMemcpyHostToDevice,
                                c->stream));
-    if (nitems > 0) {
+    if (by_rows) {
+        auto k = (c->nd == 3) ? bin_rows_kernel<3> : bin_rows_kernel<2>;
+        hipLaunchKernelGGL(k, dim3(nblk, (unsigned)n), dim3(64 * kRowWaves), 0, c->stream, N,
+                           (const double2 *)plan->out, nbins, dhi, rows_per, nrows, ppk, pks,
+                           pcnt);
+    } else if (nitems > 0) {
         auto k = (c->nd == 3) ? bin_kernel<3> : bin_kernel<2>;
         hipLaunchKernelGGL(k, dim3(nitems, (unsigned)n), dim3(kPkRows), 0, c->stream, N,
                            (const double2 *)plan->out, nbins, dib, dlo, dhi, ppk, pks, pcnt);
@@ -420,8 +553,9 @@ static int pk_batch(Context *c, const int *idx, int n, int binning, int nbins_in
     hipLaunchKernelGGL(bin_fold_kernel, dim3((unsigned)nbins, (unsigned)n), dim3(256), 0,
                        c->stream, nbins, nitems, dib, ppk, pks, pcnt, dpk, dks, dcnt);
     AXS_HIP_TRY(hipGetLastError());
-    std::vector<double> hpk((size_t)n * nb), hks((size_t)n * nb);
-    std::vector<long long> hcnt((size_t)n * nb);
+    hpk.assign((size_t)n * nb, 0.0);
+    hks.assign((size_t)n * nb, 0.0);
+    hcnt.assign((size_t)n * nb, 0);
     AXS_HIP_TRY(hipMemcpyAsync(hpk.data(), dpk, hpk.size() * 8, hipMemcpyDeviceToHost,
                                c->stream));
     AXS_HIP_TRY(hipMemcpyAsync(hks.data(), dks, hks.size() * 8, hipMemcpyDeviceToHost,
@@ -429,28 +563,109 @@ static int pk_batch(Context *c, const int *idx, int n, int binning, int nbins_in
     AXS_HIP_TRY(hipMemcpyAsync(hcnt.data(), dcnt, hcnt.size() * 8, hipMemcpyDeviceToHost,
                                c->stream));
     AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+    return AXS_OK;
+}
 
+// the reference's normalisation of one spectrum's raw bin sums
+static void pk_finalize(const Context *c, int N, int binning, int nbins, const double *rpk,
+                        const double *rks, const long long *rcnt, double *ks, double *pk,
+                        int32_t *count)
+{
     float Lbox = N * c->p.space_step;                           // pk_estimation.cpp:41
-    for (size_t e = 0; e < (size_t)n * nb; ++e) {
-        count[e] = (int32_t)hcnt[e];
-        if (hcnt[e] == 0) {
+    for (int e = 0; e < nbins; ++e) {
+        count[e] = (int32_t)rcnt[e];
+        if (rcnt[e] == 0) {
             ks[e] = 0.0;
             pk[e] = 0.0;
             continue;
         }
-        double kbar = hks[e] / (double)hcnt[e];                 // :118 / :196
-        double p = hpk[e];
+        double kbar = rks[e] / (double)rcnt[e];                 // :118 / :196
+        double p = rpk[e];
         if (binning == AXS_BIN_LINEAR) {
             if (c->nd == 2)
                 p /= 2.0f * M_PI * kbar;                        // :119
             else
                 p /= 4.0f * M_PI * (kbar * kbar);               // :197
         } else {
-            p /= (double)hcnt[e];
+            p /= (double)rcnt[e];
         }
         ks[e] = kbar * (2.0f * M_PI / Lbox);                    // :120 / :198
         pk[e] = p;
     }
+}
+
+// source of a stored component: field 0/1 = phi1/phi2 (the state), 2/3 =
+// phidot1/phidot2 (FP32: the read-back rebuilt from u)
+static int pk_source(Context *c, Member &m, int comp, PkSrc *out)
+{
+    if (!m.has_state) {
+        set_error("member has no state: call axs_set_member_state first");
+        return AXS_ERR_STATE;
+    }
+    if (comp < 2) {
+        *out = PkSrc{m.state[c->cur][PHI], c->prec == AXS_FP64 ? 0 : 1, comp};
+        return AXS_OK;
+    }
+    FieldView fv;
+    AXS_TRY(current_fields(c, m, &fv));
+    *out = PkSrc{fv.vel, c->prec == AXS_FP64 ? 0 : 2, comp - 2};
+    return AXS_OK;
+}
+
+// Spectra of the n members idx[0..n), all of side N, in one batched
+// transform (phi2, s_run.cpp:206): member idx[q]'s bins go to
+// ks / pk / count + q nbins.
+static int pk_batch(Context *c, const int *idx, int n, int binning, int nbins_in, double *ks,
+                    double *pk, int32_t *count)
+{
+    const int N = c->members[idx[0]].N;
+    BinLayout B;
+    AXS_TRY(bin_layout(N, c->nd, binning, nbins_in, B));
+    std::vector<PkSrc> srcs(n);
+    for (int q = 0; q < n; ++q) AXS_TRY(pk_source(c, c->members[idx[q]], 1, &srcs[q]));
+    std::vector<double> hpk, hks;
+    std::vector<long long> hcnt;
+    AXS_TRY(pk_raw(c, srcs.data(), n, N, B, hpk, hks, hcnt));
+    const size_t nb = (size_t)B.nbins;
+    for (int q = 0; q < n; ++q)
+        pk_finalize(c, N, binning, B.nbins, hpk.data() + q * nb, hks.data() + q * nb,
+                    hcnt.data() + q * nb, ks + q * nb, pk + q * nb, count + q * nb);
+    return AXS_OK;
+}
+
+// The spectrum of one field of a member (output_powerspec(re, im),
+// pk_estimation.cpp:37-214): a real component, or the complex phi1 + i
+// phi2 / phidot1 + i phidot2.  Every mode the bins take has |k| < N/2, so
+// its partner -k is taken too; for a complex field f = a + i b the pair
+// holds |F(k)|^2 + |F(-k)|^2 = 2 (|A(k)|^2 + |B(k)|^2), so a bin's sum is
+// the sum of the two real components' sums: two real transforms in one
+// batch, their raw sums added before the normalisation.
+int run_powerspec_field(Context *c, Member &m, int field, int binning, int nbins, double *ks,
+                        double *pk, int32_t *count)
+{
+    int comps[2], nc;
+    switch (field) {
+    case AXS_FIELD_PHI2: comps[0] = 1; nc = 1; break;
+    case AXS_FIELD_PHI1: comps[0] = 0; nc = 1; break;
+    case AXS_FIELD_PHI: comps[0] = 0; comps[1] = 1; nc = 2; break;
+    case AXS_FIELD_PHIDOT1: comps[0] = 2; nc = 1; break;
+    case AXS_FIELD_PHIDOT2: comps[0] = 3; nc = 1; break;
+    case AXS_FIELD_PHIDOT: comps[0] = 2; comps[1] = 3; nc = 2; break;
+    default:
+        set_error("powerspec: unknown field %d", field);
+        return AXS_ERR_INVALID_ARG;
+    }
+    BinLayout B;
+    AXS_TRY(bin_layout(m.N, c->nd, binning, nbins, B));
+    PkSrc srcs[2];
+    for (int q = 0; q < nc; ++q) AXS_TRY(pk_source(c, m, comps[q], &srcs[q]));
+    std::vector<double> hpk, hks;
+    std::vector<long long> hcnt;
+    AXS_TRY(pk_raw(c, srcs, nc, m.N, B, hpk, hks, hcnt));
+    const size_t nb = (size_t)B.nbins;
+    if (nc == 2)
+        for (size_t e = 0; e < nb; ++e) hpk[e] += hpk[nb + e];
+    pk_finalize(c, m.N, binning, B.nbins, hpk.data(), hks.data(), hcnt.data(), ks, pk, count);
     return AXS_OK;
 }
 

@@ -22,6 +22,7 @@ namespace axs {
 constexpr int kRedBlock = 256;
 constexpr int kRedMaxBlocks = 2048;
 
+
 struct Kahan {
     double s, c;
 };
@@ -176,7 +177,13 @@ energy_kernel(int N, int CI, uint32_t ncols, P phi, V vel, double tau2, double T
 // -- so the sums run over the fields a read-back gives.  A work-item
 // marches columns along the slowest axis (as energy_kernel), keeping phi of
 // planes sl-R .. sl+R in a register ring and loading the next plane's
-// operands while the current one is summed.
+// operands while the current one is summed.  A column's sites (at most 64
+// planes) are summed in plain FP64 -- within 63 ulp of the sum of their
+// magnitudes -- and each column sum enters the work-item's compensated sums:
+// the per-site compensated additions were a quarter of the kernel's vector
+// instructions.  In 3D with N a multiple of 64 a work-group's 256 columns
+// are 4 rows (j) of 64 columns (k), so the j +- 1 neighbours of its middle
+// rows are its own rows' cache lines.
 constexpr int kSampQ = 9;     // background (6), then kinetic, gradient, potential
 
 template <int ND, int R>
@@ -205,9 +212,17 @@ sample_fp32_kernel(int N, int CI, uint32_t ncols, const fvec4 *__restrict__ in,
         fvec4 c;
         fvec2 a[2 * R], b[2 * R];
     };
+    const bool tiled = ND == 3 && (N % 64) == 0;
     for (uint32_t col = blk * kRedBlock + threadIdx.x; col < ncols; col += stride) {
-        const uint32_t q = col % plane;
+        uint32_t q = col % plane;
+        if (tiled) {            // 4 x 64 tiles: band of 4 rows, then 64 columns
+            const uint32_t tile = q >> 8, w = q & 255u, tpr = (uint32_t)N >> 6;
+            q = ((tile / tpr) * 4u + (w >> 6)) * (uint32_t)N + (tile % tpr) * 64u + (w & 63u);
+        }
         const int s0 = (int)(col / plane) * CI, s1 = (s0 + CI < N) ? s0 + CI : N;
+        double acc[kSampQ];     // plain sums over this column's sites
+#pragma unroll
+        for (int t = 0; t < kSampQ; ++t) acc[t] = 0.0;
         // in-plane neighbours at distance l (+l, -l): 3D axis 1 (j) and
         // axis 2 (k); 2D axis 0 (i).  Index 0 is the forward neighbour.
         uint32_t ap[R], am[R], bp[R], bm[R];
@@ -294,12 +309,12 @@ sample_fp32_kernel(int N, int CI, uint32_t ncols, const fvec4 *__restrict__ in,
             }
             const float vx = c.z - s.half_dt * kf.x, vy = c.w - s.half_dt * kf.y;
             const double a = c.x, b = c.y, va = vx, vb = vy;
-            kadd(k[0], a);                               // s_run.cpp:141-146
-            kadd(k[1], b);
-            kadd(k[2], va);
-            kadd(k[3], vb);
-            kadd(k[4], fabs(atan2(a, b)));
-            kadd(k[5], sqrt(a * a + b * b));
+            acc[0] += a;                                 // s_run.cpp:141-146
+            acc[1] += b;
+            acc[2] += va;
+            acc[3] += vb;
+            acc[4] += fabs(atan2(a, b));
+            acc[5] += sqrt(a * a + b * b);
             // forward differences in energy_kernel's axis order
             const fvec2 fs = ring[R + 1], fa = cur.a[0];
             double g = 0.0, d1, d2;
@@ -318,14 +333,16 @@ sample_fp32_kernel(int N, int CI, uint32_t ncols, const fvec4 *__restrict__ in,
                 g += d1 * d1 + d2 * d2;
             }
             const double S = a * a + b * b - tau2 + T03;
-            kadd(k[6], 0.5 * (va * va + vb * vb));
-            kadd(k[7], 0.5 * g);
-            kadd(k[8], lam_4tau2 * S * S);
+            acc[6] += 0.5 * (va * va + vb * vb);
+            acc[7] += 0.5 * g;
+            acc[8] += lam_4tau2 * S * S;
 #pragma unroll
             for (int t2 = 0; t2 < 2 * R; ++t2) ring[t2] = ring[t2 + 1];
             ring[2 * R] = ring_next;
             cur = nxt;
         }
+#pragma unroll
+        for (int t = 0; t < kSampQ; ++t) kadd(k[t], acc[t]);
     }
     block_reduce<kSampQ>(k, sh);
     if (threadIdx.x == 0)
@@ -391,7 +408,7 @@ static int fp32_sums(Context *c, Member &m, double out[kSampQ])
     auto k = (c->nd == 2) ? sample_kernel_for<2>(R) : sample_kernel_for<3>(R);
     hipLaunchKernelGGL(k, dim3(nb), dim3(kRedBlock), 0, c->stream, N, CI, ncols,
                        (const fvec4 *)m.state[c->cur][PHI], (const fvec2 *)m.aux_ker, f,
-                       m.ker_valid ? 0 : 1, tau2, T03, lam_4tau2, part);
+                       ker_current(c, m) ? 0 : 1, tau2, T03, lam_4tau2, part);
     hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(64), 0, c->stream, part, nb, kSampQ, dout);
     AXS_HIP_TRY(hipGetLastError());
     AXS_HIP_TRY(hipMemcpyAsync(m.samp, dout, kSampQ * sizeof(double), hipMemcpyDeviceToHost,

@@ -122,13 +122,10 @@ int snapshot_begin(Context *c, Member &m, int fields, int64_t count)
             dst += per_field;
         }
     } else {
-        if ((fields & (AXS_SNAP_VEL | AXS_SNAP_KER)) && !m.vel_valid) {
-            // canonical phidot and ker of the current state
-            StepScalars s = make_scalars(c->p, c->tau);
-            int rc = launch_fp32_canonical(c, m, s, !m.ker_valid);
+        if (fields & (AXS_SNAP_VEL | AXS_SNAP_KER)) {
+            // canonical phidot and ker of the current state and tau
+            int rc = fp32_refresh_aux(c, m);
             if (rc != AXS_OK) return fail(rc);
-            m.ker_valid = true;
-            m.vel_valid = true;
         }
         if (fields & AXS_SNAP_PHI) {
             hipLaunchKernelGGL(pack_phi32_kernel, dim3((n + 255) / 256), dim3(256), 0, c->stream,

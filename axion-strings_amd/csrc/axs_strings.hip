@@ -21,6 +21,7 @@
 // was searched by the previous call.
 #include <algorithm>
 
+#include "axs_atan2.hpp"
 #include "axs_internal.hpp"
 #include "axs_lattice.hpp"
 
@@ -50,9 +51,20 @@ __device__ __forceinline__ bool plaquettes3(double n0, double ni, double nij, do
     return sa > accept || sb > accept || sc > accept;
 }
 
-// the reference's axion (s_run.cpp:94) scaled to [0, 1]
-// (string_finding.cpp:14-16)
+// The reference's axion (s_run.cpp:94) scaled to [0, 1]
+// (string_finding.cpp:14-16).  scaled_angle: libm's atan2 rounded
+// correctly (the double-double evaluation of axs_atan2.hpp, bit for bit the
+// value the host build and the oracle give); it decides every flag that
+// depends on the last bits of an angle.  scaled_angle_fast: the device
+// library's atan2 (within a few ulp of it), for the 2D angle pass whose
+// plaquettes near the threshold are re-evaluated with scaled_angle.
 __device__ __forceinline__ double scaled_angle(double a1, double a2)
+{
+    double a = atan2_dd(a1, a2);
+    return (a + M_PI) / (2 * M_PI);
+}
+
+__device__ __forceinline__ double scaled_angle_fast(double a1, double a2)
 {
     double a = atan2(a1, a2);
     return (a + M_PI) / (2 * M_PI);
@@ -66,29 +78,64 @@ angle_kernel(uint32_t L, P phi, double *__restrict__ ang)
     if (m < L) {
         double a1, a2;
         phi.get(m, a1, a2);
-        ang[m] = scaled_angle(a1, a2);
+        ang[m] = scaled_angle_fast(a1, a2);
     }
 }
 
+// the reference's axion array (s_run.cpp:92-94): atan2(phi1, phi2) of every
+// site, rounded correctly
+template <class P>
+__global__ void __launch_bounds__(256)
+axion_kernel(uint32_t L, P phi, double *__restrict__ out)
+{
+    uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < L) {
+        double a1, a2;
+        phi.get(m, a1, a2);
+        out[m] = atan2_dd(a1, a2);
+    }
+}
+
+// A three-edge sum from the fast angles (each within a few 1e-16 of the
+// correctly rounded one, so the sum within ~1e-15) decides the flag unless
+// it lies within this margin of the threshold; there the four corners'
+// angles are evaluated again with scaled_angle.
+constexpr double kAngleMargin = 0x1p-30;
+
 // 2D plaquette of site m (string_finding.cpp:17-20): corners (i,j),
 // (i+1,j), (i+1,j+1), (i,j+1), three edges
-__device__ __forceinline__ bool plaquette_flag2(const double *__restrict__ ang, uint32_t m, int N,
-                                                double accept)
+template <class P>
+__device__ __forceinline__ bool plaquette_flag2(const double *__restrict__ ang, const P &phi,
+                                                uint32_t m, int N, double accept)
 {
     const Site<2> s = decompose<2>(m, N);
     const uint32_t q2 = shifted<2>(s, 0, 1, N);
     const Site<2> s2 = decompose<2>(q2, N);
     const uint32_t q3 = shifted<2>(s2, 1, 1, N);
     const uint32_t q4 = shifted<2>(s, 1, 1, N);
-    const double n1 = ang[m], n2 = ang[q2], n3 = ang[q3], n4 = ang[q4];
-    const double sum = edge_theta(n1, n2) + edge_theta(n2, n3) + edge_theta(n3, n4);
+    double n1 = ang[m], n2 = ang[q2], n3 = ang[q3], n4 = ang[q4];
+    double sum = edge_theta(n1, n2) + edge_theta(n2, n3) + edge_theta(n3, n4);
+    if (fabs(sum - accept) < kAngleMargin) {
+        auto exact = [&](uint32_t q) {
+            double a1, a2;
+            phi.get(q, a1, a2);
+            return scaled_angle(a1, a2);
+        };
+        n1 = exact(m);
+        n2 = exact(q2);
+        n3 = exact(q3);
+        n4 = exact(q4);
+        sum = edge_theta(n1, n2) + edge_theta(n2, n3) + edge_theta(n3, n4);
+    }
     return sum > accept;
 }
 
 // 2D plaquette test of block `blk` (kStrSpan sites) of one lattice from
 // its angles: one ballot word per wave and pass, the block's count in bcount
+template <class P>
 __device__ __forceinline__ void flag_block(int N, uint32_t L, const double *__restrict__ ang,
-                                           double accept, uint64_t *__restrict__ words,
+                                           const P &phi, double accept,
+                                           uint64_t *__restrict__ words,
                                            uint32_t *__restrict__ bcount, uint32_t blk)
 {
     __shared__ uint32_t wave_tot[kStrBlock / 64];
@@ -97,7 +144,7 @@ __device__ __forceinline__ void flag_block(int N, uint32_t L, const double *__re
     uint32_t base = blk * (uint32_t)kStrSpan;
     for (int it = 0; it < kStrIters; ++it) {
         uint32_t m = base + it * kStrBlock + threadIdx.x;
-        bool f = (m < L) && plaquette_flag2(ang, m, N, accept);
+        bool f = (m < L) && plaquette_flag2(ang, phi, m, N, accept);
         uint64_t w = __ballot(f);
         uint32_t wi = (base + it * kStrBlock) / 64 + wave;
         if (lane == 0 && (uint64_t)wi * 64 < L) words[wi] = w;
@@ -112,11 +159,12 @@ __device__ __forceinline__ void flag_block(int N, uint32_t L, const double *__re
     }
 }
 
+template <class P>
 __global__ void __launch_bounds__(kStrBlock)
-strings_flag_kernel(int N, uint32_t L, const double *__restrict__ ang, double accept,
+strings_flag_kernel(int N, uint32_t L, const double *__restrict__ ang, P phi, double accept,
                     uint64_t *__restrict__ words, uint32_t *__restrict__ bcount)
 {
-    flag_block(N, L, ang, accept, words, bcount, blockIdx.x);
+    flag_block(N, L, ang, phi, accept, words, bcount, blockIdx.x);
 }
 
 // Ensemble-wide 2D finder: member r owns blocks [first_block, first_block +
@@ -144,19 +192,22 @@ angle_batch_kernel(const StrMember *__restrict__ mem, const uint32_t *__restrict
         if (m < M.L) {
             double a1, a2;
             phi.get(m, a1, a2);
-            ang[M.site_base + m] = scaled_angle(a1, a2);
+            ang[M.site_base + m] = scaled_angle_fast(a1, a2);
         }
     }
 }
 
+template <class P>
 __global__ void __launch_bounds__(kStrBlock)
 strings_flag_batch_kernel(const StrMember *__restrict__ mem, const uint32_t *__restrict__ blk2mem,
-                          const double *__restrict__ ang, double accept,
+                          int half, const double *__restrict__ ang, double accept,
                           uint64_t *__restrict__ words, uint32_t *__restrict__ bcount)
 {
     const StrMember M = mem[blk2mem[blockIdx.x]];
+    P phi;
+    phi.p = (decltype(phi.p))M.phi[half];
     const size_t wbase = (size_t)M.first_block * (kStrSpan / 64);
-    flag_block(M.N, M.L, ang + M.site_base, accept, words + wbase, bcount + M.first_block,
+    flag_block(M.N, M.L, ang + M.site_base, phi, accept, words + wbase, bcount + M.first_block,
                blockIdx.x - M.first_block);
 }
 
@@ -674,16 +725,40 @@ static int search_member(Context *c, Member &m, uint64_t *words, uint32_t *bcoun
     AXS_TRY(ensure_scratch(c, (size_t)L * sizeof(double)));
     double *ang = (double *)c->scratch;
     const unsigned ga = (L + 255) / 256;
-    if (c->prec == AXS_FP64)
-        hipLaunchKernelGGL(angle_kernel<PhiF64>, dim3(ga), dim3(256), 0, c->stream, L,
-                           PhiF64{(const dvec2 *)phi}, ang);
-    else
-        hipLaunchKernelGGL(angle_kernel<PhiF32>, dim3(ga), dim3(256), 0, c->stream, L,
-                           PhiF32{(const fvec4 *)phi}, ang);
-    hipLaunchKernelGGL(strings_flag_kernel, dim3(nblocks), dim3(kStrBlock), 0, c->stream,
-                       m.N, L, ang, accept, words, bcount);
+    if (c->prec == AXS_FP64) {
+        const PhiF64 p{(const dvec2 *)phi};
+        hipLaunchKernelGGL(angle_kernel<PhiF64>, dim3(ga), dim3(256), 0, c->stream, L, p, ang);
+        hipLaunchKernelGGL(strings_flag_kernel<PhiF64>, dim3(nblocks), dim3(kStrBlock), 0,
+                           c->stream, m.N, L, ang, p, accept, words, bcount);
+    } else {
+        const PhiF32 p{(const fvec4 *)phi};
+        hipLaunchKernelGGL(angle_kernel<PhiF32>, dim3(ga), dim3(256), 0, c->stream, L, p, ang);
+        hipLaunchKernelGGL(strings_flag_kernel<PhiF32>, dim3(nblocks), dim3(kStrBlock), 0,
+                           c->stream, m.N, L, ang, p, accept, words, bcount);
+    }
     AXS_HIP_TRY(hipGetLastError());
     return AXS_OK;
+}
+
+int run_axion(Context *c, Member &m, double *out)
+{
+    if (!m.has_state) {
+        set_error("member has no state: call axs_set_member_state first");
+        return AXS_ERR_STATE;
+    }
+    const uint32_t L = (uint32_t)m.L;
+    AXS_TRY(ensure_scratch(c, (size_t)L * sizeof(double)));
+    double *d = (double *)c->scratch;
+    const void *phi = m.state[c->cur][PHI];
+    const unsigned g = (L + 255) / 256;
+    if (c->prec == AXS_FP64)
+        hipLaunchKernelGGL(axion_kernel<PhiF64>, dim3(g), dim3(256), 0, c->stream, L,
+                           PhiF64{(const dvec2 *)phi}, d);
+    else
+        hipLaunchKernelGGL(axion_kernel<PhiF32>, dim3(g), dim3(256), 0, c->stream, L,
+                           PhiF32{(const fvec4 *)phi}, d);
+    AXS_HIP_TRY(hipGetLastError());
+    return copy_to_host(c, out, d, (size_t)L * sizeof(double));
 }
 
 int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *count,
@@ -775,8 +850,7 @@ int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *co
     if (coords && cap > 0) {
         int64_t ncopy = (cap < (int64_t)total) ? cap : (int64_t)total;
         if (ncopy > 0)
-            AXS_HIP_TRY(hipMemcpyAsync(coords, dlist, (size_t)ncopy * c->nd * sizeof(int32_t),
-                                       hipMemcpyDeviceToHost, c->stream));
+            AXS_TRY(copy_to_host(c, coords, dlist, (size_t)ncopy * c->nd * sizeof(int32_t)));
         AXS_HIP_TRY(hipStreamSynchronize(c->stream));
     }
     return (coords == nullptr || cap >= (int64_t)total) ? AXS_OK : AXS_ERR_CAPACITY;
@@ -825,14 +899,18 @@ static int run_string_counts_2d_batch(Context *c, int64_t *n_plaquettes, int64_t
     unsigned long long *counters = (unsigned long long *)(sp + words_b + cnt_b);
     double *ang = (double *)c->scratch;
     AXS_HIP_TRY(hipMemsetAsync(counters, 0, nm * 16, c->stream));
-    if (c->prec == AXS_FP64)
+    const double accept = accept_threshold(c->p.thr);
+    if (c->prec == AXS_FP64) {
         hipLaunchKernelGGL(angle_batch_kernel<PhiF64>, dim3(nblocks), dim3(kStrBlock), 0,
                            c->stream, dmem, dblk, c->cur, ang);
-    else
+        hipLaunchKernelGGL(strings_flag_batch_kernel<PhiF64>, dim3(nblocks), dim3(kStrBlock), 0,
+                           c->stream, dmem, dblk, c->cur, ang, accept, words, bcount);
+    } else {
         hipLaunchKernelGGL(angle_batch_kernel<PhiF32>, dim3(nblocks), dim3(kStrBlock), 0,
                            c->stream, dmem, dblk, c->cur, ang);
-    hipLaunchKernelGGL(strings_flag_batch_kernel, dim3(nblocks), dim3(kStrBlock), 0, c->stream,
-                       dmem, dblk, ang, accept_threshold(c->p.thr), words, bcount);
+        hipLaunchKernelGGL(strings_flag_batch_kernel<PhiF32>, dim3(nblocks), dim3(kStrBlock), 0,
+                           c->stream, dmem, dblk, c->cur, ang, accept, words, bcount);
+    }
     hipLaunchKernelGGL(count_batch_kernel, dim3((nblocks + 7) / 8), dim3(256), 0, c->stream, dmem,
                        dblk, nblocks, words, counters);
     AXS_HIP_TRY(hipGetLastError());

@@ -144,8 +144,15 @@ int axs_step(axs_ctx *ctx, int64_t nsteps);
 
 /* ---- strings ----------------------------------------------------------- */
 
+/* The axion field of member `member`: axion[m] = atan2(phi1[m], phi2[m])
+ * for every site (s_run.cpp:92-94, 187-189, the all_data axion array),
+ * rounded correctly to double (libm's atan2 is within an ulp of this; which
+ * last bit it returns depends on the library).  axion has N^ndims entries. */
+int axs_member_axion(axs_ctx *ctx, int member, double *axion);
+
 /* cores2 / cores3 (string_finding.cpp:3-86) on axion = atan2(phi1, phi2)
- * (s_run.cpp:92-94) of member `member`.  Writes up to `cap` (i,j) or
+ * (s_run.cpp:92-94) of member `member`, with the correctly rounded angles
+ * of axs_member_axion.  Writes up to `cap` (i,j) or
  * (i,j,k) int32 tuples in ascending site order (the reference's single-
  * thread order) to coords (may be NULL) and the number of flagged sites to
  * *count.  Returns AXS_ERR_CAPACITY (with *count set) if cap is too small. */
@@ -188,6 +195,22 @@ int axs_member_powerspec(axs_ctx *ctx, int member, int binning, int nbins,
                          double *ks, double *pk, int32_t *count);
 int axs_powerspec(axs_ctx *ctx, int binning, int nbins,
                   double *ks, double *pk, int32_t *count);
+
+/* output_powerspec(re, im) of one field of member `member`
+ * (pk_estimation.cpp:37-214 takes any real array and an optional imaginary
+ * one; s_run.cpp:206 passes phi2 alone): AXS_FIELD_PHI2 is
+ * axs_member_powerspec; AXS_FIELD_PHI1, AXS_FIELD_PHIDOT1, AXS_FIELD_PHIDOT2
+ * a real component; AXS_FIELD_PHI and AXS_FIELD_PHIDOT the complex fields
+ * phi1 + i phi2 and phidot1 + i phidot2 (pi in the Hamiltonian form).
+ * Binning and outputs as axs_member_powerspec.                            */
+#define AXS_FIELD_PHI2    0
+#define AXS_FIELD_PHI1    1
+#define AXS_FIELD_PHI     2
+#define AXS_FIELD_PHIDOT2 3
+#define AXS_FIELD_PHIDOT1 4
+#define AXS_FIELD_PHIDOT  5
+int axs_member_powerspec_field(axs_ctx *ctx, int member, int field, int binning, int nbins,
+                               double *ks, double *pk, int32_t *count);
 
 /* The spectra of every member in one call (ensembles have no reference
  * counterpart; each member's spectrum is the axs_member_powerspec one).

@@ -1,6 +1,7 @@
 """End-to-end: the host driver axs_main (run_standard, s_run.cpp:26-232) on
 a small lattice, every output file compared with a replay of the same run
 on the CPU oracle; checkpoint / restart continues bit-exactly."""
+import math
 import os
 import subprocess
 
@@ -13,7 +14,7 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 PARAM = """% test run
-lambda              1.0
+lambda              {lam}
 enable_PRS          1
 T_initial           {T0}
 NDIMS               {nd}
@@ -53,11 +54,12 @@ def _cadence(t, n, final):
     return every > 0 and t % every == 0
 
 
-def _run_driver(tmp_path, nd, N, steps, init="thermal", T0=4.0, extra=""):
+def _run_driver(tmp_path, nd, N, steps, init="thermal", T0=4.0, extra="", lam=1.0):
     out = tmp_path / "out"
     out.mkdir(exist_ok=True)
     pf = tmp_path / "run.param"
-    pf.write_text(PARAM.format(nd=nd, N=N, steps=steps, out=out, init=init, T0=T0, extra=extra))
+    pf.write_text(PARAM.format(nd=nd, N=N, steps=steps, out=out, init=init, T0=T0, extra=extra,
+                               lam=lam))
     r = subprocess.run([axs.DRIVER_PATH, str(pf)], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stdout + r.stderr
     return out
@@ -111,6 +113,7 @@ def test_driver_outputs_match_oracle(tmp_path, nd, N, steps, init, eom, lap):
     for s, (tau, *_rest) in enumerate(snaps):
         fields = lines[1 + s].split(",")
         assert fields[0] == str(s) and float(fields[1]) == pytest.approx(tau, abs=1e-6)
+        _check_timing_columns(fields, tau, 1.0)
 
     for s, (tau, f1, f2, c, ks, pk, cnt) in enumerate(snaps):
         raw1 = np.fromfile(out / f"snapshot{s}-phi1", dtype=np.float64)
@@ -137,6 +140,36 @@ def test_driver_outputs_match_oracle(tmp_path, nd, N, steps, init, eom, lap):
             assert float(p_) == pytest.approx(pk[b], rel=1e-5, abs=1e-12 * pk[sel].max())
     log = open(out / "output.txt").read()
     assert "Running with" in log and "Time taken:" in log and "Writing snapshot 0:" in log
+
+
+def _check_timing_columns(fields, tau, lam):
+    # physics.cpp:51-53 and 65-67: H = 1/tau^2, so hubble_scale = 1/H = tau^2,
+    # and string_tension = log(m_saxion / H) = log(sqrt(lambda) tau^2), 0 when
+    # lambda = 0 (s_run.cpp:162-163); float values printed with %f
+    assert float(fields[2]) == pytest.approx(tau * tau, rel=3e-7, abs=1e-6)
+    want = math.log(math.sqrt(lam) * tau * tau) if lam != 0.0 else 0.0
+    assert float(fields[3]) == pytest.approx(want, rel=3e-7, abs=1.5e-6)
+
+
+@pytest.mark.parametrize("lam", [0.5, 2.0, 0.0])
+def test_snapshot_timing_columns(tmp_path, lam):
+    """snapshot-timings.csv: tau, 1/H and the string tension of every
+    snapshot for couplings other than 1, and the zero column at lambda = 0"""
+    nd, N, steps = 2, 64, 61
+    out = _run_driver(tmp_path, nd, N, steps, init="kinv", lam=lam)
+    lines = open(out / "snapshot-timings.csv").read().splitlines()
+    assert lines[0] == "snapshot,tau,hubble_scale,string_tension,"
+    taus, t = [], 1.0
+    for step in range(steps):
+        if _cadence(step, 3, steps):
+            taus.append(t)
+        t = t + float(np.float32(0.1))
+    assert len(lines) - 1 == len(taus) == 3
+    for s, tau in enumerate(taus):
+        fields = lines[1 + s].split(",")
+        assert fields[0] == str(s) and float(fields[1]) == pytest.approx(tau, abs=1e-6)
+        assert fields[4] == ""
+        _check_timing_columns(fields, tau, lam)
 
 
 def test_checkpoint_restart_is_bit_exact(tmp_path):

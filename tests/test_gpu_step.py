@@ -113,18 +113,23 @@ def test_fp32_state_round_trip():
 
 
 def test_step_timing_reported():
-    # the HIP-event time per step brackets the step kernels: close to the
-    # host's wall time on a lattice where the kernels dominate
+    # the HIP-event time per step brackets the step kernels: on a lattice
+    # where they dominate (512^3 FP64, ~2.4 ms per one-step launch, 40
+    # launches) it is the host's wall time less the call's own set-up and
+    # synchronisation -- within 10% below and never above it
     import time
-    N = 256
-    lat = _gpu_run(3, N, random_state(3, N, seed=1), 5)
+    N = 512
+    lat = axs.Lattice(3, N, precision="fp64")
+    lat.init("kinv", seed=3)
+    lat.tau = 1.0
     for graph in (False, True):
         flags = axs.AXS_MARCH_PREFETCH | axs.AXS_MARCH_NT_STORE | (axs.AXS_STEP_GRAPH if graph else 0)
         lat.set_kernel("auto", flags=flags)
+        lat.step(2)
         t0 = time.perf_counter()
         lat.step(40)
         wall_ms = (time.perf_counter() - t0) * 1e3 / 40
-        assert 0.3 * wall_ms <= lat.last_step_ms <= 1.05 * wall_ms, (graph, lat.last_step_ms, wall_ms)
+        assert 0.9 * wall_ms <= lat.last_step_ms <= wall_ms, (graph, lat.last_step_ms, wall_ms)
     s, a = lat.bytes_per_site()
     assert (s, a) == (96.0, 64.0)
 
