@@ -62,7 +62,7 @@ EXPORTS = (
     "axs_set_state", "axs_get_state", "axs_step", "axs_member_axion", "axs_member_strings",
     "axs_strings",
     "axs_string_counts", "axs_member_background", "axs_background",
-    "axs_member_energy", "axs_energy",
+    "axs_member_energy", "axs_energy", "axs_member_energy_noalloc",
     "axs_member_powerspec", "axs_member_powerspec_field", "axs_powerspec",
     "axs_powerspec_all",
     "axs_last_step_ms", "axs_bytes_per_site", "axs_strerror", "axs_last_error",
@@ -152,6 +152,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "axs_background": ([c_void_p, _DP], c_int),
         "axs_member_energy": ([c_void_p, c_int, _DP], c_int),
         "axs_energy": ([c_void_p, _DP], c_int),
+        "axs_member_energy_noalloc": ([c_void_p, c_int, _DP], c_int),
         "axs_member_powerspec": ([c_void_p, c_int, c_int, c_int, _DP, _DP, POINTER(c_int32)], c_int),
         "axs_member_powerspec_field": ([c_void_p, c_int, c_int, c_int, c_int, _DP, _DP,
                                         POINTER(c_int32)], c_int),
@@ -401,6 +402,14 @@ class Lattice:
     def energy(self, member: int = 0) -> np.ndarray:
         out = np.zeros(3, np.float64)
         _check(self.lib.axs_member_energy(self._ctx, member, _dp(out)), "axs_member_energy")
+        return out
+
+    def energy_noalloc(self, member: int = 0) -> np.ndarray:
+        """energy() through the entry point that allocates nothing (a fixed
+        per-GPU workspace)"""
+        out = np.zeros(3, np.float64)
+        _check(self.lib.axs_member_energy_noalloc(self._ctx, member, _dp(out)),
+               "axs_member_energy_noalloc")
         return out
 
     def powerspec(self, member: int = 0, binning: str = "linear", nbins: int = 64,

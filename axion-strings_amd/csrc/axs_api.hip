@@ -449,6 +449,11 @@ static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ct
                   prop.gcnArchName);
         return AXS_ERR_NO_DEVICE;
     }
+    // the fixed workspace of axs_member_energy_noalloc on this device (once
+    // per device); a failure here is reported by that entry point, which
+    // tries again, and leaves the other entry points as they are
+    (void)noalloc_setup(p->device);
+
     axs_ctx *h = new axs_ctx();
     Context *c = &h->impl;
     c->p = *p;
@@ -909,6 +914,23 @@ int axs_member_energy(axs_ctx *ctx, int member, double out[3])
 }
 
 int axs_energy(axs_ctx *ctx, double out[3]) { return axs_member_energy(ctx, 0, out); }
+
+int axs_member_energy_noalloc(axs_ctx *ctx, int member, double out[3])
+{
+    AXS_TRY(check_member(ctx, member));
+    if (!out) {
+        set_error("null output");
+        return AXS_ERR_INVALID_ARG;
+    }
+    if (ctx->impl.p.eom != AXS_EOM_FIELD_RESCALED) {
+        set_error("the energy functional is defined for the field-rescaled form");
+        return AXS_ERR_UNSUPPORTED;
+    }
+    Workspace ws;
+    AXS_TRY(noalloc_setup(ctx->impl.device));     // kept from creation when it succeeded there
+    AXS_TRY(noalloc_workspace(ctx->impl.device, &ws));
+    return run_energy(&ctx->impl, ctx->impl.members[member], out, &ws);
+}
 
 int axs_member_powerspec(axs_ctx *ctx, int member, int binning, int nbins, double *ks, double *pk,
                          int32_t *count)

@@ -311,10 +311,22 @@ int run_string_counts(Context *c, int64_t *n_plaquettes, int64_t *num_cores);
 int run_axion(Context *c, Member &m, double *out);
 int run_strings(Context *c, Member &m, int32_t *coords, int64_t cap, int64_t *count,
                 int64_t *num_cores);
+// device workspace handed to a launcher in place of the context's scratch
+struct Workspace {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+// the fixed per-GPU workspace of axs_member_energy_noalloc: its device
+// address, taken once per device with the device current (axs_reduce.hip)
+int noalloc_setup(int device);
+int noalloc_workspace(int device, Workspace *ws);
+
 int run_background(Context *c, Member &m, double out[6]);
 int run_init(Context *c, Member &m, int mode, uint64_t seed);
 int run_autotune(Context *c, int trials, double *best_ms);
-int run_energy(Context *c, Member &m, double out[3]);
+// ws: a caller-given workspace used instead of the context's scratch
+// (nothing is allocated); null: the context's scratch
+int run_energy(Context *c, Member &m, double out[3], const Workspace *ws = nullptr);
 int run_powerspec_all(Context *c, int binning, int nbins, double *ks, double *pk,
                       int32_t *count);
 int run_powerspec(Context *c, Member &m, int binning, int nbins, double *ks, double *pk,

@@ -13,6 +13,7 @@
 // pairs in block order.  The partition depends only on the lattice size,
 // so results are identical from run to run.
 #include <algorithm>
+#include <mutex>
 
 #include "axs_internal.hpp"
 #include "axs_lattice.hpp"
@@ -22,6 +23,69 @@ namespace axs {
 constexpr int kRedBlock = 256;
 constexpr int kRedMaxBlocks = 2048;
 
+// Fixed device workspace of axs_member_energy_noalloc: the per-block
+// partial sums (at most kRedMaxBlocks blocks of nine Neumaier pairs, 288 KiB)
+// and the folded sums.  One copy per GPU, held until the process ends.  Every
+// call writes each partial it later reads (one per block) and the folded
+// sums, so nothing left by an earlier call is read and nothing needs
+// clearing.  Calls that overlap on one GPU share it and race.
+constexpr size_t kNoAllocBytes = 512 * 1024;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static void *g_noalloc_ptr[kMaxDevices] = {};
+
+// once per device, with the device current: the device address of that
+// device's copy of `scratch` (host code uses only this address)
+int noalloc_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("device %d outside the %d devices the fixed workspace table holds", device,
+                  kMaxDevices);
+        return AXS_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return AXS_OK;
+    void *ptr = nullptr;
+    AXS_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) {
+        set_error("hipGetSymbolAddress returned no address for the fixed workspace");
+        return AXS_ERR_HIP;
+    }
+    g_noalloc_ptr[device] = ptr;
+    return AXS_OK;
+}
+
+int noalloc_workspace(int device, Workspace *ws)
+{
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (device < 0 || device >= kMaxDevices || !g_noalloc_ptr[device]) {
+        set_error("the fixed workspace of device %d is not set up", device);
+        return AXS_ERR_STATE;
+    }
+    ws->p = g_noalloc_ptr[device];
+    ws->bytes = kNoAllocBytes;
+    return AXS_OK;
+}
+
+// `need` bytes of device workspace: the caller's (checked against its size)
+// or the context's scratch (grown on demand)
+static int workspace_for(Context *c, const Workspace *ws, size_t need, void **out)
+{
+    if (ws) {
+        if (need > ws->bytes) {
+            set_error("the reduction needs %zu bytes of workspace; the fixed workspace holds %zu",
+                      need, ws->bytes);
+            return AXS_ERR_CAPACITY;
+        }
+        *out = ws->p;
+        return AXS_OK;
+    }
+    AXS_TRY(ensure_scratch(c, need));
+    *out = c->scratch;
+    return AXS_OK;
+}
 
 struct Kahan {
     double s, c;
@@ -376,7 +440,7 @@ static auto sample_kernel_for(int R)
                       : sample_fp32_kernel<ND, 4>;
 }
 
-static int fp32_sums(Context *c, Member &m, double out[kSampQ])
+static int fp32_sums(Context *c, Member &m, double out[kSampQ], const Workspace *ws = nullptr)
 {
     if (!m.has_state) {
         set_error("member has no state: call axs_set_member_state first");
@@ -400,9 +464,10 @@ static int fp32_sums(Context *c, Member &m, double out[kSampQ])
     const uint32_t ncols = plane * (uint32_t)((N + CI - 1) / CI);
     const int nb = red_blocks(ncols);
     const size_t part_b = (size_t)nb * kSampQ * sizeof(Kahan);
-    AXS_TRY(ensure_scratch(c, part_b + kSampQ * sizeof(double)));
-    Kahan *part = (Kahan *)c->scratch;
-    double *dout = (double *)((char *)c->scratch + part_b);
+    void *wsp = nullptr;
+    AXS_TRY(workspace_for(c, ws, part_b + kSampQ * sizeof(double), &wsp));
+    Kahan *part = (Kahan *)wsp;
+    double *dout = (double *)((char *)wsp + part_b);
     const double lam = (double)c->p.lambda, T0 = (double)c->p.T_initial;
     const double tau2 = c->tau * c->tau, T03 = (T0 * T0) / 3.0, lam_4tau2 = lam / (4.0 * tau2);
     auto k = (c->nd == 2) ? sample_kernel_for<2>(R) : sample_kernel_for<3>(R);
@@ -461,11 +526,11 @@ int run_background(Context *c, Member &m, double out[6])
     return AXS_OK;
 }
 
-int run_energy(Context *c, Member &m, double out[3])
+int run_energy(Context *c, Member &m, double out[3], const Workspace *ws)
 {
     if (c->prec == AXS_FP32) {
         double sums[kSampQ];
-        AXS_TRY(fp32_sums(c, m, sums));
+        AXS_TRY(fp32_sums(c, m, sums, ws));
         energy_scale(c, sums + 6, out);
         return AXS_OK;
     }
@@ -483,9 +548,10 @@ int run_energy(Context *c, Member &m, double out[3])
     const uint32_t ncols = plane * (uint32_t)((N + CI - 1) / CI);
     int nb = red_blocks(ncols);
     size_t part_b = (size_t)nb * 3 * sizeof(Kahan);
-    AXS_TRY(ensure_scratch(c, part_b + 64));
-    Kahan *part = (Kahan *)c->scratch;
-    double *dout = (double *)((char *)c->scratch + part_b);
+    void *wsp = nullptr;
+    AXS_TRY(workspace_for(c, ws, part_b + 64, &wsp));
+    Kahan *part = (Kahan *)wsp;
+    double *dout = (double *)((char *)wsp + part_b);
     double lam = (double)c->p.lambda, T0 = (double)c->p.T_initial;
     double tau2 = c->tau * c->tau;
     double T03 = (T0 * T0) / 3.0;

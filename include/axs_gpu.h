@@ -184,6 +184,17 @@ int axs_background(axs_ctx *ctx, double sums[6]);
 int axs_member_energy(axs_ctx *ctx, int member, double out[3]);
 int axs_energy(axs_ctx *ctx, double out[3]);
 
+/* axs_member_energy without allocating: no counterpart in the reference.
+ * For callers that cannot allocate device memory (HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same result as
+ * axs_member_energy, bit for bit; the partial sums go to a fixed device
+ * workspace of 512 KiB per GPU (enough for every lattice: at most 2048
+ * blocks of partials), held until the process ends, instead of the
+ * context's scratch.  Calls on one GPU that overlap (several host threads
+ * or contexts at once) share that workspace and race.  Returns
+ * AXS_ERR_CAPACITY if a reduction would need more than the workspace.     */
+int axs_member_energy_noalloc(axs_ctx *ctx, int member, double out[3]);
+
 /* ---- power spectrum ---------------------------------------------------- */
 
 /* output_powerspec (pk_estimation.cpp:37-214) of phi2 (s_run.cpp:206) of
