@@ -264,116 +264,6 @@ bin_kernel(int N, const double2 *__restrict__ F, int nbins, const uint32_t *__re
     }
 }
 
-// Row-order binning (nbins <= kRowBinsMax): the half spectrum is read once,
-// in storage order.  Work-group blockIdx.x takes rows [r0, r0 + rows_per);
-// wave w of it rows r0 + w, r0 + w + kRowWaves, ..., lanes along z (64
-// consecutive modes per load).  A row's modes with |k| < N/2 fall in
-// bins that do not decrease along z, so a chunk of 64 spans bins from its
-// first valid lane's to its last's: each of those gets a fixed butterfly
-// sum of its lanes' |F|^2, |k| and counts, which lane 0 adds (compensated)
-// to the wave's accumulator for the bin in LDS.  At the end the waves'
-// accumulators are merged in wave order into the block's partials, stored
-// bin-major (part[(member * nbins + b) * nblk + block]) so that
-// bin_fold_kernel merges each bin's blocks in block order: deterministic.
-constexpr int kRowWaves = 4;
-constexpr int kRowBinsMax = 512;
-
-template <int ND>
-__global__ void __launch_bounds__(64 * kRowWaves)
-bin_rows_kernel(int N, const double2 *__restrict__ F, int nbins, const int64_t *__restrict__ hi2g,
-                uint32_t rows_per, uint32_t nrows, KahanP *__restrict__ part_pk,
-                KahanP *__restrict__ part_ks, long long *__restrict__ part_cnt)
-{
-    __shared__ KahanP apk[kRowWaves][kRowBinsMax], aks[kRowWaves][kRowBinsMax];
-    __shared__ long long acnt[kRowWaves][kRowBinsMax];
-    __shared__ int64_t hi2[kRowBinsMax];
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int H = N / 2 + 1;
-    F += (size_t)blockIdx.y * (size_t)nrows * (size_t)H;
-    for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
-        hi2[b] = hi2g[b];
-        for (int w = 0; w < kRowWaves; ++w) {
-            apk[w][b] = KahanP{0.0, 0.0};
-            aks[w][b] = KahanP{0.0, 0.0};
-            acnt[w][b] = 0;
-        }
-    }
-    __syncthreads();
-    const int64_t top = (int64_t)(N / 2) * (N / 2);           // |k| < N/2
-    const uint32_t r0 = blockIdx.x * rows_per;
-    const uint32_t r1 = (r0 + rows_per < nrows) ? r0 + rows_per : nrows;
-    for (uint32_t r = r0 + (uint32_t)wave; r < r1; r += kRowWaves) {
-        int64_t fi = 0, fj;
-        if (ND == 3) {
-            const int64_t ii = r / (uint32_t)N, jj = r % (uint32_t)N;
-            fi = ii < N / 2 ? ii : ii - N;
-            fj = jj < N / 2 ? jj : jj - N;
-        } else {
-            fj = (int64_t)r < N / 2 ? (int64_t)r : (int64_t)r - N;
-        }
-        const int64_t r2 = fi * fi + fj * fj;
-        if (fi == -N / 2 || fj == -N / 2 || r2 >= top) continue;   // uniform: |k| >= N/2
-        int64_t zmax = isqrt_ceil(top - r2);        // z < zmax  <=>  r2 + z^2 < top
-        if (zmax > H) zmax = H;
-        const int64_t zs = (r2 == 0) ? 1 : 0;       // not the k = 0 mode
-        int bcur = 0;
-        while (bcur < nbins - 1 && hi2[bcur] <= r2 + zs * zs) ++bcur;
-        const double2 *row = F + (size_t)r * (size_t)H;
-        for (int64_t z0 = 0; z0 < zmax; z0 += 64) {
-            const int64_t z = z0 + lane;
-            const bool on = z >= zs && z < zmax;
-            double p = 0.0, km = 0.0;
-            long long w = 0;
-            int b = bcur;
-            if (on) {
-                const double2 f = row[z];
-                const int64_t k2 = r2 + z * z;
-                w = (z == 0 || z == N / 2) ? 1 : 2;
-                p = (double)w * (f.x * f.x + f.y * f.y);
-                km = (double)w * sqrt((double)k2);
-                while (hi2[b] <= k2) ++b;            // k2 < top = hi2[nbins - 1]
-            }
-            const uint64_t act = __ballot(on);
-            if (!act) continue;
-            const int first = __builtin_ctzll(act), last = 63 - __builtin_clzll(act);
-            const int bmin = __builtin_amdgcn_readlane(b, first);
-            const int bmax = __builtin_amdgcn_readlane(b, last);
-            for (int bb = bmin; bb <= bmax; ++bb) {
-                const bool mine = on && b == bb;
-                double v = mine ? p : 0.0, u = mine ? km : 0.0;
-                long long c = mine ? w : 0;
-                for (int o = 32; o > 0; o >>= 1) {
-                    v += __shfl_xor(v, o, 64);
-                    u += __shfl_xor(u, o, 64);
-                    c += __shfl_xor(c, o, 64);
-                }
-                if (lane == 0) {
-                    kaddp(apk[wave][bb], v);
-                    kaddp(aks[wave][bb], u);
-                    acnt[wave][bb] += c;
-                }
-            }
-            bcur = bmax;
-        }
-    }
-    __syncthreads();
-    const size_t nblk = gridDim.x;
-    for (int b = threadIdx.x; b < nbins; b += blockDim.x) {
-        KahanP a = apk[0][b], k = aks[0][b];
-        long long n = acnt[0][b];
-        for (int w = 1; w < kRowWaves; ++w) {
-            a = kmergep(a, apk[w][b]);
-            k = kmergep(k, aks[w][b]);
-            n += acnt[w][b];
-        }
-        const size_t slot = ((size_t)blockIdx.y * nbins + b) * nblk + blockIdx.x;
-        part_pk[slot] = a;
-        part_ks[slot] = k;
-        part_cnt[slot] = n;
-    }
-}
-
 // bin blockIdx.x of lattice blockIdx.y: work-item t merges the bin's items
 // t, t + 256, ... in order, then a fixed pairwise tree over the work-items
 // (deterministic for a given item count)
@@ -484,19 +374,7 @@ static int pk_raw(Context *c, const PkSrc *srcs, int n, int N, const BinLayout &
     const uint32_t L = (uint32_t)((c->nd == 3) ? (int64_t)N * N * N : (int64_t)N * N);
 
     const size_t nb = (size_t)nbins;
-    // row-order binning where the bins' accumulators fit in LDS: nblk
-    // work-groups per lattice, each bin's items are its blocks' partials
-    const bool by_rows = nbins <= kRowBinsMax;
-    const uint32_t nrows = (c->nd == 3) ? (uint32_t)N * (uint32_t)N : (uint32_t)N;
-    const uint32_t nblk = std::min<uint32_t>(2048u, (nrows + 63) / 64);
-    const uint32_t rows_per = (nrows + nblk - 1) / nblk;
-    std::vector<uint32_t> row_items;
-    if (by_rows) {
-        row_items.resize(nb + 1);
-        for (size_t b = 0; b <= nb; ++b) row_items[b] = (uint32_t)b * nblk;
-    }
-    const std::vector<uint32_t> &items = by_rows ? row_items : B.items;
-    const uint32_t nitems = items[nb];
+    const uint32_t nitems = B.items[nb];
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t o_src = 0, o_lo = align((size_t)n * sizeof(PkSrc)), o_hi = o_lo + align(nb * 8);
     const size_t o_ib = o_hi + align(nb * 8);
@@ -538,14 +416,9 @@ static int pk_raw(Context *c, const PkSrc *srcs, int n, int N, const BinLayout &
 
     AXS_HIP_TRY(hipMemcpyAsync(dlo, B.lo.data(), nb * 8, hipMemcpyHostToDevice, c->stream));
     AXS_HIP_TRY(hipMemcpyAsync(dhi, B.hi.data(), nb * 8, hipMemcpyHostToDevice, c->stream));
-    AXS_HIP_TRY(hipMemcpyAsync(dib, items.data(), (nb + 1) * 4, hipMemcpyHostToDevice,
+    AXS_HIP_TRY(hipMemcpyAsync(dib, B.items.data(), (nb + 1) * 4, hipMemcpyHostToDevice,
                                c->stream));
-    if (by_rows) {
-        auto k = (c->nd == 3) ? bin_rows_kernel<3> : bin_rows_kernel<2>;
-        hipLaunchKernelGGL(k, dim3(nblk, (unsigned)n), dim3(64 * kRowWaves), 0, c->stream, N,
-                           (const double2 *)plan->out, nbins, dhi, rows_per, nrows, ppk, pks,
-                           pcnt);
-    } else if (nitems > 0) {
+    if (nitems > 0) {
         auto k = (c->nd == 3) ? bin_kernel<3> : bin_kernel<2>;
         hipLaunchKernelGGL(k, dim3(nitems, (unsigned)n), dim3(kPkRows), 0, c->stream, N,
                            (const double2 *)plan->out, nbins, dib, dlo, dhi, ppk, pks, pcnt);
