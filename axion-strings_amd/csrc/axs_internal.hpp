@@ -95,8 +95,8 @@ struct Member {
     double ker_tau = 0.0;
     bool vel_valid = false;   // FP32: aux_vel holds phidot of the current state
     uint64_t version = 0;     // bumped whenever the state changes
-    // FP32: background sums and energy components of state `samp_version`
-    // at tau `samp_tau` (one pass computes both)
+    // background sums and energy components of state `samp_version` at tau
+    // `samp_tau` (one pass computes both)
     bool samp_valid = false;
     uint64_t samp_version = 0;
     double samp_tau = 0.0;
@@ -181,6 +181,15 @@ struct Context {
     double snap_tau = 0.0;
 
     std::vector<FftPlan *> fft_plans;
+    // data-independent sums of the power-spectrum bin tables in use (|k|
+    // numerators and mode counts per bin), kept per (side, dimensions,
+    // binning, bins): computed once, by the binning kernel's geometry pass
+    struct PkGeom {
+        int N = 0, nd = 0, binning = 0, nbins = 0;
+        std::vector<double> ks;
+        std::vector<long long> cnt;
+    };
+    std::vector<PkGeom> pk_geom;
     std::vector<void *> inv_plans;   // device-init inverse transforms (axs_init.hip)
     std::vector<void *> spec_plans;  // spectral-Laplacian transforms (axs_spectral.hip)
 

@@ -303,12 +303,202 @@ bin_fold_kernel(int nbins, uint32_t nitems, const uint32_t *__restrict__ item_be
     }
 }
 
+// ---------------------------------------------------------------------------
+// Row-order binning, for bin tables that fit the work-group's LDS.  The half
+// spectrum is read once, in storage order: wave w of the grid takes rows w,
+// w + W, ... (W waves in all), its 64 lanes 64 consecutive modes z of a row
+// at a time.  Along a row |k|^2 = r2 + z^2 grows with z, so the lanes' bins
+// never decrease across the wave and each bin present is one run of lanes: a
+// segmented inclusive scan (DPP row shifts and row broadcasts, registers
+// only) leaves each run's sum in its last lane, which adds it (compensated)
+// to the wave's accumulator for that bin in LDS -- the runs' bins differ, so
+// no two lanes touch one accumulator.  At the end the work-group merges its
+// waves' accumulators in wave order into one partial per bin, and
+// fold_rows_kernel merges the work-groups' partials per bin in work-group
+// order: deterministic for a given lattice.
+// A mode's bin: s = floor(|k|) from a float square root corrected by
+// integer compares, the first bin that can hold |k|^2 >= s^2 from a table of
+// the shells s, then up the bins while |k|^2 >= hi (at most a few steps,
+// only where bins are narrower than a shell).
+// GEOM sums the data-independent |k| (weighted as the modes) and counts the
+// modes from the runs' lengths, without reading the spectrum.
+constexpr int kScanWaves = 4;
+constexpr size_t kScanLdsMax = 64 * 1024;
+
+// v of lane (lane - d) in the same 16-lane row (row_shr:d), of lane 15 of
+// the previous row (row_bcast:15, rows 1 and 3) or of lane 31 (row_bcast:31,
+// rows 2 and 3); lanes without a source read 0
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_take(double v)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROWS, 0xf, false);
+    const uint32_t hi =
+        (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROWS, 0xf, false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+static size_t scan_lds_bytes(int N, int nbins, bool geom)
+{
+    size_t b = (size_t)kScanWaves * nbins * sizeof(KahanP);
+    if (geom) b += (size_t)kScanWaves * nbins * sizeof(long long);
+    b += (size_t)nbins * sizeof(uint32_t);
+    b += ((size_t)(N / 2) * sizeof(uint16_t) + 3) & ~(size_t)3;
+    return b;
+}
+
+template <int ND, bool GEOM>
+__global__ void __launch_bounds__(64 * kScanWaves)
+bin_scan_kernel(int N, const double2 *__restrict__ F, int nbins, const uint32_t *__restrict__ hi2g,
+                const uint16_t *__restrict__ shellg, uint32_t nrows, KahanP *__restrict__ part,
+                long long *__restrict__ part_cnt)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    KahanP *acc = reinterpret_cast<KahanP *>(smem);                    // [wave][bin]
+    long long *acnt = reinterpret_cast<long long *>(acc + kScanWaves * nbins);
+    uint32_t *hi2 = reinterpret_cast<uint32_t *>(acnt + (GEOM ? kScanWaves * nbins : 0));
+    uint16_t *shell = reinterpret_cast<uint16_t *>(hi2 + nbins);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int half = N / 2;
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
+        hi2[i] = hi2g[i];
+        for (int w = 0; w < kScanWaves; ++w) {
+            acc[w * nbins + i] = KahanP{0.0, 0.0};
+            if (GEOM) acnt[w * nbins + i] = 0;
+        }
+    }
+    for (int i = threadIdx.x; i < half; i += blockDim.x) shell[i] = shellg[i];
+    __syncthreads();
+    const uint32_t H = (uint32_t)half + 1;
+    if (!GEOM) F += (size_t)blockIdx.y * (size_t)nrows * H;
+    KahanP *wacc = acc + wave * nbins;
+    long long *wcnt = acnt + wave * nbins;
+    const uint32_t top = (uint32_t)half * (uint32_t)half;         // |k| < N/2
+    const uint32_t W = gridDim.x * kScanWaves;
+    const uint64_t upto = (2ull << lane) - 1;                     // lanes 0 .. lane
+    for (uint32_t r = blockIdx.x * kScanWaves + (uint32_t)wave; r < nrows; r += W) {
+        int fi = 0, fj;
+        if (ND == 3) {
+            const uint32_t ii = r / (uint32_t)N, jj = r - ii * (uint32_t)N;
+            fi = (int)ii < half ? (int)ii : (int)ii - N;
+            fj = (int)jj < half ? (int)jj : (int)jj - N;
+        } else {
+            fj = (int)r < half ? (int)r : (int)r - N;
+        }
+        const uint32_t r2 = (uint32_t)(fi * fi) + (uint32_t)(fj * fj);
+        if (r2 >= top) continue;                                 // the whole row has |k| >= N/2
+        const uint32_t zmax = (uint32_t)isqrt_ceil((int64_t)(top - r2));   // z < zmax <=> |k|^2 < top
+        const uint32_t zs = (r2 == 0) ? 1u : 0u;                 // not the k = 0 mode
+        const double2 *row = F + (size_t)r * H;
+        for (uint32_t z0 = 0; z0 < zmax; z0 += 64) {
+            const uint32_t z = z0 + (uint32_t)lane;
+            const bool on = z >= zs && z < zmax;
+            const uint32_t k2 = r2 + z * z;
+            int b;
+            double v = 0.0;
+            if (on) {
+                uint32_t sh = (uint32_t)__builtin_amdgcn_sqrtf((float)k2);
+                if (sh * sh > k2) --sh;
+                if ((sh + 1) * (sh + 1) <= k2) ++sh;
+                b = shell[sh];
+                while (hi2[b] <= k2) ++b;
+                if (GEOM) {
+                    const double km = sqrt((double)k2);
+                    v = (z == 0) ? km : 2.0 * km;
+                } else {
+                    const double2 f = row[z];
+                    const double p2 = f.x * f.x + f.y * f.y;
+                    v = (z == 0) ? p2 : 2.0 * p2;
+                }
+            } else {
+                b = (z < zs) ? -1 : nbins;
+            }
+            const int bprev = __builtin_amdgcn_update_dpp(-2, b, 0x138, 0xf, 0xf, false);
+            const uint64_t heads = __ballot(bprev != b);        // first lane of each run
+            const int start = 63 - __clzll(heads & upto);
+            const int len = lane - start;                        // earlier lanes of the run
+            double u;
+            u = dpp_take<0x111, 0xf>(v);
+            if (len >= 1) v += u;
+            u = dpp_take<0x112, 0xf>(v);
+            if (len >= 2) v += u;
+            u = dpp_take<0x114, 0xf>(v);
+            if (len >= 4) v += u;
+            u = dpp_take<0x118, 0xf>(v);
+            if (len >= 8) v += u;
+            u = dpp_take<0x142, 0xa>(v);
+            if (len > (lane & 15)) v += u;                       // run reaches the row before
+            u = dpp_take<0x143, 0xc>(v);
+            if (len > (lane & 31)) v += u;                       // run reaches lanes 0..31
+            const bool last = lane == 63 || (((heads >> lane) >> 1) & 1ull);
+            if (last && b >= 0 && b < nbins) {
+                KahanP a = wacc[b];
+                kaddp(a, v);
+                wacc[b] = a;
+                if (GEOM) wcnt[b] += 2 * (long long)(len + 1) - ((z - (uint32_t)len == 0) ? 1 : 0);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
+        KahanP a = acc[i];
+        for (int w = 1; w < kScanWaves; ++w) a = kmergep(a, acc[w * nbins + i]);
+        const size_t slot = ((size_t)blockIdx.y * nbins + i) * gridDim.x + blockIdx.x;
+        part[slot] = a;
+        if (GEOM) {
+            long long n = 0;
+            for (int w = 0; w < kScanWaves; ++w) n += acnt[w * nbins + i];
+            part_cnt[slot] = n;
+        }
+    }
+}
+
+// bin blockIdx.x of lattice blockIdx.y: the nblk work-groups' partials of
+// bin_scan_kernel merged in work-group order (strided runs, then a fixed
+// pairwise tree); counts summed when part_cnt is given
+__global__ void __launch_bounds__(256)
+fold_rows_kernel(int nbins, uint32_t nblk, const KahanP *__restrict__ part,
+                 const long long *__restrict__ part_cnt, double *__restrict__ out,
+                 long long *__restrict__ out_cnt)
+{
+    __shared__ KahanP sa[256];
+    __shared__ long long sn[256];
+    const size_t base = ((size_t)blockIdx.y * nbins + blockIdx.x) * nblk;
+    KahanP a{0.0, 0.0};
+    long long n = 0;
+    for (uint32_t t = threadIdx.x; t < nblk; t += 256) {
+        a = kmergep(a, part[base + t]);
+        if (part_cnt) n += part_cnt[base + t];
+    }
+    sa[threadIdx.x] = a;
+    sn[threadIdx.x] = n;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) {
+            sa[threadIdx.x] = kmergep(sa[threadIdx.x], sa[threadIdx.x + h]);
+            sn[threadIdx.x] += sn[threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const size_t o = (size_t)blockIdx.y * nbins + blockIdx.x;
+        out[o] = sa[0].s + sa[0].c;
+        if (out_cnt) out_cnt[o] = sn[0];
+    }
+}
+
 // bins of a lattice of side N as ranges [lo, hi) of the integer |k|^2, and
 // the work items (bounding-box rows in chunks of kPkRows) of each bin
 struct BinLayout {
-    int nbins = 0;
+    int N = 0, nd = 0, binning = 0, nbins = 0;
     std::vector<int64_t> lo, hi;
     std::vector<uint32_t> items;      // first item of each bin, items[nbins] = total
+    // row-order binning: hi as 32-bit, and for each shell s < N/2 the first
+    // bin whose range can hold |k|^2 >= s^2
+    std::vector<uint32_t> hi32;
+    std::vector<uint16_t> shell;
+    bool scan = false;                // row-order binning fits the LDS
 };
 
 static int bin_layout(int N, int nd, int binning, int nbins, BinLayout &B)
@@ -319,6 +509,9 @@ static int bin_layout(int N, int nd, int binning, int nbins, BinLayout &B)
         set_error("powerspec: bad binning %d / nbins %d", binning, nbins);
         return AXS_ERR_INVALID_ARG;
     }
+    B.N = N;
+    B.nd = nd;
+    B.binning = binning;
     B.nbins = nbins;
     B.lo.assign(nbins, 0);
     B.hi.assign(nbins, 0);
@@ -357,6 +550,101 @@ static int bin_layout(int N, int nd, int binning, int nbins, BinLayout &B)
             rows = (nd == 3) ? side * side : side;
         }
         B.items[b + 1] = B.items[b] + (uint32_t)((rows + kPkRows - 1) / kPkRows);
+    }
+    B.scan = N <= 65536 && scan_lds_bytes(N, nbins, true) <= kScanLdsMax;
+    if (B.scan) {
+        B.hi32.assign(hi.begin(), hi.end());
+        B.shell.assign(N / 2, 0);
+        int b = 0;
+        for (int64_t sh = 0; sh < N / 2; ++sh) {
+            while (b < nbins - 1 && hi[b] <= sh * sh) ++b;
+            B.shell[sh] = (uint16_t)b;
+        }
+    }
+    return AXS_OK;
+}
+
+// work-groups of the row-order binning per lattice: about 16 rows per wave,
+// at most 2048 (a function of the side only, so a lattice's sums do not
+// depend on the batch it is transformed in)
+static uint32_t scan_blocks(int nd, int N)
+{
+    const uint64_t nrows = (nd == 3) ? (uint64_t)N * N : (uint64_t)N;
+    const uint64_t want = (nrows + 16 * kScanWaves - 1) / (16 * kScanWaves);
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, 2048));
+}
+
+// Row-order binning of the n half spectra in plan->out: the power sums on
+// every call; the |k| sums and mode counts, which depend only on the bin
+// table, from the context's cache (one geometry pass the first time).
+static int bin_rows(Context *c, FftPlan *plan, int n, const BinLayout &B,
+                    std::vector<double> &hpk, std::vector<double> &hks,
+                    std::vector<long long> &hcnt)
+{
+    const int N = B.N, nbins = B.nbins;
+    const size_t nb = (size_t)nbins;
+    const uint32_t nrows = (c->nd == 3) ? (uint32_t)N * (uint32_t)N : (uint32_t)N;
+    const uint32_t nblk = scan_blocks(c->nd, N);
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_hi = 0, o_sh = align(nb * 4);
+    const size_t o_part = o_sh + align((size_t)(N / 2) * 2);
+    const size_t o_pcnt = o_part + align((size_t)n * nb * nblk * sizeof(KahanP));
+    const size_t o_out = o_pcnt + align(nb * nblk * sizeof(long long));
+    const size_t o_cnt = o_out + align((size_t)n * nb * 8);
+    const size_t need = o_cnt + align(nb * 8);
+    AXS_TRY(ensure_scratch(c, need));
+    char *sp = (char *)c->scratch;
+    uint32_t *dhi = (uint32_t *)(sp + o_hi);
+    uint16_t *dsh = (uint16_t *)(sp + o_sh);
+    KahanP *part = (KahanP *)(sp + o_part);
+    long long *pcnt = (long long *)(sp + o_pcnt);
+    double *dout = (double *)(sp + o_out);
+    long long *dcnt = (long long *)(sp + o_cnt);
+    AXS_HIP_TRY(hipMemcpyAsync(dhi, B.hi32.data(), nb * 4, hipMemcpyHostToDevice, c->stream));
+    AXS_HIP_TRY(hipMemcpyAsync(dsh, B.shell.data(), (size_t)(N / 2) * 2, hipMemcpyHostToDevice,
+                               c->stream));
+    const double2 *F = (const double2 *)plan->out;
+
+    Context::PkGeom *g = nullptr;
+    for (auto &e : c->pk_geom)
+        if (e.N == N && e.nd == c->nd && e.binning == B.binning && e.nbins == nbins) g = &e;
+    if (!g) {
+        Context::PkGeom e;
+        e.N = N;
+        e.nd = c->nd;
+        e.binning = B.binning;
+        e.nbins = nbins;
+        e.ks.assign(nb, 0.0);
+        e.cnt.assign(nb, 0);
+        auto kg = (c->nd == 3) ? bin_scan_kernel<3, true> : bin_scan_kernel<2, true>;
+        hipLaunchKernelGGL(kg, dim3(nblk, 1), dim3(64 * kScanWaves), scan_lds_bytes(N, nbins, true),
+                           c->stream, N, F, nbins, dhi, dsh, nrows, part, pcnt);
+        hipLaunchKernelGGL(fold_rows_kernel, dim3((unsigned)nbins, 1), dim3(256), 0, c->stream,
+                           nbins, nblk, part, pcnt, dout, dcnt);
+        AXS_HIP_TRY(hipGetLastError());
+        AXS_HIP_TRY(hipMemcpyAsync(e.ks.data(), dout, nb * 8, hipMemcpyDeviceToHost, c->stream));
+        AXS_HIP_TRY(hipMemcpyAsync(e.cnt.data(), dcnt, nb * 8, hipMemcpyDeviceToHost, c->stream));
+        AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+        c->pk_geom.push_back(std::move(e));
+        g = &c->pk_geom.back();
+    }
+    auto kd = (c->nd == 3) ? bin_scan_kernel<3, false> : bin_scan_kernel<2, false>;
+    hipLaunchKernelGGL(kd, dim3(nblk, (unsigned)n), dim3(64 * kScanWaves),
+                       scan_lds_bytes(N, nbins, false), c->stream, N, F, nbins, dhi, dsh, nrows,
+                       part, (long long *)nullptr);
+    hipLaunchKernelGGL(fold_rows_kernel, dim3((unsigned)nbins, (unsigned)n), dim3(256), 0,
+                       c->stream, nbins, nblk, part, (const long long *)nullptr, dout,
+                       (long long *)nullptr);
+    AXS_HIP_TRY(hipGetLastError());
+    hpk.assign((size_t)n * nb, 0.0);
+    AXS_HIP_TRY(hipMemcpyAsync(hpk.data(), dout, hpk.size() * 8, hipMemcpyDeviceToHost,
+                               c->stream));
+    AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+    hks.resize((size_t)n * nb);
+    hcnt.resize((size_t)n * nb);
+    for (int q = 0; q < n; ++q) {
+        std::copy(g->ks.begin(), g->ks.end(), hks.begin() + (size_t)q * nb);
+        std::copy(g->cnt.begin(), g->cnt.end(), hcnt.begin() + (size_t)q * nb);
     }
     return AXS_OK;
 }
@@ -413,6 +701,7 @@ static int pk_raw(Context *c, const PkSrc *srcs, int n, int N, const BinLayout &
         set_error("rocfft_execute failed");
         return AXS_ERR_FFT;
     }
+    if (B.scan) return bin_rows(c, plan, n, B, hpk, hks, hcnt);
 
     AXS_HIP_TRY(hipMemcpyAsync(dlo, B.lo.data(), nb * 8, hipMemcpyHostToDevice, c->stream));
     AXS_HIP_TRY(hipMemcpyAsync(dhi, B.hi.data(), nb * 8, hipMemcpyHostToDevice, c->stream));

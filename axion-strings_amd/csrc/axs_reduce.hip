@@ -1,20 +1,22 @@
 // axs_reduce.hip -- whole-lattice reductions in compensated FP64:
 //   background sums of s_run.cpp:135-150 (six quantities);
 //   energy components (kinetic, gradient, potential).
-// The FP64 form sums its stored fields (background_kernel, energy_kernel);
-// the FP32 form gets both sets from one pass over its (phi, u) state that
-// rebuilds phidot per site (sample_fp32_kernel), kept per state and tau.
+// Both sets come from one pass over the member's state, kept per state and
+// tau: sample_fp64_kernel over the FP64 form's stored (phi, phidot) pairs,
+// sample_fp32_kernel over the FP32 form's (phi, u) state, rebuilding phidot
+// per site.
 //
-// Each work-item keeps Neumaier (sum, compensation) pairs over a fixed
-// subset of sites (background: a strided set; energy: columns along the
-// slowest axis, marched so that the slow-axis neighbour stays in
-// registers); waves combine pairs with cross-lane shuffles, the block
-// writes one pair per quantity, and a single-block pass folds the block
-// pairs in block order.  The partition depends only on the lattice size,
-// so results are identical from run to run.
+// A work-item marches columns along the slowest axis, so the slow-axis
+// neighbour stays in registers; it sums a column's sites in plain FP64 and
+// adds each column sum to its Neumaier (sum, compensation) pairs; waves
+// combine pairs with cross-lane shuffles, the block writes one pair per
+// quantity, and a single-block pass folds the block pairs in block order.
+// The partition depends only on the lattice size, so results are identical
+// from run to run.
 #include <algorithm>
 #include <mutex>
 
+#include "axs_atan2.hpp"
 #include "axs_internal.hpp"
 #include "axs_lattice.hpp"
 
@@ -136,101 +138,14 @@ __device__ __forceinline__ void block_reduce(Kahan (&k)[Q], Kahan (*sh)[kRedBloc
     }
 }
 
-template <class P, class V>
-__global__ void __launch_bounds__(kRedBlock)
-background_kernel(uint32_t L, P phi, V vel, Kahan *__restrict__ part)
+// the arctangent table of abs_atan2_fast, staged in LDS by the block
+__device__ __forceinline__ void stage_atan_table(double (*tab)[2])
 {
-    __shared__ Kahan sh[6][kRedBlock / 64];
-    Kahan k[6];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) k[q] = Kahan{0.0, 0.0};
-    uint32_t stride = gridDim.x * kRedBlock;
-    for (uint32_t m = blockIdx.x * kRedBlock + threadIdx.x; m < L; m += stride) {
-        double a, b, va, vb;
-        phi.get(m, a, b);
-        vel.get(m, va, vb);
-        kadd(k[0], a);
-        kadd(k[1], b);
-        kadd(k[2], va);
-        kadd(k[3], vb);
-        kadd(k[4], fabs(atan2(a, b)));            // s_run.cpp:145
-        kadd(k[5], sqrt(a * a + b * b));          // s_run.cpp:146
+    for (int i = threadIdx.x; i < 257; i += blockDim.x) {
+        tab[i][0] = kAtanTable[i][0];
+        tab[i][1] = kAtanTable[i][1];
     }
-    block_reduce<6>(k, sh);
-    if (threadIdx.x == 0)
-        for (int q = 0; q < 6; ++q) part[blockIdx.x * 6 + q] = k[q];
-}
-
-// Energy densities summed over the lattice.  A work-item owns one site q
-// of the plane orthogonal to the slowest axis (3D: i, 2D: j) and marches
-// along that axis over a chunk of CI planes: the forward neighbour along
-// the slow axis is the next iteration's centre, kept in registers, so phi
-// is read from HBM once per site (the in-plane neighbours +1 are cache
-// hits).  The gradient terms are added in axis order (3D: i, j, k; 2D: i,
-// j), as the reference's axis order of neighbours.
-template <int ND, class P, class V>
-__global__ void __launch_bounds__(kRedBlock)
-energy_kernel(int N, int CI, uint32_t ncols, P phi, V vel, double tau2, double T03,
-              double lam_4tau2, Kahan *__restrict__ part)
-{
-    __shared__ Kahan sh[3][kRedBlock / 64];
-    Kahan k[3];
-#pragma unroll
-    for (int q = 0; q < 3; ++q) k[q] = Kahan{0.0, 0.0};
-    const uint32_t plane = (ND == 3) ? (uint32_t)N * (uint32_t)N : (uint32_t)N;
-    const uint32_t stride = gridDim.x * kRedBlock;
-    // XCD-aware order (work-groups go round-robin to the 8 XCDs): each XCD
-    // takes a contiguous run of columns, so row j + 1 is read on the same L2
-    uint32_t blk = blockIdx.x;
-    if ((gridDim.x & 7u) == 0) blk = (blk & 7u) * (gridDim.x >> 3) + (blk >> 3);
-    for (uint32_t col = blk * kRedBlock + threadIdx.x; col < ncols; col += stride) {
-        const uint32_t q = col % plane, ch = col / plane;
-        // in-plane forward neighbours: 3D (j, k+1) and (j+1, k); 2D i+1
-        uint32_t qk, qj = 0;
-        if (ND == 3) {
-            const uint32_t j = q / (uint32_t)N, kk = q - j * (uint32_t)N;
-            qk = j * (uint32_t)N + (kk + 1 == (uint32_t)N ? 0u : kk + 1);
-            qj = (j + 1 == (uint32_t)N ? 0u : j + 1) * (uint32_t)N + kk;
-        } else {
-            qk = (q + 1 == (uint32_t)N) ? 0u : q + 1;
-        }
-        const int s0 = (int)ch * CI, s1 = (s0 + CI < N) ? s0 + CI : N;
-        double a, b;
-        phi.get((uint32_t)s0 * plane + q, a, b);
-        for (int sl = s0; sl < s1; ++sl) {
-            const uint32_t base = (uint32_t)sl * plane;
-            const uint32_t nbase = (sl + 1 == N) ? 0u : base + plane;
-            double sa, sb, ka, kb, va, vb;
-            phi.get(nbase + q, sa, sb);              // slow-axis neighbour, next centre
-            phi.get(base + qk, ka, kb);
-            vel.get(base + q, va, vb);
-            double g = 0.0;
-            if (ND == 3) {
-                double ja, jb;
-                phi.get(base + qj, ja, jb);
-                double d1 = sa - a, d2 = sb - b;
-                g += d1 * d1 + d2 * d2;
-                d1 = ja - a, d2 = jb - b;
-                g += d1 * d1 + d2 * d2;
-                d1 = ka - a, d2 = kb - b;
-                g += d1 * d1 + d2 * d2;
-            } else {
-                double d1 = ka - a, d2 = kb - b;
-                g += d1 * d1 + d2 * d2;
-                d1 = sa - a, d2 = sb - b;
-                g += d1 * d1 + d2 * d2;
-            }
-            const double S = a * a + b * b - tau2 + T03;
-            kadd(k[0], 0.5 * (va * va + vb * vb));
-            kadd(k[1], 0.5 * g);
-            kadd(k[2], lam_4tau2 * S * S);
-            a = sa;
-            b = sb;
-        }
-    }
-    block_reduce<3>(k, sh);
-    if (threadIdx.x == 0)
-        for (int q = 0; q < 3; ++q) part[blockIdx.x * 3 + q] = k[q];
+    __syncthreads();
 }
 
 // FP32 form: the six background sums and the three energy components in one
@@ -250,13 +165,15 @@ energy_kernel(int N, int CI, uint32_t ncols, P phi, V vel, double tau2, double T
 // rows are its own rows' cache lines.
 constexpr int kSampQ = 9;     // background (6), then kinetic, gradient, potential
 
-template <int ND, int R>
-__global__ void __launch_bounds__(kRedBlock)
+template <int ND, int R, bool UNI>
+__global__ void __launch_bounds__(kRedBlock) __attribute__((amdgpu_waves_per_eu(R == 1 ? 4 : 1)))
 sample_fp32_kernel(int N, int CI, uint32_t ncols, const fvec4 *__restrict__ in,
                    const fvec2 *__restrict__ kin, StepScalarsF s, int recompute, double tau2,
                    double T03, double lam_4tau2, Kahan *__restrict__ part)
 {
     __shared__ Kahan sh[kSampQ][kRedBlock / 64];
+    __shared__ double atab[257][2];
+    stage_atan_table(atab);
     Kahan k[kSampQ];
 #pragma unroll
     for (int q = 0; q < kSampQ; ++q) k[q] = Kahan{0.0, 0.0};
@@ -269,9 +186,14 @@ sample_fp32_kernel(int N, int CI, uint32_t ncols, const fvec4 *__restrict__ in,
         while (x >= N) x -= N;
         return x;
     };
-    // phi (x, y) of a site as one 8-byte load from the (phi, u) array
-    const fvec2 *__restrict__ in2 = reinterpret_cast<const fvec2 *>(in);
-    auto xy = [&](uint32_t m) { return in2[2 * (size_t)m]; };
+    // Slices of the slow axis by byte address: the slice base is one value
+    // per wave when UNI (a slice holds a multiple of 64 sites, so a wave's
+    // 64 columns share their slices) and the per-site offsets are 32-bit
+    const char *__restrict__ inb = reinterpret_cast<const char *>(in);
+    const char *__restrict__ kinb = reinterpret_cast<const char *>(kin);
+    auto slice = [&](int sl) { return inb + (size_t)(uint32_t)sl * plane * sizeof(fvec4); };
+    auto ld4 = [](const char *sp, uint32_t o) { return *reinterpret_cast<const fvec4 *>(sp + o); };
+    auto ld2 = [](const char *sp, uint32_t o) { return *reinterpret_cast<const fvec2 *>(sp + o); };
     struct Plane {
         fvec4 c;
         fvec2 a[2 * R], b[2 * R];
@@ -283,54 +205,57 @@ sample_fp32_kernel(int N, int CI, uint32_t ncols, const fvec4 *__restrict__ in,
             const uint32_t tile = q >> 8, w = q & 255u, tpr = (uint32_t)N >> 6;
             q = ((tile / tpr) * 4u + (w >> 6)) * (uint32_t)N + (tile % tpr) * 64u + (w & 63u);
         }
-        const int s0 = (int)(col / plane) * CI, s1 = (s0 + CI < N) ? s0 + CI : N;
+        int s0 = (int)(col / plane) * CI;
+        if (UNI) s0 = __builtin_amdgcn_readfirstlane(s0);
+        const int s1 = (s0 + CI < N) ? s0 + CI : N;
         double acc[kSampQ];     // plain sums over this column's sites
 #pragma unroll
         for (int t = 0; t < kSampQ; ++t) acc[t] = 0.0;
-        // in-plane neighbours at distance l (+l, -l): 3D axis 1 (j) and
-        // axis 2 (k); 2D axis 0 (i).  Index 0 is the forward neighbour.
+        // byte offsets within a slice: the site (oc) and its in-plane
+        // neighbours at distance l (+l, -l): 3D axis 1 (j) and axis 2 (k);
+        // 2D axis 0 (i).  Index 0 is the forward neighbour.
+        const uint32_t oc = q * (uint32_t)sizeof(fvec4);
         uint32_t ap[R], am[R], bp[R], bm[R];
         if (ND == 3) {
             const int j = (int)(q / (uint32_t)N), kk = (int)(q - (uint32_t)j * (uint32_t)N);
 #pragma unroll
             for (int l = 1; l <= R; ++l) {
-                ap[l - 1] = (uint32_t)wrap(j + l) * (uint32_t)N + (uint32_t)kk;
-                am[l - 1] = (uint32_t)wrap(j - l) * (uint32_t)N + (uint32_t)kk;
-                bp[l - 1] = (uint32_t)j * (uint32_t)N + (uint32_t)wrap(kk + l);
-                bm[l - 1] = (uint32_t)j * (uint32_t)N + (uint32_t)wrap(kk - l);
+                ap[l - 1] = ((uint32_t)wrap(j + l) * (uint32_t)N + (uint32_t)kk) * 16u;
+                am[l - 1] = ((uint32_t)wrap(j - l) * (uint32_t)N + (uint32_t)kk) * 16u;
+                bp[l - 1] = ((uint32_t)j * (uint32_t)N + (uint32_t)wrap(kk + l)) * 16u;
+                bm[l - 1] = ((uint32_t)j * (uint32_t)N + (uint32_t)wrap(kk - l)) * 16u;
             }
         } else {
 #pragma unroll
             for (int l = 1; l <= R; ++l) {
-                ap[l - 1] = (uint32_t)wrap((int)q + l);
-                am[l - 1] = (uint32_t)wrap((int)q - l);
+                ap[l - 1] = (uint32_t)wrap((int)q + l) * 16u;
+                am[l - 1] = (uint32_t)wrap((int)q - l) * 16u;
                 bp[l - 1] = bm[l - 1] = 0;
             }
         }
         auto load = [&](int sl, Plane &P) {
-            const uint32_t base = (uint32_t)sl * plane;
-            P.c = in[base + q];
+            const char *sp = slice(sl);
+            P.c = ld4(sp, oc);
 #pragma unroll
             for (int l = 0; l < R; ++l) {
-                P.a[2 * l] = xy(base + ap[l]);
-                P.a[2 * l + 1] = xy(base + am[l]);
+                P.a[2 * l] = ld2(sp, ap[l]);
+                P.a[2 * l + 1] = ld2(sp, am[l]);
                 if (ND == 3) {
-                    P.b[2 * l] = xy(base + bp[l]);
-                    P.b[2 * l + 1] = xy(base + bm[l]);
+                    P.b[2 * l] = ld2(sp, bp[l]);
+                    P.b[2 * l + 1] = ld2(sp, bm[l]);
                 }
             }
         };
         fvec2 ring[2 * R + 1];             // phi of slow index sl - R + t
 #pragma unroll
-        for (int t = 0; t < 2 * R + 1; ++t) ring[t] = xy((uint32_t)wrap(s0 - R + t) * plane + q);
+        for (int t = 0; t < 2 * R + 1; ++t) ring[t] = ld2(slice(wrap(s0 - R + t)), oc);
         Plane cur, nxt;
         load(s0, cur);
         for (int sl = s0; sl < s1; ++sl) {
-            const uint32_t base = (uint32_t)sl * plane;
             fvec2 ring_next = ring[2 * R];
             if (sl + 1 < s1) {
                 load(sl + 1, nxt);
-                ring_next = xy((uint32_t)wrap(sl + 1 + R) * plane + q);
+                ring_next = ld2(slice(wrap(sl + 1 + R)), oc);
             }
             const fvec4 c = cur.c;
             fvec2 kf;
@@ -369,7 +294,8 @@ sample_fp32_kernel(int N, int CI, uint32_t ncols, const fvec4 *__restrict__ in,
                 kf.x = F32Math::force(s.alpha, s1x, c.x, t);
                 kf.y = F32Math::force(s.alpha, s2x, c.y, t);
             } else {
-                kf = kin[base + q];
+                kf = *reinterpret_cast<const fvec2 *>(kinb + (size_t)(uint32_t)sl * plane * sizeof(fvec2) +
+                                                      q * (uint32_t)sizeof(fvec2));
             }
             const float vx = c.z - s.half_dt * kf.x, vy = c.w - s.half_dt * kf.y;
             const double a = c.x, b = c.y, va = vx, vb = vy;
@@ -377,7 +303,7 @@ sample_fp32_kernel(int N, int CI, uint32_t ncols, const fvec4 *__restrict__ in,
             acc[1] += b;
             acc[2] += va;
             acc[3] += vb;
-            acc[4] += fabs(atan2(a, b));
+            acc[4] += abs_atan2_fast(a, b, atab);
             acc[5] += sqrt(a * a + b * b);
             // forward differences in energy_kernel's axis order
             const fvec2 fs = ring[R + 1], fa = cur.a[0];
@@ -413,14 +339,133 @@ sample_fp32_kernel(int N, int CI, uint32_t ncols, const fvec4 *__restrict__ in,
         for (int q = 0; q < kSampQ; ++q) part[blockIdx.x * kSampQ + q] = k[q];
 }
 
-__global__ void fold_kernel(const Kahan *__restrict__ part, int nblocks, int Q,
-                            double *__restrict__ out)
+// FP64 form: the six background sums and the three energy components in one
+// pass over the stored (phi, phidot) pairs.  A work-item marches a column
+// along the slowest axis (3D: i, 2D: j) over a chunk of CI planes: the
+// forward neighbour along that axis is the next iteration's centre, kept in
+// registers, so phi is read from HBM once per site, and the in-plane forward
+// neighbours (3D: j + 1, k + 1; 2D: i + 1) are cache hits; the next plane's
+// loads are issued while the current one is summed.  The gradient terms are
+// added in the reference's axis order (3D: i, j, k; 2D: i, j).  Columns,
+// tiles and sums as in sample_fp32_kernel: plain FP64 sums over a column's
+// sites, each entering the work-item's compensated sums.
+template <int ND, bool UNI>
+__global__ void __launch_bounds__(kRedBlock) __attribute__((amdgpu_waves_per_eu(4)))
+sample_fp64_kernel(int N, int CI, uint32_t ncols, const dvec2 *__restrict__ phi,
+                   const dvec2 *__restrict__ vel, double tau2, double T03, double lam_4tau2,
+                   Kahan *__restrict__ part)
 {
-    int q = threadIdx.x;
-    if (q >= Q) return;
-    Kahan a = part[q];
-    for (int b = 1; b < nblocks; ++b) a = kmerge(a, part[b * Q + q]);
-    out[q] = a.s + a.c;
+    __shared__ Kahan sh[kSampQ][kRedBlock / 64];
+    __shared__ double atab[257][2];
+    stage_atan_table(atab);
+    Kahan k[kSampQ];
+#pragma unroll
+    for (int q = 0; q < kSampQ; ++q) k[q] = Kahan{0.0, 0.0};
+    const uint32_t plane = (ND == 3) ? (uint32_t)N * (uint32_t)N : (uint32_t)N;
+    const uint32_t stride = gridDim.x * kRedBlock;
+    uint32_t blk = blockIdx.x;
+    if ((gridDim.x & 7u) == 0) blk = (blk & 7u) * (gridDim.x >> 3) + (blk >> 3);
+    const char *__restrict__ pb = reinterpret_cast<const char *>(phi);
+    const char *__restrict__ vb = reinterpret_cast<const char *>(vel);
+    const size_t slice_b = (size_t)plane * sizeof(dvec2);
+    auto ld = [](const char *sp, uint32_t o) { return *reinterpret_cast<const dvec2 *>(sp + o); };
+    const bool tiled = ND == 3 && (N % 64) == 0;
+    for (uint32_t col = blk * kRedBlock + threadIdx.x; col < ncols; col += stride) {
+        uint32_t q = col % plane;
+        if (tiled) {            // 4 x 64 tiles: band of 4 rows, then 64 columns
+            const uint32_t tile = q >> 8, w = q & 255u, tpr = (uint32_t)N >> 6;
+            q = ((tile / tpr) * 4u + (w >> 6)) * (uint32_t)N + (tile % tpr) * 64u + (w & 63u);
+        }
+        int s0 = (int)(col / plane) * CI;
+        if (UNI) s0 = __builtin_amdgcn_readfirstlane(s0);
+        const int s1 = (s0 + CI < N) ? s0 + CI : N;
+        // byte offsets within a slice: the site and its in-plane forward
+        // neighbours (3D: (j + 1, k) and (j, k + 1); 2D: i + 1)
+        const uint32_t oc = q * (uint32_t)sizeof(dvec2);
+        uint32_t oj = 0, ok;
+        if (ND == 3) {
+            const uint32_t j = q / (uint32_t)N, kk = q - j * (uint32_t)N;
+            ok = (j * (uint32_t)N + (kk + 1 == (uint32_t)N ? 0u : kk + 1)) * 16u;
+            oj = ((j + 1 == (uint32_t)N ? 0u : j + 1) * (uint32_t)N + kk) * 16u;
+        } else {
+            ok = (q + 1 == (uint32_t)N ? 0u : q + 1) * 16u;
+        }
+        double acc[kSampQ];
+#pragma unroll
+        for (int t = 0; t < kSampQ; ++t) acc[t] = 0.0;
+        const char *sp = pb + (size_t)(uint32_t)s0 * slice_b;
+        dvec2 c = ld(sp, oc);
+        int nsl = (s0 + 1 == N) ? 0 : s0 + 1;
+        dvec2 nx = ld(pb + (size_t)(uint32_t)nsl * slice_b, oc);   // slow-axis neighbour
+        dvec2 nj = ND == 3 ? ld(sp, oj) : dvec2{0.0, 0.0};
+        dvec2 nk = ld(sp, ok);
+        dvec2 v = ld(vb + (size_t)(uint32_t)s0 * slice_b, oc);
+        for (int sl = s0; sl < s1; ++sl) {
+            dvec2 nx2 = nx, nj2 = nj, nk2 = nk, v2 = v;
+            if (sl + 1 < s1) {            // the next plane's operands
+                const int sl1 = sl + 1, n2 = (sl1 + 1 == N) ? 0 : sl1 + 1;
+                const char *sp1 = pb + (size_t)(uint32_t)sl1 * slice_b;
+                nx2 = ld(pb + (size_t)(uint32_t)n2 * slice_b, oc);
+                if (ND == 3) nj2 = ld(sp1, oj);
+                nk2 = ld(sp1, ok);
+                v2 = ld(vb + (size_t)(uint32_t)sl1 * slice_b, oc);
+            }
+            const double a = c.x, b = c.y, va = v.x, vv = v.y;
+            acc[0] += a;                                 // s_run.cpp:141-146
+            acc[1] += b;
+            acc[2] += va;
+            acc[3] += vv;
+            acc[4] += abs_atan2_fast(a, b, atab);
+            acc[5] += sqrt(a * a + b * b);
+            double g = 0.0, d1, d2;
+            if (ND == 3) {
+                d1 = nx.x - a, d2 = nx.y - b;
+                g += d1 * d1 + d2 * d2;
+                d1 = nj.x - a, d2 = nj.y - b;
+                g += d1 * d1 + d2 * d2;
+                d1 = nk.x - a, d2 = nk.y - b;
+                g += d1 * d1 + d2 * d2;
+            } else {
+                d1 = nk.x - a, d2 = nk.y - b;
+                g += d1 * d1 + d2 * d2;
+                d1 = nx.x - a, d2 = nx.y - b;
+                g += d1 * d1 + d2 * d2;
+            }
+            const double S = a * a + b * b - tau2 + T03;
+            acc[6] += 0.5 * (va * va + vv * vv);
+            acc[7] += 0.5 * g;
+            acc[8] += lam_4tau2 * S * S;
+            c = nx;
+            nx = nx2;
+            nj = nj2;
+            nk = nk2;
+            v = v2;
+        }
+#pragma unroll
+        for (int t = 0; t < kSampQ; ++t) kadd(k[t], acc[t]);
+    }
+    block_reduce<kSampQ>(k, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < kSampQ; ++q) part[blockIdx.x * kSampQ + q] = k[q];
+}
+
+// quantity blockIdx.x of the block pairs part[b * Q + q]: work-item t
+// merges blocks t, t + 256, ... in order, then a fixed pairwise tree over
+// the work-items (deterministic for a given block count)
+__global__ void __launch_bounds__(256)
+fold_kernel(const Kahan *__restrict__ part, int nblocks, int Q, double *__restrict__ out)
+{
+    __shared__ Kahan sa[256];
+    const int q = blockIdx.x;
+    Kahan a{0.0, 0.0};
+    for (int b = threadIdx.x; b < nblocks; b += 256) a = kmerge(a, part[b * Q + q]);
+    sa[threadIdx.x] = a;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sa[threadIdx.x] = kmerge(sa[threadIdx.x], sa[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = sa[0].s + sa[0].c;
 }
 
 static int red_blocks(uint32_t L)
@@ -431,16 +476,20 @@ static int red_blocks(uint32_t L)
 
 // FP32 form: the nine sums of sample_fp32_kernel for the member's current
 // state, computed once per (state, tau) and kept for the other call
-template <int ND>
+template <int ND, bool UNI>
 static auto sample_kernel_for(int R)
 {
-    return (R == 1)   ? sample_fp32_kernel<ND, 1>
-           : (R == 2) ? sample_fp32_kernel<ND, 2>
-           : (R == 3) ? sample_fp32_kernel<ND, 3>
-                      : sample_fp32_kernel<ND, 4>;
+    return (R == 1)   ? sample_fp32_kernel<ND, 1, UNI>
+           : (R == 2) ? sample_fp32_kernel<ND, 2, UNI>
+           : (R == 3) ? sample_fp32_kernel<ND, 3, UNI>
+                      : sample_fp32_kernel<ND, 4, UNI>;
 }
 
-static int fp32_sums(Context *c, Member &m, double out[kSampQ], const Workspace *ws = nullptr)
+// The nine sums of the member's current state (background, then the energy
+// components), computed once per (state, tau) and kept for the other call:
+// sample_fp64_kernel over the FP64 form's stored pairs, sample_fp32_kernel
+// over the FP32 form's (phi, u) state.
+static int sample_sums(Context *c, Member &m, double out[kSampQ], const Workspace *ws = nullptr)
 {
     if (!m.has_state) {
         set_error("member has no state: call axs_set_member_state first");
@@ -452,12 +501,15 @@ static int fp32_sums(Context *c, Member &m, double out[kSampQ], const Workspace 
     }
     const StepScalarsF f = to_float(make_scalars(c->p, c->tau));
     const int N = m.N, R = f.ncoef - 1;
-    if (R < 1 || R > 4) {
+    if (c->prec == AXS_FP32 && (R < 1 || R > 4)) {
         set_error("stencil radius %d is not supported", R);
         return AXS_ERR_UNSUPPORTED;
     }
     const uint32_t plane = (c->nd == 3) ? (uint32_t)N * (uint32_t)N : (uint32_t)N;
-    // slow-axis chunks as in run_energy
+    // chunks along the slow axis: as few as still give a work-item to every
+    // lane of four work-groups per CU; at most 64 planes per chunk, so that
+    // the work-items of one round march in step and the rows they share are
+    // still in L2
     const uint64_t want = (uint64_t)c->num_cu * 4 * kRedBlock;
     const uint32_t nch = (uint32_t)std::min<uint64_t>((uint64_t)N, (want + plane - 1) / plane);
     const int CI = std::min(64, (int)((N + nch - 1) / nch));
@@ -470,11 +522,22 @@ static int fp32_sums(Context *c, Member &m, double out[kSampQ], const Workspace 
     double *dout = (double *)((char *)wsp + part_b);
     const double lam = (double)c->p.lambda, T0 = (double)c->p.T_initial;
     const double tau2 = c->tau * c->tau, T03 = (T0 * T0) / 3.0, lam_4tau2 = lam / (4.0 * tau2);
-    auto k = (c->nd == 2) ? sample_kernel_for<2>(R) : sample_kernel_for<3>(R);
-    hipLaunchKernelGGL(k, dim3(nb), dim3(kRedBlock), 0, c->stream, N, CI, ncols,
-                       (const fvec4 *)m.state[c->cur][PHI], (const fvec2 *)m.aux_ker, f,
-                       ker_current(c, m) ? 0 : 1, tau2, T03, lam_4tau2, part);
-    hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(64), 0, c->stream, part, nb, kSampQ, dout);
+    const bool uni = plane % 64 == 0;       // every wave's columns share their slices
+    if (c->prec == AXS_FP64) {
+        auto k = (c->nd == 2) ? (uni ? sample_fp64_kernel<2, true> : sample_fp64_kernel<2, false>)
+                              : (uni ? sample_fp64_kernel<3, true> : sample_fp64_kernel<3, false>);
+        hipLaunchKernelGGL(k, dim3(nb), dim3(kRedBlock), 0, c->stream, N, CI, ncols,
+                           (const dvec2 *)m.state[c->cur][PHI], (const dvec2 *)m.state[c->cur][VEL],
+                           tau2, T03, lam_4tau2, part);
+    } else {
+        auto k = (c->nd == 2)
+                     ? (uni ? sample_kernel_for<2, true>(R) : sample_kernel_for<2, false>(R))
+                     : (uni ? sample_kernel_for<3, true>(R) : sample_kernel_for<3, false>(R));
+        hipLaunchKernelGGL(k, dim3(nb), dim3(kRedBlock), 0, c->stream, N, CI, ncols,
+                           (const fvec4 *)m.state[c->cur][PHI], (const fvec2 *)m.aux_ker, f,
+                           ker_current(c, m) ? 0 : 1, tau2, T03, lam_4tau2, part);
+    }
+    hipLaunchKernelGGL(fold_kernel, dim3(kSampQ), dim3(256), 0, c->stream, part, nb, kSampQ, dout);
     AXS_HIP_TRY(hipGetLastError());
     AXS_HIP_TRY(hipMemcpyAsync(m.samp, dout, kSampQ * sizeof(double), hipMemcpyDeviceToHost,
                                c->stream));
@@ -497,82 +560,17 @@ static void energy_scale(const Context *c, const double raw[3], double out[3])
 
 int run_background(Context *c, Member &m, double out[6])
 {
-    if (c->prec == AXS_FP32) {
-        double sums[kSampQ];
-        AXS_TRY(fp32_sums(c, m, sums));
-        std::copy(sums, sums + 6, out);
-        return AXS_OK;
-    }
-    FieldView fv;
-    AXS_TRY(current_fields(c, m, &fv));
-    uint32_t L = (uint32_t)m.L;
-    int nb = red_blocks(L);
-    size_t part_b = (size_t)nb * 6 * sizeof(Kahan);
-    AXS_TRY(ensure_scratch(c, part_b + 64));
-    Kahan *part = (Kahan *)c->scratch;
-    double *dout = (double *)((char *)c->scratch + part_b);
-    if (c->prec == AXS_FP64)
-        hipLaunchKernelGGL((background_kernel<PhiF64, VelF64>), dim3(nb), dim3(kRedBlock), 0,
-                           c->stream, L, PhiF64{(const dvec2 *)fv.phi},
-                           VelF64{(const dvec2 *)fv.vel}, part);
-    else
-        hipLaunchKernelGGL((background_kernel<PhiF32, VelF32>), dim3(nb), dim3(kRedBlock), 0,
-                           c->stream, L, PhiF32{(const fvec4 *)fv.phi},
-                           VelF32{(const fvec2 *)fv.vel}, part);
-    hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(64), 0, c->stream, part, nb, 6, dout);
-    AXS_HIP_TRY(hipGetLastError());
-    AXS_HIP_TRY(hipMemcpyAsync(out, dout, 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+    double sums[kSampQ];
+    AXS_TRY(sample_sums(c, m, sums));
+    std::copy(sums, sums + 6, out);
     return AXS_OK;
 }
 
 int run_energy(Context *c, Member &m, double out[3], const Workspace *ws)
 {
-    if (c->prec == AXS_FP32) {
-        double sums[kSampQ];
-        AXS_TRY(fp32_sums(c, m, sums, ws));
-        energy_scale(c, sums + 6, out);
-        return AXS_OK;
-    }
-    FieldView fv;
-    AXS_TRY(current_fields(c, m, &fv));
-    const int N = m.N;
-    const uint32_t plane = (c->nd == 3) ? (uint32_t)N * (uint32_t)N : (uint32_t)N;
-    // chunks along the slow axis: as few as still give a work-item to every
-    // lane of four work-groups per CU
-    const uint64_t want = (uint64_t)c->num_cu * 4 * kRedBlock;
-    const uint32_t nch = (uint32_t)std::min<uint64_t>((uint64_t)N, (want + plane - 1) / plane);
-    // at most 64 planes per chunk: work-items of one round march in step,
-    // so the row j + 1 they share is still in L2
-    const int CI = std::min(64, (int)((N + nch - 1) / nch));
-    const uint32_t ncols = plane * (uint32_t)((N + CI - 1) / CI);
-    int nb = red_blocks(ncols);
-    size_t part_b = (size_t)nb * 3 * sizeof(Kahan);
-    void *wsp = nullptr;
-    AXS_TRY(workspace_for(c, ws, part_b + 64, &wsp));
-    Kahan *part = (Kahan *)wsp;
-    double *dout = (double *)((char *)wsp + part_b);
-    double lam = (double)c->p.lambda, T0 = (double)c->p.T_initial;
-    double tau2 = c->tau * c->tau;
-    double T03 = (T0 * T0) / 3.0;
-    double lam_4tau2 = lam / (4.0 * tau2);
-    if (c->prec == AXS_FP64) {
-        auto k = (c->nd == 2) ? energy_kernel<2, PhiF64, VelF64> : energy_kernel<3, PhiF64, VelF64>;
-        hipLaunchKernelGGL(k, dim3(nb), dim3(kRedBlock), 0, c->stream, N, CI, ncols,
-                           PhiF64{(const dvec2 *)fv.phi}, VelF64{(const dvec2 *)fv.vel}, tau2, T03,
-                           lam_4tau2, part);
-    } else {
-        auto k = (c->nd == 2) ? energy_kernel<2, PhiF32, VelF32> : energy_kernel<3, PhiF32, VelF32>;
-        hipLaunchKernelGGL(k, dim3(nb), dim3(kRedBlock), 0, c->stream, N, CI, ncols,
-                           PhiF32{(const fvec4 *)fv.phi}, VelF32{(const fvec2 *)fv.vel}, tau2, T03,
-                           lam_4tau2, part);
-    }
-    hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(64), 0, c->stream, part, nb, 3, dout);
-    AXS_HIP_TRY(hipGetLastError());
-    double raw[3];
-    AXS_HIP_TRY(hipMemcpyAsync(raw, dout, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    AXS_HIP_TRY(hipStreamSynchronize(c->stream));
-    energy_scale(c, raw, out);
+    double sums[kSampQ];
+    AXS_TRY(sample_sums(c, m, sums, ws));
+    energy_scale(c, sums + 6, out);
     return AXS_OK;
 }
 

@@ -43,6 +43,10 @@ float axs_host_xi(int ndims, int64_t count, double tau, int N, float space_step)
  * ~2^-100 of a rounding boundary), the same bits as the device build.      */
 void axs_host_atan2_dd(int64_t n, const double *y, const double *x, double *out);
 
+/* |atan2(y[i], x[i])| by the table method the background sums use on the
+   device (within 6e-16 of the exact angle). */
+void axs_host_abs_atan2_fast(int64_t n, const double *y, const double *x, double *out);
+
 /* run_standard (s_run.cpp:26-232) with the parameters of `path`; returns 0
  * on success.  This is what axs_main runs.                                */
 int axs_host_run(const char *path);

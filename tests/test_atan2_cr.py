@@ -129,3 +129,69 @@ def test_glibc_atan2_is_not_correctly_rounded():
     assert np.all(np.abs(libm[off] - cr[off]) <= np.spacing(np.abs(cr[off])))
     for i in off[:20]:
         assert _cr(y[i], x[i]) == cr[i]
+
+
+def _fast(y, x):
+    h = axs.load_host_library()
+    f = h.axs_host_abs_atan2_fast
+    f.argtypes = [ctypes.c_int64] + [ctypes.POINTER(ctypes.c_double)] * 3
+    f.restype = None
+    y = np.ascontiguousarray(y, np.float64).reshape(-1)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1)
+    out = np.empty_like(y)
+    P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+    f(y.size, P(y), P(x), P(out))
+    return out
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_abs_atan2_fast_bound(seed):
+    """The |atan2| of the background sums (table reduction in plain double):
+    within 4.5e-16 and two ulps of the correctly rounded |atan2| on random,
+    wide-range, near-axis, near-diagonal and near-table-midpoint arguments,
+    and on FP32 values (the FP32 form's phi); most results are exact."""
+    rng = np.random.default_rng(seed)
+    y, x = _args(rng, 100000)
+    n = 100000
+    xm = rng.standard_normal(n)
+    mid = (np.arange(n) % 256 + 0.5) / 256 * (1 + rng.uniform(-1e-12, 1e-12, n))
+    y = np.concatenate([y, xm * mid, rng.standard_normal(n).astype(np.float32)])
+    x = np.concatenate([x, xm, rng.standard_normal(n).astype(np.float32)])
+    got = _fast(y, x)
+    want = np.abs(O.atan2_cr(y, x))
+    err = np.abs(got - want)
+    assert err.max() <= 4.5e-16
+    assert np.all(err <= 2 * np.spacing(want))
+    assert (err == 0).mean() > 0.6
+
+
+def test_abs_atan2_fast_special_values():
+    """C99 Annex F values exactly where a component is infinite or NaN or
+    both are zero (NaN for NaN); the other pairs of the grid (one zero,
+    subnormal, tiny or huge components -- scaled into range first -- on the
+    axes and diagonals) within the bound."""
+    ys, xs = np.meshgrid(SPECIAL, SPECIAL, indexing="ij")
+    y, x = ys.ravel(), xs.ravel()
+    got = _fast(y, x)
+    want = np.abs(_dd(y, x))
+    special = ~np.isfinite(y) | ~np.isfinite(x) | ((y == 0) & (x == 0))
+    assert special.sum() > 0 and (~special).sum() > 0
+    np.testing.assert_array_equal(got[special], want[special])
+    err = np.abs(got[~special] - want[~special])
+    assert np.all(err <= 2 * np.spacing(want[~special]))
+
+
+def test_abs_atan2_fast_extreme_magnitudes():
+    """Pairs whose larger magnitude lies outside [2^-1000, 2^1000] (scaled
+    by a common power of two) against the correctly rounded value."""
+    rng = np.random.default_rng(11)
+    n = 20000
+    e = rng.choice(np.r_[np.arange(-1074, -1000), np.arange(1000, 1024)], n)
+    with np.errstate(over="ignore"):
+        y = np.ldexp(rng.uniform(0.5, 1, n) * rng.choice([-1, 1], n), e)
+        x = np.ldexp(rng.uniform(0.5, 1, n) * rng.choice([-1, 1], n), e - rng.integers(-40, 40, n))
+    ok = np.isfinite(x) & np.isfinite(y)
+    y, x = y[ok], x[ok]
+    got = _fast(y, x)
+    want = np.abs(O.atan2_cr(y, x))
+    assert np.all(np.abs(got - want) <= np.maximum(2 * np.spacing(want), 1e-300))

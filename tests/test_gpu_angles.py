@@ -233,3 +233,35 @@ def test_strings_3d_sums_at_the_threshold(thr, N):
     d_libm, d_np = _flag_diffs(p1, p2, 3, N, thr, c_ref)
     print(f"3D N={N} thr={thr}: {n_ref} flagged ({int(above.sum())} engineered above); "
           f"glibc angles would flip {d_libm}, numpy angles {d_np}")
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_background_angle_each_value(prec):
+    """The |atan2(phi1, phi2)| term of the background sums (s_run.cpp:145),
+    value by value: each member of an ensemble has one site at (a, b) and
+    every other site at (0, 1), whose angle is exactly 0, so the member's
+    angle sum is the device's |atan2(a, b)| itself.  Each must lie within
+    two ulps (and 4.5e-16) of the correctly rounded |atan2|; pairs cover
+    every octant, ratios at the arctangent table's midpoints and both
+    slice layouts (sides 64 and 16)."""
+    rng = np.random.default_rng(2024)
+    M = 384
+    Ns = [64] * (M // 2) + [16] * (M // 2)
+    a = rng.standard_normal(M) * np.exp(rng.uniform(-5, 5, M))
+    b = rng.standard_normal(M) * np.exp(rng.uniform(-5, 5, M))
+    q = M // 3
+    b[:q] = rng.choice([-1, 1], q) * rng.uniform(0.1, 10, q)
+    a[:q] = b[:q] * (rng.integers(0, 256, q) + 0.5) / 256 * rng.choice([-1, 1], q)
+    if prec == "fp32":
+        a, b = a.astype(np.float32).astype(np.float64), b.astype(np.float32).astype(np.float64)
+    lat = axs.Lattice(2, members=Ns, precision=prec)
+    for r, N in enumerate(Ns):
+        p1, p2 = np.zeros(N * N), np.ones(N * N)
+        p1[0], p2[0] = a[r], b[r]
+        z = np.zeros(N * N)
+        lat.set_state(p1, p2, z, z, member=r)
+    got = np.array([lat.background(member=r)[4] for r in range(M)])
+    want = np.abs(O.atan2_cr(a, b))
+    err = np.abs(got - want)
+    assert err.max() <= 4.5e-16, (err.max(), a[err.argmax()], b[err.argmax()])
+    assert np.all(err <= 2 * np.spacing(want))

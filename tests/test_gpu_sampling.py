@@ -345,8 +345,11 @@ def test_energy_free_field_conserved():
 
 
 @pytest.mark.parametrize("nd,N,nsteps,prec", [(2, 256, 50, "fp64"), (3, 64, 20, "fp64"),
-                                              (3, 48, 10, "fp64"), (3, 64, 20, "fp32")])
+                                              (3, 48, 10, "fp64"), (3, 64, 20, "fp32"),
+                                              (2, 2048, 5, "fp32")])
 def test_powerspec_linear_bins(nd, N, nsteps, prec):
+    """Row-order binning (bin tables that fit the work-group's memory) and,
+    for the 1024 linear bins of 2048^2, the per-bin binning."""
     lat = _evolved(nd, N, nsteps, prec)
     g = lat.get_state()
     ks_r, pk_r, cnt_r = O.powerspec(g["phi2"], nd, N, 0.5)
@@ -357,8 +360,12 @@ def test_powerspec_linear_bins(nd, N, nsteps, prec):
     np.testing.assert_allclose(pk[sel], pk_r[sel], rtol=1e-9, atol=1e-12 * pk_r[sel].max())
 
 
-@pytest.mark.parametrize("nd,N,nbins", [(2, 256, 64), (3, 64, 64), (3, 48, 20)])
+@pytest.mark.parametrize("nd,N,nbins", [(2, 256, 64), (3, 64, 64), (3, 48, 20), (2, 256, 500),
+                                         (2, 256, 700), (3, 64, 1000)])
 def test_powerspec_log_bins(nd, N, nbins):
+    """64 and 20 log bins; 500 (row-order binning, most bins narrower than a
+    unit shell); 700 and 1000 (too many bins for the row-order binning's
+    work-group memory: the per-bin binning)."""
     lat = _evolved(nd, N, 10)
     g = lat.get_state()
     ks_r, pk_r, cnt_r = O.powerspec_log(g["phi2"], nd, N, 0.5, nbins)
@@ -367,3 +374,15 @@ def test_powerspec_log_bins(nd, N, nbins):
     sel = cnt > 0
     np.testing.assert_allclose(ks[sel], ks_r[sel], rtol=1e-13)
     np.testing.assert_allclose(pk[sel], pk_r[sel], rtol=1e-9, atol=1e-12 * pk_r[sel].max())
+
+
+@pytest.mark.parametrize("nd,N,binning", [(3, 64, "log"), (3, 64, "linear"), (2, 512, "log")])
+def test_powerspec_repeatable(nd, N, binning):
+    """Same state, same bits: the first call also builds the bin table's
+    |k| sums and counts, later calls take them from the context."""
+    lat = _evolved(nd, N, 10)
+    a = lat.powerspec(binning=binning, nbins=64)
+    for _ in range(2):
+        b = lat.powerspec(binning=binning, nbins=64)
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
