@@ -464,6 +464,7 @@ static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ct
     c->num_cu = prop.multiProcessorCount;
     c->tau = 1.0;                                              // physics.cpp:32
     if (const char *f = getenv("AXS_STEP_FAIL_AFTER")) c->step_fail_after = atoi(f);
+    if (const char *f = getenv("AXS_PK_ROCFFT")) c->pk_rocfft = atoi(f) != 0;
     int rc = AXS_OK;
     auto fail = [&](int code) {
         delete h;

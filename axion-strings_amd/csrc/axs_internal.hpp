@@ -154,6 +154,9 @@ struct Context {
     // in place of its (n+1)-th launch (direct loop) or of the graph launch
     // (AXS_STEP_GRAPH); -1 = off.  Read from AXS_STEP_FAIL_AFTER at creation.
     int step_fail_after = -1;
+    // power spectra through rocFFT at every side (AXS_PK_ROCFFT=1 at
+    // creation), for comparing the two transforms
+    bool pk_rocfft = false;
 
     // device scratch for reductions / compaction (grown on demand; growing
     // discards the contents)

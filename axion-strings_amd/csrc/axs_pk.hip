@@ -32,12 +32,15 @@ namespace axs {
 
 struct FftPlan {
     int N = 0, nd = 0, batch = 1;
+    bool own = false;           // the transform of our own (power-of-two sides)
     rocfft_plan plan = nullptr;
     rocfft_execution_info info = nullptr;
     void *work = nullptr;
     size_t work_bytes = 0;
-    double *in = nullptr;       // real input copy (rocFFT may overwrite input)
-    double *out = nullptr;      // complex half spectrum, interleaved
+    double *in = nullptr;       // rocFFT: real input copy (rocFFT may overwrite input)
+    double *out = nullptr;      // rocFFT: complex half spectrum, interleaved
+    double2 *ya = nullptr, *yb = nullptr;   // own: the passes' intermediate arrays
+    double2 *tw = nullptr;      // own: W_N^m, m < N
     ~FftPlan()
     {
         if (plan) rocfft_plan_destroy(plan);
@@ -45,6 +48,9 @@ struct FftPlan {
         if (work) (void)hipFree(work);
         if (in) (void)hipFree(in);
         if (out) (void)hipFree(out);
+        if (ya) (void)hipFree(ya);
+        if (yb) (void)hipFree(yb);
+        if (tw) (void)hipFree(tw);
     }
 };
 
@@ -55,7 +61,7 @@ static int rocfft_ready = 0;
 static int get_plan(Context *c, int N, int batch, FftPlan **out)
 {
     for (FftPlan *p : c->fft_plans)
-        if (p->N == N && p->nd == c->nd && p->batch == batch) {
+        if (!p->own && p->N == N && p->nd == c->nd && p->batch == batch) {
             *out = p;
             return AXS_OK;
         }
@@ -114,27 +120,28 @@ struct PkSrc {
     int kind, comp;
 };
 
+// the value of site m of a source
+__device__ __forceinline__ double pk_value(const PkSrc &s, size_t m)
+{
+    if (s.kind == 0) {
+        const dvec2 w = ((const dvec2 *)s.p)[m];
+        return s.comp ? w.y : w.x;
+    }
+    if (s.kind == 1) {
+        const fvec2 w = ((const fvec2 *)s.p)[2 * m];                    // (phi1, phi2)
+        return (double)(s.comp ? w.y : w.x);
+    }
+    const fvec2 w = ((const fvec2 *)s.p)[m];
+    return (double)(s.comp ? w.y : w.x);
+}
+
 // the selected component of every lattice of a batch (one per blockIdx.y)
 // as contiguous double arrays, lattice b at y + b L
 __global__ void __launch_bounds__(256)
 pack_kernel(uint32_t L, const PkSrc *__restrict__ src, double *__restrict__ y)
 {
     const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < L) {
-        const PkSrc s = src[blockIdx.y];
-        double v;
-        if (s.kind == 0) {
-            const dvec2 w = ((const dvec2 *)s.p)[m];
-            v = s.comp ? w.y : w.x;
-        } else if (s.kind == 1) {
-            const fvec2 w = ((const fvec2 *)s.p)[2 * (size_t)m];      // (phi1, phi2)
-            v = (double)(s.comp ? w.y : w.x);
-        } else {
-            const fvec2 w = ((const fvec2 *)s.p)[m];
-            v = (double)(s.comp ? w.y : w.x);
-        }
-        y[(size_t)blockIdx.y * L + m] = v;
-    }
+    if (m < L) y[(size_t)blockIdx.y * L + m] = pk_value(src[blockIdx.y], m);
 }
 
 struct KahanP {
@@ -488,6 +495,386 @@ fold_rows_kernel(int nbins, uint32_t nblk, const KahanP *__restrict__ part,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Power-of-two sides: the transform in passes of our own that each read and
+// write HBM once, the first reading the field straight from the state and
+// the last binning the modes as it produces them (no pack, no spectrum
+// written, no binning pass).  3D, lattice f[i][j][z], M = N/2:
+//   Z  rows along z (real): g_m = f_2m + i f_2m+1, an M-point complex DFT
+//      G, then X_k = E_k + W_N^k O_k with E_k = (G_k + conj G_{M-k}) / 2,
+//      O_k = -i (G_k - conj G_{M-k}) / 2 (G_M = G_0), for k < M (the plane
+//      k_z = N/2 has |k| >= N/2 and is never binned); written as Y[i][k][j].
+//   Y  rows along j: N-point DFTs, written as Y'[k_z][k_j][i].
+//   X  rows along i: N-point DFTs, then every mode's w |F|^2 (w = 1 at
+//      k_z = 0, else 2 for the Hermitian partner) binned as bin_scan_kernel
+//      does; along a row |k| runs up over the first half and down over the
+//      second, so the 64 lanes of a chunk (N >= 128) still hold each bin
+//      present as one run.
+// 2D (f[j][i]) is Z along i, written as Y[k][j], then the binning pass
+// along j.  A work-group transforms kFftPts complex points in LDS, rows of
+// one length L: radix-4 Stockham stages (one radix-2 stage first when
+// log2 L is odd), each thread holding 16 points in registers between a
+// stage's reads and writes (two barriers per stage, one buffer).  Rows are
+// padded by one point in LDS so that the column reads of the transposed
+// writes fall in distinct banks.  Twiddles: one table of W_N^m, m < N.
+constexpr int kFftPts = 4096;
+constexpr int kFftThr = 256;
+
+__device__ __forceinline__ double2 cmulw(double2 a, double2 w)
+{
+    return double2{fma(a.x, w.x, -(a.y * w.y)), fma(a.x, w.y, a.y * w.x)};
+}
+
+__device__ __forceinline__ void dft2(double2 (&v)[2])
+{
+    const double2 a = v[0], b = v[1];
+    v[0] = double2{a.x + b.x, a.y + b.y};
+    v[1] = double2{a.x - b.x, a.y - b.y};
+}
+
+// forward 4-point DFT (W_4 = -i)
+__device__ __forceinline__ void dft4(double2 (&v)[4])
+{
+    const double2 t0{v[0].x + v[2].x, v[0].y + v[2].y}, t1{v[0].x - v[2].x, v[0].y - v[2].y};
+    const double2 t2{v[1].x + v[3].x, v[1].y + v[3].y};
+    const double2 t3{v[1].y - v[3].y, v[3].x - v[1].x};            // -i (v1 - v3)
+    v[0] = double2{t0.x + t2.x, t0.y + t2.y};
+    v[2] = double2{t0.x - t2.x, t0.y - t2.y};
+    v[1] = double2{t1.x + t3.x, t1.y + t3.y};
+    v[3] = double2{t1.x - t3.x, t1.y - t3.y};
+}
+
+// 16-point DFT as 4 x 4 (Cooley-Tukey): DFT4 over v[b + 4a] for each b,
+// twiddles W_16^{bp}, DFT4 over b; output p + 4q
+__device__ __forceinline__ void dft16(double2 (&v)[16])
+{
+    constexpr double c1 = 0x1.d906bcf328d46p-1, s1 = 0x1.87de2a6aea963p-2;   // cos, sin pi/8
+    constexpr double h = 0x1.6a09e667f3bcdp-1;                               // sqrt(1/2)
+    double2 y[4][4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        double2 t[4] = {v[b], v[b + 4], v[b + 8], v[b + 12]};
+        dft4(t);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) y[b][p] = t[p];
+    }
+    // W_16^1 = (c1, -s1), ^2 = (h, -h), ^3 = (s1, -c1), ^4 = -i, ^6 = (-h, -h), ^9 = (-c1, s1)
+    y[1][1] = cmulw(y[1][1], double2{c1, -s1});
+    y[1][2] = cmulw(y[1][2], double2{h, -h});
+    y[1][3] = cmulw(y[1][3], double2{s1, -c1});
+    y[2][1] = cmulw(y[2][1], double2{h, -h});
+    y[2][2] = double2{y[2][2].y, -y[2][2].x};
+    y[2][3] = cmulw(y[2][3], double2{-h, -h});
+    y[3][1] = cmulw(y[3][1], double2{s1, -c1});
+    y[3][2] = cmulw(y[3][2], double2{-h, -h});
+    y[3][3] = cmulw(y[3][3], double2{-c1, s1});
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        double2 t[4] = {y[0][p], y[1][p], y[2][p], y[3][p]};
+        dft4(t);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[p + 4 * q] = t[q];
+    }
+}
+
+// LDS position of point i of a row: one pad every 16 points, so that the
+// first stage's strided writes fall in distinct banks
+__device__ __forceinline__ int fpad(int i) { return i + (i >> 4); }
+
+// row stride in LDS of rows of L points (padded; the odd extra point keeps
+// the rows' starts in different banks)
+template <int L>
+constexpr int fft_ls() { return L + L / 16 + 1; }
+
+// One radix-RAD Stockham stage over all rows in buf (rows of L points at
+// stride LS, kFftPts points): Ns = product of the earlier radices; butterfly
+// j of a row reads points j + k L/RAD, twiddles them by W_L^{(j mod Ns) k
+// L/(Ns RAD)} = tw[(...) ts], and writes point k to (j - j mod Ns) RAD +
+// j mod Ns + k Ns.
+template <int L, int RAD, int PTS>
+__device__ __forceinline__ void fft_stage(double2 *buf, const double2 *__restrict__ tw, int ts,
+                                          int Ns)
+{
+    constexpr int Q = L / RAD, LS = fft_ls<L>(), THR = PTS / 16;
+    constexpr int NB = PTS / RAD / THR;
+    double2 v[NB][RAD];
+    int ob[NB], oj[NB];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int b = threadIdx.x + q * THR;
+        const int row = b / Q, j = b - row * Q;
+        const int jm = j & (Ns - 1);
+        const double2 *rb = buf + row * LS;
+#pragma unroll
+        for (int k = 0; k < RAD; ++k) v[q][k] = rb[fpad(j + k * Q)];
+        if (Ns > 1) {
+            const int step = jm * (L / (Ns * RAD)) * ts;
+#pragma unroll
+            for (int k = 1; k < RAD; ++k) v[q][k] = cmulw(v[q][k], tw[k * step]);
+        }
+        if constexpr (RAD == 16)
+            dft16(v[q]);
+        else if constexpr (RAD == 4)
+            dft4(v[q]);
+        else
+            dft2(v[q]);
+        ob[q] = row * LS;
+        oj[q] = (j - jm) * RAD + jm;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+#pragma unroll
+        for (int k = 0; k < RAD; ++k) buf[ob[q] + fpad(oj[q] + k * Ns)] = v[q][k];
+    __syncthreads();
+}
+
+// forward DFT of every row in buf (PTS points, PTS / 16 threads);
+// tw[m ts] = W_L^m.  Radix-16 stages while 16 divides what is left, then
+// radix 4 and 2.
+template <int L, int PTS = kFftPts>
+__device__ __forceinline__ void fft_rows(double2 *buf, const double2 *__restrict__ tw, int ts)
+{
+    constexpr int n = __builtin_ctz(L);
+    constexpr int n16 = n / 4, rest = n % 4;
+    int Ns = 1;
+#pragma unroll
+    for (int t = 0; t < n16; ++t, Ns *= 16) fft_stage<L, 16, PTS>(buf, tw, ts, Ns);
+    if constexpr (rest >= 2) {
+        fft_stage<L, 4, PTS>(buf, tw, ts, Ns);
+        Ns *= 4;
+    }
+    if constexpr ((rest & 1) != 0) fft_stage<L, 2, PTS>(buf, tw, ts, Ns);
+}
+
+// Z pass: rows of N reals of lattice blockIdx.y (rpl rows: N^2 in 3D, N in
+// 2D), R = kFftPts / M rows per work-group; writes the k < M half of each
+// row as Y[lattice][i][k][j] (3D) or Y[lattice][k][j] (2D)
+template <int ND, int N, class V>
+__device__ __forceinline__ void fft_z_load(double2 *buf, const V *__restrict__ pv, int comp,
+                                           size_t site0)
+{
+    constexpr int M = N / 2, LS = fft_ls<M>();
+    // the source pointer came from memory (a generic pointer): name it a
+    // global one, so that the loads are global loads that need not wait for
+    // each other or for the LDS stores
+    using GV = const __attribute__((address_space(1))) V;
+    GV *p = (GV *)pv;
+    // one site per lane (consecutive lanes, consecutive sites: whole lines),
+    // even sites into the real parts and odd ones into the imaginary parts
+#pragma unroll 8
+    for (int q = 0; q < 2 * kFftPts / kFftThr; ++q) {
+        const int e = threadIdx.x + q * kFftThr;
+        const int r = e / N, z = e - r * N;
+        const V w = p[site0 + (size_t)e];
+        reinterpret_cast<double *>(buf + r * LS + fpad(z >> 1))[z & 1] =
+            (double)(comp ? w.y : w.x);
+    }
+}
+
+template <int ND, int N>
+__global__ void __launch_bounds__(kFftThr)
+fft_z_kernel(const PkSrc *__restrict__ src, double2 *__restrict__ y,
+             const double2 *__restrict__ tw)
+{
+    constexpr int M = N / 2, R = kFftPts / M, LS = fft_ls<M>();
+    constexpr uint32_t rpl = (ND == 3) ? (uint32_t)N * N : (uint32_t)N;
+    __shared__ double2 buf[R * LS];
+    const uint32_t b = blockIdx.y, row0 = blockIdx.x * R;
+    const PkSrc sc = src[b];
+    const size_t site0 = (size_t)row0 * N;
+    if (sc.kind == 0)
+        fft_z_load<ND, N>(buf, (const dvec2 *)sc.p, sc.comp, site0);
+    else if (sc.kind == 1)      // (phi1, phi2, u1, u2): the first pair of each quadruple
+        fft_z_load<ND, N>(buf, (const fvec4 *)sc.p, sc.comp, site0);
+    else
+        fft_z_load<ND, N>(buf, (const fvec2 *)sc.p, sc.comp, site0);
+    __syncthreads();
+    fft_rows<M>(buf, tw, 2);                         // W_M^m = W_N^2m
+#pragma unroll 4
+    for (int q = 0; q < kFftPts / kFftThr; ++q) {
+        const int e = threadIdx.x + q * kFftThr;
+        const int r = e % R, k = e / R;
+        const double2 a = buf[r * LS + fpad(k)], c = buf[r * LS + fpad((M - k) & (M - 1))];
+        const double2 E{0.5 * (a.x + c.x), 0.5 * (a.y - c.y)};
+        const double2 O{0.5 * (a.y + c.y), -0.5 * (a.x - c.x)};
+        const double2 w = cmulw(O, tw[k]);
+        const double2 X{E.x + w.x, E.y + w.y};
+        const uint32_t rl = row0 + (uint32_t)r;
+        size_t idx;
+        if (ND == 3) {
+            const uint32_t i = rl / (uint32_t)N, j = rl - i * (uint32_t)N;
+            idx = (((size_t)b * N + i) * M + (uint32_t)k) * N + j;
+        } else {
+            idx = ((size_t)b * M + (uint32_t)k) * N + rl;
+        }
+        y[idx] = X;
+    }
+    (void)rpl;
+}
+
+// Y pass (3D): rows Y[b][i][k_z][*] along j, R = kFftPtsY / N rows of
+// consecutive i per work-group (eight at N = 1024: whole 128-byte lines in
+// the transposed write); writes Y'[b][k_z][k_j][i]
+constexpr int kFftPtsY = 8192;
+
+template <int N>
+__global__ void __launch_bounds__(kFftPtsY / 16)
+fft_y_kernel(const double2 *__restrict__ ya, double2 *__restrict__ yb,
+             const double2 *__restrict__ tw)
+{
+    constexpr int M = N / 2, R = kFftPtsY / N, LS = fft_ls<N>(), GI = N / R;
+    constexpr int THR = kFftPtsY / 16;
+    __shared__ double2 buf[R * LS];
+    const uint32_t gq = blockIdx.x;
+    const uint32_t ig = gq % GI, t = gq / GI, kz = t % M, b = t / M;
+    const uint32_t i0 = ig * R;
+#pragma unroll 4
+    for (int q = 0; q < kFftPtsY / THR; ++q) {
+        const int e = threadIdx.x + q * THR;
+        const int r = e / N, j = e - r * N;
+        buf[r * LS + fpad(j)] = ya[(((size_t)b * N + i0 + r) * M + kz) * N + j];
+    }
+    __syncthreads();
+    fft_rows<N, kFftPtsY>(buf, tw, 1);
+#pragma unroll 4
+    for (int q = 0; q < kFftPtsY / THR; ++q) {
+        const int e = threadIdx.x + q * THR;
+        const int r = e % R, kj = e / R;
+        yb[(((size_t)b * M + kz) * N + kj) * N + i0 + r] = buf[r * LS + fpad(kj)];
+    }
+}
+
+// Last pass with the binning: rows along the last axis (3D: Y'[b][k_z][k_j]
+// [*] along i; 2D: Y[b][k][*] along j), R = kFftPts / N rows per group; each
+// work-group takes a contiguous run of groups (blockIdx.y = lattice); each
+// wave bins rows wave, wave + 4, ... of a group into its accumulators as
+// bin_scan_kernel does, and each work-group writes one partial per bin at
+// the end.
+template <int ND, int N>
+__global__ void __launch_bounds__(kFftThr)
+fft_bin_kernel(const double2 *__restrict__ y, const double2 *__restrict__ tw, int nbins,
+               const uint32_t *__restrict__ hi2g, const uint16_t *__restrict__ shellg,
+               uint32_t rpl, KahanP *__restrict__ part)
+{
+    constexpr int R = kFftPts / N, LS = fft_ls<N>(), half = N / 2;
+    constexpr int W = kFftThr / 64;
+    __shared__ double2 buf[R * LS];
+    extern __shared__ __align__(16) unsigned char smem[];
+    KahanP *acc = reinterpret_cast<KahanP *>(smem);                    // [wave][bin]
+    uint32_t *hi2 = reinterpret_cast<uint32_t *>(acc + W * nbins);
+    uint16_t *shell = reinterpret_cast<uint16_t *>(hi2 + nbins);
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int i = threadIdx.x; i < nbins; i += kFftThr) {
+        hi2[i] = hi2g[i];
+        for (int w = 0; w < W; ++w) acc[w * nbins + i] = KahanP{0.0, 0.0};
+    }
+    for (int i = threadIdx.x; i < half; i += kFftThr) shell[i] = shellg[i];
+    __syncthreads();
+    KahanP *wacc = acc + wave * nbins;
+    const double2 *yl = y + (size_t)blockIdx.y * rpl * N;
+    const uint32_t top = (uint32_t)half * (uint32_t)half;
+    const uint64_t upto = (2ull << lane) - 1;
+    const uint32_t ngroups = rpl / R;
+    const uint32_t g0 = (uint32_t)(((uint64_t)ngroups * blockIdx.x) / gridDim.x);
+    const uint32_t g1 = (uint32_t)(((uint64_t)ngroups * (blockIdx.x + 1)) / gridDim.x);
+    for (uint32_t gq = g0; gq < g1; ++gq) {
+        const uint32_t row0 = gq * R;
+        // a group whose rows all lie outside |k| < N/2 is not transformed
+        uint32_t rmin = 0xffffffffu;
+        for (int r = 0; r < R; ++r) {
+            const uint32_t g = row0 + (uint32_t)r;
+            uint32_t r2;
+            if (ND == 3) {
+                const uint32_t kz = g / (uint32_t)N, kj = g - kz * (uint32_t)N;
+                const int fj = (int)kj < half ? (int)kj : (int)kj - N;
+                r2 = kz * kz + (uint32_t)(fj * fj);
+            } else {
+                r2 = g * g;
+            }
+            rmin = r2 < rmin ? r2 : rmin;
+        }
+        if (rmin >= top) continue;
+#pragma unroll 4
+        for (int q = 0; q < kFftPts / kFftThr; ++q) {
+            const int e = threadIdx.x + q * kFftThr;
+            const int r = e / N, x = e - r * N;
+            buf[r * LS + fpad(x)] = yl[(size_t)(row0 + r) * N + x];
+        }
+        __syncthreads();
+        fft_rows<N>(buf, tw, 1);
+        for (int r = wave; r < R; r += W) {
+            const uint32_t g = row0 + (uint32_t)r;
+            uint32_t r2;
+            bool zero_plane;
+            if (ND == 3) {
+                const uint32_t kz = g / (uint32_t)N, kj = g - kz * (uint32_t)N;
+                const int fj = (int)kj < half ? (int)kj : (int)kj - N;
+                r2 = kz * kz + (uint32_t)(fj * fj);
+                zero_plane = kz == 0;
+            } else {
+                r2 = g * g;
+                zero_plane = g == 0;
+            }
+            if (r2 >= top) continue;                 // the whole row has |k| >= N/2
+            for (int x0 = 0; x0 < N; x0 += 64) {
+                // chunks lie in one half of the row (N >= 128): the smallest
+                // |f| of the chunk decides whether any lane is inside
+                const int fmin = x0 < half ? x0 : N - (x0 + 63);
+                if (r2 + (uint32_t)(fmin * fmin) >= top) continue;
+                const int x = x0 + lane;
+                const int fx = x < half ? x : x - N;
+                const uint32_t k2 = r2 + (uint32_t)(fx * fx);
+                const bool on = k2 < top && k2 > 0;
+                int b;
+                double v = 0.0;
+                if (on) {
+                    uint32_t sh = (uint32_t)__builtin_amdgcn_sqrtf((float)k2);
+                    if (sh * sh > k2) --sh;
+                    if ((sh + 1) * (sh + 1) <= k2) ++sh;
+                    b = shell[sh];
+                    while (hi2[b] <= k2) ++b;
+                    const double2 f = buf[r * LS + fpad(x)];
+                    const double p2 = f.x * f.x + f.y * f.y;
+                    v = zero_plane ? p2 : 2.0 * p2;
+                } else {
+                    b = (k2 == 0) ? -1 : nbins;
+                }
+                const int bprev = __builtin_amdgcn_update_dpp(-2, b, 0x138, 0xf, 0xf, false);
+                const uint64_t heads = __ballot(bprev != b);
+                const int start = 63 - __clzll(heads & upto);
+                const int len = lane - start;
+                double u;
+                u = dpp_take<0x111, 0xf>(v);
+                if (len >= 1) v += u;
+                u = dpp_take<0x112, 0xf>(v);
+                if (len >= 2) v += u;
+                u = dpp_take<0x114, 0xf>(v);
+                if (len >= 4) v += u;
+                u = dpp_take<0x118, 0xf>(v);
+                if (len >= 8) v += u;
+                u = dpp_take<0x142, 0xa>(v);
+                if (len > (lane & 15)) v += u;
+                u = dpp_take<0x143, 0xc>(v);
+                if (len > (lane & 31)) v += u;
+                const bool last = lane == 63 || (((heads >> lane) >> 1) & 1ull);
+                if (last && b >= 0 && b < nbins) {
+                    KahanP a = wacc[b];
+                    kaddp(a, v);
+                    wacc[b] = a;
+                }
+            }
+        }
+        __syncthreads();                             // before the next group's loads
+    }
+    for (int i = threadIdx.x; i < nbins; i += kFftThr) {
+        KahanP a = acc[i];
+        for (int w = 1; w < W; ++w) a = kmergep(a, acc[w * nbins + i]);
+        part[((size_t)blockIdx.y * nbins + i) * gridDim.x + blockIdx.x] = a;
+    }
+}
+
 // bins of a lattice of side N as ranges [lo, hi) of the integer |k|^2, and
 // the work items (bounding-box rows in chunks of kPkRows) of each bin
 struct BinLayout {
@@ -574,70 +961,97 @@ static uint32_t scan_blocks(int nd, int N)
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, 2048));
 }
 
-// Row-order binning of the n half spectra in plan->out: the power sums on
-// every call; the |k| sums and mode counts, which depend only on the bin
-// table, from the context's cache (one geometry pass the first time).
-static int bin_rows(Context *c, FftPlan *plan, int n, const BinLayout &B,
-                    std::vector<double> &hpk, std::vector<double> &hks,
-                    std::vector<long long> &hcnt)
+// Device tables and partial sums of the row-order binning of n spectra
+// (nblk work-groups per spectrum), in the context's scratch.
+struct ScanTabs {
+    uint32_t *hi = nullptr;
+    uint16_t *shell = nullptr;
+    PkSrc *src = nullptr;
+    KahanP *part = nullptr;
+    long long *pcnt = nullptr;
+    double *out = nullptr;
+    long long *cnt = nullptr;
+};
+
+static int scan_tabs(Context *c, const BinLayout &B, int n, uint32_t nblk, ScanTabs &t)
 {
-    const int N = B.N, nbins = B.nbins;
-    const size_t nb = (size_t)nbins;
-    const uint32_t nrows = (c->nd == 3) ? (uint32_t)N * (uint32_t)N : (uint32_t)N;
-    const uint32_t nblk = scan_blocks(c->nd, N);
+    const int N = B.N;
+    const size_t nb = (size_t)B.nbins;
+    const uint32_t gblk = scan_blocks(c->nd, N);                  // geometry pass
+    const size_t nparts = std::max<size_t>((size_t)n * nblk, gblk);
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t o_hi = 0, o_sh = align(nb * 4);
-    const size_t o_part = o_sh + align((size_t)(N / 2) * 2);
-    const size_t o_pcnt = o_part + align((size_t)n * nb * nblk * sizeof(KahanP));
-    const size_t o_out = o_pcnt + align(nb * nblk * sizeof(long long));
+    const size_t o_src = o_sh + align((size_t)(N / 2) * 2);
+    const size_t o_part = o_src + align((size_t)n * sizeof(PkSrc));
+    const size_t o_pcnt = o_part + align(nb * nparts * sizeof(KahanP));
+    const size_t o_out = o_pcnt + align(nb * gblk * sizeof(long long));
     const size_t o_cnt = o_out + align((size_t)n * nb * 8);
     const size_t need = o_cnt + align(nb * 8);
     AXS_TRY(ensure_scratch(c, need));
     char *sp = (char *)c->scratch;
-    uint32_t *dhi = (uint32_t *)(sp + o_hi);
-    uint16_t *dsh = (uint16_t *)(sp + o_sh);
-    KahanP *part = (KahanP *)(sp + o_part);
-    long long *pcnt = (long long *)(sp + o_pcnt);
-    double *dout = (double *)(sp + o_out);
-    long long *dcnt = (long long *)(sp + o_cnt);
-    AXS_HIP_TRY(hipMemcpyAsync(dhi, B.hi32.data(), nb * 4, hipMemcpyHostToDevice, c->stream));
-    AXS_HIP_TRY(hipMemcpyAsync(dsh, B.shell.data(), (size_t)(N / 2) * 2, hipMemcpyHostToDevice,
-                               c->stream));
-    const double2 *F = (const double2 *)plan->out;
+    t.hi = (uint32_t *)(sp + o_hi);
+    t.shell = (uint16_t *)(sp + o_sh);
+    t.src = (PkSrc *)(sp + o_src);
+    t.part = (KahanP *)(sp + o_part);
+    t.pcnt = (long long *)(sp + o_pcnt);
+    t.out = (double *)(sp + o_out);
+    t.cnt = (long long *)(sp + o_cnt);
+    AXS_HIP_TRY(hipMemcpyAsync(t.hi, B.hi32.data(), nb * 4, hipMemcpyHostToDevice, c->stream));
+    AXS_HIP_TRY(hipMemcpyAsync(t.shell, B.shell.data(), (size_t)(N / 2) * 2,
+                               hipMemcpyHostToDevice, c->stream));
+    return AXS_OK;
+}
 
-    Context::PkGeom *g = nullptr;
+// the bin table's |k| sums and mode counts: from the context, or by the
+// geometry pass of bin_scan_kernel (which reads no spectrum) the first time
+static int bin_geometry(Context *c, const BinLayout &B, const ScanTabs &t,
+                        Context::PkGeom **out)
+{
+    const int N = B.N, nbins = B.nbins;
+    const size_t nb = (size_t)nbins;
     for (auto &e : c->pk_geom)
-        if (e.N == N && e.nd == c->nd && e.binning == B.binning && e.nbins == nbins) g = &e;
-    if (!g) {
-        Context::PkGeom e;
-        e.N = N;
-        e.nd = c->nd;
-        e.binning = B.binning;
-        e.nbins = nbins;
-        e.ks.assign(nb, 0.0);
-        e.cnt.assign(nb, 0);
-        auto kg = (c->nd == 3) ? bin_scan_kernel<3, true> : bin_scan_kernel<2, true>;
-        hipLaunchKernelGGL(kg, dim3(nblk, 1), dim3(64 * kScanWaves), scan_lds_bytes(N, nbins, true),
-                           c->stream, N, F, nbins, dhi, dsh, nrows, part, pcnt);
-        hipLaunchKernelGGL(fold_rows_kernel, dim3((unsigned)nbins, 1), dim3(256), 0, c->stream,
-                           nbins, nblk, part, pcnt, dout, dcnt);
-        AXS_HIP_TRY(hipGetLastError());
-        AXS_HIP_TRY(hipMemcpyAsync(e.ks.data(), dout, nb * 8, hipMemcpyDeviceToHost, c->stream));
-        AXS_HIP_TRY(hipMemcpyAsync(e.cnt.data(), dcnt, nb * 8, hipMemcpyDeviceToHost, c->stream));
-        AXS_HIP_TRY(hipStreamSynchronize(c->stream));
-        c->pk_geom.push_back(std::move(e));
-        g = &c->pk_geom.back();
-    }
-    auto kd = (c->nd == 3) ? bin_scan_kernel<3, false> : bin_scan_kernel<2, false>;
-    hipLaunchKernelGGL(kd, dim3(nblk, (unsigned)n), dim3(64 * kScanWaves),
-                       scan_lds_bytes(N, nbins, false), c->stream, N, F, nbins, dhi, dsh, nrows,
-                       part, (long long *)nullptr);
+        if (e.N == N && e.nd == c->nd && e.binning == B.binning && e.nbins == nbins) {
+            *out = &e;
+            return AXS_OK;
+        }
+    Context::PkGeom e;
+    e.N = N;
+    e.nd = c->nd;
+    e.binning = B.binning;
+    e.nbins = nbins;
+    e.ks.assign(nb, 0.0);
+    e.cnt.assign(nb, 0);
+    const uint32_t nrows = (c->nd == 3) ? (uint32_t)N * (uint32_t)N : (uint32_t)N;
+    const uint32_t gblk = scan_blocks(c->nd, N);
+    auto kg = (c->nd == 3) ? bin_scan_kernel<3, true> : bin_scan_kernel<2, true>;
+    hipLaunchKernelGGL(kg, dim3(gblk, 1), dim3(64 * kScanWaves), scan_lds_bytes(N, nbins, true),
+                       c->stream, N, (const double2 *)nullptr, nbins, t.hi, t.shell, nrows,
+                       t.part, t.pcnt);
+    hipLaunchKernelGGL(fold_rows_kernel, dim3((unsigned)nbins, 1), dim3(256), 0, c->stream, nbins,
+                       gblk, t.part, t.pcnt, t.out, t.cnt);
+    AXS_HIP_TRY(hipGetLastError());
+    AXS_HIP_TRY(hipMemcpyAsync(e.ks.data(), t.out, nb * 8, hipMemcpyDeviceToHost, c->stream));
+    AXS_HIP_TRY(hipMemcpyAsync(e.cnt.data(), t.cnt, nb * 8, hipMemcpyDeviceToHost, c->stream));
+    AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+    c->pk_geom.push_back(std::move(e));
+    *out = &c->pk_geom.back();
+    return AXS_OK;
+}
+
+// fold the n x nbins x nblk power partials, copy the sums, and the
+// geometry's |k| sums and counts beside them
+static int bin_finish(Context *c, int n, const BinLayout &B, const ScanTabs &t, uint32_t nblk,
+                      const Context::PkGeom *g, std::vector<double> &hpk,
+                      std::vector<double> &hks, std::vector<long long> &hcnt)
+{
+    const int nbins = B.nbins;
+    const size_t nb = (size_t)nbins;
     hipLaunchKernelGGL(fold_rows_kernel, dim3((unsigned)nbins, (unsigned)n), dim3(256), 0,
-                       c->stream, nbins, nblk, part, (const long long *)nullptr, dout,
+                       c->stream, nbins, nblk, t.part, (const long long *)nullptr, t.out,
                        (long long *)nullptr);
     AXS_HIP_TRY(hipGetLastError());
     hpk.assign((size_t)n * nb, 0.0);
-    AXS_HIP_TRY(hipMemcpyAsync(hpk.data(), dout, hpk.size() * 8, hipMemcpyDeviceToHost,
+    AXS_HIP_TRY(hipMemcpyAsync(hpk.data(), t.out, hpk.size() * 8, hipMemcpyDeviceToHost,
                                c->stream));
     AXS_HIP_TRY(hipStreamSynchronize(c->stream));
     hks.resize((size_t)n * nb);
@@ -649,6 +1063,132 @@ static int bin_rows(Context *c, FftPlan *plan, int n, const BinLayout &B,
     return AXS_OK;
 }
 
+// Row-order binning of the n half spectra in plan->out (rocFFT's output).
+static int bin_rows(Context *c, FftPlan *plan, int n, const BinLayout &B,
+                    std::vector<double> &hpk, std::vector<double> &hks,
+                    std::vector<long long> &hcnt)
+{
+    const int N = B.N, nbins = B.nbins;
+    const uint32_t nrows = (c->nd == 3) ? (uint32_t)N * (uint32_t)N : (uint32_t)N;
+    const uint32_t nblk = scan_blocks(c->nd, N);
+    ScanTabs t;
+    AXS_TRY(scan_tabs(c, B, n, nblk, t));
+    Context::PkGeom *g = nullptr;
+    AXS_TRY(bin_geometry(c, B, t, &g));
+    auto kd = (c->nd == 3) ? bin_scan_kernel<3, false> : bin_scan_kernel<2, false>;
+    hipLaunchKernelGGL(kd, dim3(nblk, (unsigned)n), dim3(64 * kScanWaves),
+                       scan_lds_bytes(N, nbins, false), c->stream, N,
+                       (const double2 *)plan->out, nbins, t.hi, t.shell, nrows, t.part,
+                       (long long *)nullptr);
+    return bin_finish(c, n, B, t, nblk, g, hpk, hks, hcnt);
+}
+
+// sides the transform of our own takes: powers of two, 128 .. 1024 (3D) or
+// 4096 (2D), with a bin table that fits the binning pass's LDS
+static bool own_fft_ok(const Context *c, int N, const BinLayout &B)
+{
+    if (c->pk_rocfft || (N & (N - 1)) != 0 || N < 128 || N > (c->nd == 3 ? 1024 : 4096))
+        return false;
+    const size_t lds = (size_t)(kFftThr / 64) * B.nbins * sizeof(KahanP) +
+                       (size_t)B.nbins * 4 + (size_t)(N / 2) * 2;
+    return B.scan && lds <= kScanLdsMax;
+}
+
+static int get_own(Context *c, int N, int batch, FftPlan **out)
+{
+    for (FftPlan *p : c->fft_plans)
+        if (p->own && p->N == N && p->nd == c->nd && p->batch == batch) {
+            *out = p;
+            return AXS_OK;
+        }
+    FftPlan *p = new FftPlan();
+    p->own = true;
+    p->N = N;
+    p->nd = c->nd;
+    p->batch = batch;
+    const size_t M = (size_t)N / 2;
+    const size_t na = (size_t)batch * ((c->nd == 3) ? (size_t)N * M * N : M * N);
+    hipError_t e = hipMalloc(&p->ya, na * sizeof(double2));
+    if (e == hipSuccess && c->nd == 3) e = hipMalloc(&p->yb, na * sizeof(double2));
+    if (e == hipSuccess) e = hipMalloc(&p->tw, (size_t)N * sizeof(double2));
+    if (e != hipSuccess) {
+        delete p;
+        set_error("power-spectrum buffers: %s", hipGetErrorString(e));
+        return AXS_ERR_OUT_OF_MEMORY;
+    }
+    std::vector<double2> tw(N);
+    for (int m = 0; m < N; ++m) {
+        const long double a = 2.0L * 3.141592653589793238462643383279502884L * m / N;
+        tw[m] = double2{(double)cosl(a), (double)-sinl(a)};
+    }
+    e = hipMemcpy(p->tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        delete p;
+        set_error("power-spectrum twiddles: %s", hipGetErrorString(e));
+        return AXS_ERR_HIP;
+    }
+    c->fft_plans.push_back(p);
+    *out = p;
+    return AXS_OK;
+}
+
+template <int N>
+static void launch_own(Context *c, const FftPlan *p, int n, const BinLayout &B,
+                       const ScanTabs &t, uint32_t nblk)
+{
+    constexpr int M = N / 2;
+    if (c->nd == 3) {
+        const uint32_t rpl = (uint32_t)N * N;
+        constexpr int RZ = kFftPts / M, RY = kFftPtsY / N;
+        hipLaunchKernelGGL((fft_z_kernel<3, N>), dim3(rpl / RZ, (unsigned)n), dim3(kFftThr), 0,
+                           c->stream, t.src, p->ya, p->tw);
+        hipLaunchKernelGGL((fft_y_kernel<N>), dim3((uint32_t)n * M * (N / RY)),
+                           dim3(kFftPtsY / 16), 0, c->stream, p->ya, p->yb, p->tw);
+        const size_t lds = (size_t)(kFftThr / 64) * B.nbins * sizeof(KahanP) +
+                           (size_t)B.nbins * 4 + (size_t)M * 2;
+        hipLaunchKernelGGL((fft_bin_kernel<3, N>), dim3(nblk, (unsigned)n), dim3(kFftThr), lds,
+                           c->stream, p->yb, p->tw, B.nbins, t.hi, t.shell, (uint32_t)(M * N),
+                           t.part);
+    } else {
+        const uint32_t rpl = (uint32_t)N;
+        constexpr int RZ = kFftPts / M;
+        hipLaunchKernelGGL((fft_z_kernel<2, N>), dim3(rpl / RZ, (unsigned)n), dim3(kFftThr), 0,
+                           c->stream, t.src, p->ya, p->tw);
+        const size_t lds = (size_t)(kFftThr / 64) * B.nbins * sizeof(KahanP) +
+                           (size_t)B.nbins * 4 + (size_t)M * 2;
+        hipLaunchKernelGGL((fft_bin_kernel<2, N>), dim3(nblk, (unsigned)n), dim3(kFftThr), lds,
+                           c->stream, p->ya, p->tw, B.nbins, t.hi, t.shell, (uint32_t)M, t.part);
+    }
+}
+
+// The n spectra of side N by the transform of our own (own_fft_ok).
+static int pk_raw_own(Context *c, const PkSrc *srcs, int n, int N, const BinLayout &B,
+                      std::vector<double> &hpk, std::vector<double> &hks,
+                      std::vector<long long> &hcnt)
+{
+    FftPlan *p = nullptr;
+    AXS_TRY(get_own(c, N, n, &p));
+    const uint32_t rows = (c->nd == 3) ? (uint32_t)(N / 2) * (uint32_t)N : (uint32_t)(N / 2);
+    const uint32_t groups = rows / (uint32_t)(kFftPts / N);
+    const uint32_t nblk = std::max<uint32_t>(1, std::min<uint32_t>(groups, 2048));
+    ScanTabs t;
+    AXS_TRY(scan_tabs(c, B, n, nblk, t));
+    Context::PkGeom *g = nullptr;
+    AXS_TRY(bin_geometry(c, B, t, &g));
+    AXS_HIP_TRY(hipMemcpyAsync(t.src, srcs, (size_t)n * sizeof(PkSrc), hipMemcpyHostToDevice,
+                               c->stream));
+    switch (N) {
+    case 128: launch_own<128>(c, p, n, B, t, nblk); break;
+    case 256: launch_own<256>(c, p, n, B, t, nblk); break;
+    case 512: launch_own<512>(c, p, n, B, t, nblk); break;
+    case 1024: launch_own<1024>(c, p, n, B, t, nblk); break;
+    case 2048: launch_own<2048>(c, p, n, B, t, nblk); break;
+    default: launch_own<4096>(c, p, n, B, t, nblk); break;
+    }
+    AXS_HIP_TRY(hipGetLastError());
+    return bin_finish(c, n, B, t, nblk, g, hpk, hks, hcnt);
+}
+
 // Raw bin sums (|F|^2, |k| and mode counts per bin, before the
 // normalisation) of n real fields of side N in one batched transform:
 // source q's bins go to hpk / hks / hcnt + q nbins.
@@ -657,6 +1197,7 @@ static int pk_raw(Context *c, const PkSrc *srcs, int n, int N, const BinLayout &
                   std::vector<long long> &hcnt)
 {
     const int nbins = B.nbins;
+    if (own_fft_ok(c, N, B)) return pk_raw_own(c, srcs, n, N, B, hpk, hks, hcnt);
     FftPlan *plan = nullptr;
     AXS_TRY(get_plan(c, N, n, &plan));
     const uint32_t L = (uint32_t)((c->nd == 3) ? (int64_t)N * N * N : (int64_t)N * N);

@@ -386,3 +386,53 @@ def test_powerspec_repeatable(nd, N, binning):
         b = lat.powerspec(binning=binning, nbins=64)
         for x, y in zip(a, b):
             np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("nd,N,prec,binning", [(3, 128, "fp64", "log"), (3, 128, "fp32", "linear"),
+                                               (3, 256, "fp64", "linear"), (2, 128, "fp32", "log"),
+                                               (2, 1024, "fp64", "linear"),
+                                               (2, 4096, "fp32", "log")])
+def test_powerspec_own_transform(nd, N, prec, binning, monkeypatch):
+    """Power-of-two sides from 128 take the transform of our own (passes that
+    read phi from the state and bin in the last pass): against the oracle on
+    the read-back field, and against the rocFFT path on the same state."""
+    lat = _evolved(nd, N, 5, prec)
+    g = lat.get_state()
+    st = lat.get_state(fields=("phi1", "phi2", "phidot1", "phidot2"))
+    if binning == "log":
+        ks_r, pk_r, cnt_r = O.powerspec_log(g["phi2"], nd, N, 0.5, 64)
+    else:
+        ks_r, pk_r, cnt_r = O.powerspec(g["phi2"], nd, N, 0.5)
+    ks, pk, cnt = lat.powerspec(binning=binning, nbins=64)
+    np.testing.assert_array_equal(cnt, cnt_r)
+    sel = cnt > 0
+    # the oracle sums |k| plainly over up to ~2e5 modes per bin (as the
+    # reference does): its own rounding reaches a few 1e-13 here
+    np.testing.assert_allclose(ks[sel], ks_r[sel], rtol=1e-11)
+    np.testing.assert_allclose(pk[sel], pk_r[sel], rtol=1e-9, atol=1e-12 * pk_r[sel].max())
+    monkeypatch.setenv("AXS_PK_ROCFFT", "1")
+    ref = axs.Lattice(nd, N, precision=prec)
+    ref.set_state(st["phi1"], st["phi2"], st["phidot1"], st["phidot2"], tau=lat.tau)
+    ks2, pk2, cnt2 = ref.powerspec(binning=binning, nbins=64)
+    np.testing.assert_array_equal(cnt, cnt2)
+    np.testing.assert_array_equal(ks, ks2)
+    np.testing.assert_allclose(pk[sel], pk2[sel], rtol=1e-11, atol=1e-14 * pk2[sel].max())
+
+
+@pytest.mark.parametrize("field,prec", [("phi", "fp32"), ("phi1", "fp32"), ("phidot", "fp32"),
+                                        ("phidot2", "fp64")])
+def test_powerspec_own_transform_fields(field, prec):
+    """Complex-field spectra (two real transforms in one batch), phi1 and the
+    velocities (the FP32 form's phidot read-back, the FP64 form's stored
+    pairs) through the own transform, against the oracle on the read-back
+    fields."""
+    lat = _evolved(3, 128, 5, prec)
+    g = lat.get_state(fields=("phi1", "phi2", "phidot1", "phidot2"))
+    re, im = {"phi": ("phi1", "phi2"), "phi1": ("phi1", None), "phidot": ("phidot1", "phidot2"),
+              "phidot2": ("phidot2", None)}[field]
+    ks_r, pk_r, cnt_r = O.powerspec_log(g[re], 3, 128, 0.5, 64,
+                                        field_imag=None if im is None else g[im])
+    ks, pk, cnt = lat.powerspec(binning="log", nbins=64, field=field)
+    np.testing.assert_array_equal(cnt, cnt_r)
+    sel = cnt > 0
+    np.testing.assert_allclose(pk[sel], pk_r[sel], rtol=1e-9, atol=1e-12 * pk_r[sel].max())
