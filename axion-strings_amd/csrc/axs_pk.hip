@@ -748,31 +748,42 @@ fft_y_kernel(const double2 *__restrict__ ya, double2 *__restrict__ yb,
 // Last pass with the binning: rows along the last axis (3D: Y'[b][k_z][k_j]
 // [*] along i; 2D: Y[b][k][*] along j), R = kFftPts / N rows per group; each
 // work-group takes a contiguous run of groups (blockIdx.y = lattice); each
-// wave bins rows wave, wave + 4, ... of a group into its accumulators as
+// wave bins rows wave, wave + W, ... of a group into its accumulators as
 // bin_scan_kernel does, and each work-group writes one partial per bin at
-// the end.
+// the end.  Groups of 2048 points (one row where N > 2048): smaller
+// work-groups, more of them per CU, to overlap their loads, transforms and
+// binning.
+template <int N>
+constexpr int fft_bin_pts() { return N > 2048 ? N : 2048; }
+
 template <int ND, int N>
-__global__ void __launch_bounds__(kFftThr)
+__global__ void __launch_bounds__(fft_bin_pts<N>() / 16)
 fft_bin_kernel(const double2 *__restrict__ y, const double2 *__restrict__ tw, int nbins,
                const uint32_t *__restrict__ hi2g, const uint16_t *__restrict__ shellg,
                uint32_t rpl, KahanP *__restrict__ part)
 {
-    constexpr int R = kFftPts / N, LS = fft_ls<N>(), half = N / 2;
-    constexpr int W = kFftThr / 64;
+    constexpr int PB = fft_bin_pts<N>(), THR = PB / 16;
+    constexpr int R = PB / N, LS = fft_ls<N>(), half = N / 2;
+    constexpr int W = THR / 64;
     __shared__ double2 buf[R * LS];
+    // each wave's bin sums are plain doubles (a wave adds at most two runs
+    // per bin and row, over 64 rows at 1024^3: a relative error of at most
+    // ~128 ulp, far below the 1e-9 the spectra are checked to), merged
+    // compensated across waves and work-groups; half the LDS of compensated
+    // pairs, so that 512 bins leave room for three work-groups per CU
     extern __shared__ __align__(16) unsigned char smem[];
-    KahanP *acc = reinterpret_cast<KahanP *>(smem);                    // [wave][bin]
+    double *acc = reinterpret_cast<double *>(smem);                    // [wave][bin]
     uint32_t *hi2 = reinterpret_cast<uint32_t *>(acc + W * nbins);
     uint16_t *shell = reinterpret_cast<uint16_t *>(hi2 + nbins);
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (int i = threadIdx.x; i < nbins; i += kFftThr) {
+    for (int i = threadIdx.x; i < nbins; i += THR) {
         hi2[i] = hi2g[i];
-        for (int w = 0; w < W; ++w) acc[w * nbins + i] = KahanP{0.0, 0.0};
+        for (int w = 0; w < W; ++w) acc[w * nbins + i] = 0.0;
     }
-    for (int i = threadIdx.x; i < half; i += kFftThr) shell[i] = shellg[i];
+    for (int i = threadIdx.x; i < half; i += THR) shell[i] = shellg[i];
     __syncthreads();
-    KahanP *wacc = acc + wave * nbins;
+    double *wacc = acc + wave * nbins;
     const double2 *yl = y + (size_t)blockIdx.y * rpl * N;
     const uint32_t top = (uint32_t)half * (uint32_t)half;
     const uint64_t upto = (2ull << lane) - 1;
@@ -797,13 +808,13 @@ fft_bin_kernel(const double2 *__restrict__ y, const double2 *__restrict__ tw, in
         }
         if (rmin >= top) continue;
 #pragma unroll 4
-        for (int q = 0; q < kFftPts / kFftThr; ++q) {
-            const int e = threadIdx.x + q * kFftThr;
+        for (int q = 0; q < PB / THR; ++q) {
+            const int e = threadIdx.x + q * THR;
             const int r = e / N, x = e - r * N;
             buf[r * LS + fpad(x)] = yl[(size_t)(row0 + r) * N + x];
         }
         __syncthreads();
-        fft_rows<N>(buf, tw, 1);
+        fft_rows<N, PB>(buf, tw, 1);
         for (int r = wave; r < R; r += W) {
             const uint32_t g = row0 + (uint32_t)r;
             uint32_t r2;
@@ -859,18 +870,14 @@ fft_bin_kernel(const double2 *__restrict__ y, const double2 *__restrict__ tw, in
                 u = dpp_take<0x143, 0xc>(v);
                 if (len > (lane & 31)) v += u;
                 const bool last = lane == 63 || (((heads >> lane) >> 1) & 1ull);
-                if (last && b >= 0 && b < nbins) {
-                    KahanP a = wacc[b];
-                    kaddp(a, v);
-                    wacc[b] = a;
-                }
+                if (last && b >= 0 && b < nbins) wacc[b] += v;
             }
         }
         __syncthreads();                             // before the next group's loads
     }
-    for (int i = threadIdx.x; i < nbins; i += kFftThr) {
-        KahanP a = acc[i];
-        for (int w = 1; w < W; ++w) a = kmergep(a, acc[w * nbins + i]);
+    for (int i = threadIdx.x; i < nbins; i += THR) {
+        KahanP a{acc[i], 0.0};
+        for (int w = 1; w < W; ++w) a = kmergep(a, KahanP{acc[w * nbins + i], 0.0});
         part[((size_t)blockIdx.y * nbins + i) * gridDim.x + blockIdx.x] = a;
     }
 }
@@ -1089,8 +1096,9 @@ static bool own_fft_ok(const Context *c, int N, const BinLayout &B)
 {
     if (c->pk_rocfft || (N & (N - 1)) != 0 || N < 128 || N > (c->nd == 3 ? 1024 : 4096))
         return false;
-    const size_t lds = (size_t)(kFftThr / 64) * B.nbins * sizeof(KahanP) +
-                       (size_t)B.nbins * 4 + (size_t)(N / 2) * 2;
+    const int waves = std::max(2048, N) / 16 / 64;            // fft_bin_kernel's
+    const size_t lds = (size_t)waves * B.nbins * sizeof(double) + (size_t)B.nbins * 4 +
+                       (size_t)(N / 2) * 2;
     return B.scan && lds <= kScanLdsMax;
 }
 
@@ -1144,9 +1152,10 @@ static void launch_own(Context *c, const FftPlan *p, int n, const BinLayout &B,
                            c->stream, t.src, p->ya, p->tw);
         hipLaunchKernelGGL((fft_y_kernel<N>), dim3((uint32_t)n * M * (N / RY)),
                            dim3(kFftPtsY / 16), 0, c->stream, p->ya, p->yb, p->tw);
-        const size_t lds = (size_t)(kFftThr / 64) * B.nbins * sizeof(KahanP) +
+        constexpr int TB = fft_bin_pts<N>() / 16;
+        const size_t lds = (size_t)(TB / 64) * B.nbins * sizeof(double) +
                            (size_t)B.nbins * 4 + (size_t)M * 2;
-        hipLaunchKernelGGL((fft_bin_kernel<3, N>), dim3(nblk, (unsigned)n), dim3(kFftThr), lds,
+        hipLaunchKernelGGL((fft_bin_kernel<3, N>), dim3(nblk, (unsigned)n), dim3(TB), lds,
                            c->stream, p->yb, p->tw, B.nbins, t.hi, t.shell, (uint32_t)(M * N),
                            t.part);
     } else {
@@ -1154,9 +1163,10 @@ static void launch_own(Context *c, const FftPlan *p, int n, const BinLayout &B,
         constexpr int RZ = kFftPts / M;
         hipLaunchKernelGGL((fft_z_kernel<2, N>), dim3(rpl / RZ, (unsigned)n), dim3(kFftThr), 0,
                            c->stream, t.src, p->ya, p->tw);
-        const size_t lds = (size_t)(kFftThr / 64) * B.nbins * sizeof(KahanP) +
+        constexpr int TB = fft_bin_pts<N>() / 16;
+        const size_t lds = (size_t)(TB / 64) * B.nbins * sizeof(double) +
                            (size_t)B.nbins * 4 + (size_t)M * 2;
-        hipLaunchKernelGGL((fft_bin_kernel<2, N>), dim3(nblk, (unsigned)n), dim3(kFftThr), lds,
+        hipLaunchKernelGGL((fft_bin_kernel<2, N>), dim3(nblk, (unsigned)n), dim3(TB), lds,
                            c->stream, p->ya, p->tw, B.nbins, t.hi, t.shell, (uint32_t)M, t.part);
     }
 }
@@ -1169,8 +1179,8 @@ static int pk_raw_own(Context *c, const PkSrc *srcs, int n, int N, const BinLayo
     FftPlan *p = nullptr;
     AXS_TRY(get_own(c, N, n, &p));
     const uint32_t rows = (c->nd == 3) ? (uint32_t)(N / 2) * (uint32_t)N : (uint32_t)(N / 2);
-    const uint32_t groups = rows / (uint32_t)(kFftPts / N);
-    const uint32_t nblk = std::max<uint32_t>(1, std::min<uint32_t>(groups, 2048));
+    const uint32_t groups = rows / (uint32_t)(std::max(2048, N) / N);   // fft_bin_kernel's
+    const uint32_t nblk = std::max<uint32_t>(1, std::min<uint32_t>(groups, 4096));
     ScanTabs t;
     AXS_TRY(scan_tabs(c, B, n, nblk, t));
     Context::PkGeom *g = nullptr;
