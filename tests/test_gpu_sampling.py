@@ -391,7 +391,7 @@ def test_powerspec_repeatable(nd, N, binning):
 @pytest.mark.parametrize("nd,N,prec,binning", [(3, 128, "fp64", "log"), (3, 128, "fp32", "linear"),
                                                (3, 256, "fp64", "linear"), (2, 128, "fp32", "log"),
                                                (2, 1024, "fp64", "linear"),
-                                               (2, 4096, "fp32", "log")])
+                                               (2, 2048, "fp64", "log"), (2, 4096, "fp32", "log")])
 def test_powerspec_own_transform(nd, N, prec, binning, monkeypatch):
     """Power-of-two sides from 128 take the transform of our own (passes that
     read phi from the state and bin in the last pass): against the oracle on
@@ -436,3 +436,24 @@ def test_powerspec_own_transform_fields(field, prec):
     np.testing.assert_array_equal(cnt, cnt_r)
     sel = cnt > 0
     np.testing.assert_allclose(pk[sel], pk_r[sel], rtol=1e-9, atol=1e-12 * pk_r[sel].max())
+
+
+@pytest.mark.parametrize("N,prec", [(1024, "fp32"), (512, "fp64")])
+def test_powerspec_own_transform_large(N, prec, monkeypatch):
+    """The own transform at the benchmark's 1024^3 (and 512^3), where no
+    oracle transform fits the test's time: against rocFFT on the same device
+    field, log and linear bins."""
+    lat = axs.Lattice(3, N, precision=prec)
+    lat.init("kinv", seed=7)
+    own = [lat.powerspec(binning="log", nbins=64), lat.powerspec(binning="linear")]
+    lat.close()
+    monkeypatch.setenv("AXS_PK_ROCFFT", "1")
+    ref = axs.Lattice(3, N, precision=prec)
+    ref.init("kinv", seed=7)
+    rf = [ref.powerspec(binning="log", nbins=64), ref.powerspec(binning="linear")]
+    ref.close()
+    for (ks, pk, cnt), (ks2, pk2, cnt2) in zip(own, rf):
+        np.testing.assert_array_equal(cnt, cnt2)
+        np.testing.assert_array_equal(ks, ks2)
+        sel = cnt > 0
+        np.testing.assert_allclose(pk[sel], pk2[sel], rtol=1e-11, atol=1e-14 * pk2[sel].max())
