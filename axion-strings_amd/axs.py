@@ -61,7 +61,7 @@ EXPORTS = (
     "axs_get_tau",
     "axs_set_state", "axs_get_state", "axs_step", "axs_member_axion", "axs_member_strings",
     "axs_strings",
-    "axs_string_counts", "axs_member_background", "axs_background",
+    "axs_string_counts", "axs_member_background", "axs_background", "axs_member_nan_check",
     "axs_member_energy", "axs_energy", "axs_member_energy_noalloc",
     "axs_member_powerspec", "axs_member_powerspec_field", "axs_powerspec",
     "axs_powerspec_all",
@@ -150,6 +150,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "axs_string_counts": ([c_void_p, POINTER(c_int64), POINTER(c_int64)], c_int),
         "axs_member_background": ([c_void_p, c_int, _DP], c_int),
         "axs_background": ([c_void_p, _DP], c_int),
+        "axs_member_nan_check": ([c_void_p, c_int, POINTER(c_int64)], c_int),
         "axs_member_energy": ([c_void_p, c_int, _DP], c_int),
         "axs_energy": ([c_void_p, _DP], c_int),
         "axs_member_energy_noalloc": ([c_void_p, c_int, _DP], c_int),
@@ -367,6 +368,14 @@ class Lattice:
         _check(self.lib.axs_member_background(self._ctx, member, _dp(out)),
                "axs_member_background")
         return out
+
+    def nan_check(self, member: int = 0) -> int:
+        """index of the first site whose phi1 or phi2 is NaN, -1 if none
+        (the reference's debug scan, s_run.cpp:4-12)"""
+        first = c_int64()
+        _check(self.lib.axs_member_nan_check(self._ctx, member, ctypes.byref(first)),
+               "axs_member_nan_check")
+        return first.value
 
     def snapshot_begin(self, member: int = 0, fields=("phi1", "phi2", "phidot1", "phidot2",
                                                       "ker1", "ker2"), count: int = 0):

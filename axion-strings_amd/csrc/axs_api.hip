@@ -900,6 +900,16 @@ int axs_member_background(axs_ctx *ctx, int member, double sums[6])
 
 int axs_background(axs_ctx *ctx, double sums[6]) { return axs_member_background(ctx, 0, sums); }
 
+int axs_member_nan_check(axs_ctx *ctx, int member, int64_t *first_nan)
+{
+    AXS_TRY(check_member(ctx, member));
+    if (!first_nan) {
+        set_error("null output");
+        return AXS_ERR_INVALID_ARG;
+    }
+    return run_nan_check(&ctx->impl, ctx->impl.members[member], first_nan);
+}
+
 int axs_member_energy(axs_ctx *ctx, int member, double out[3])
 {
     AXS_TRY(check_member(ctx, member));

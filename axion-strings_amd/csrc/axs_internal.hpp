@@ -334,6 +334,7 @@ int noalloc_setup(int device);
 int noalloc_workspace(int device, Workspace *ws);
 
 int run_background(Context *c, Member &m, double out[6]);
+int run_nan_check(Context *c, Member &m, int64_t *first);
 int run_init(Context *c, Member &m, int mode, uint64_t seed);
 int run_autotune(Context *c, int trials, double *best_ms);
 // ws: a caller-given workspace used instead of the context's scratch

@@ -468,6 +468,31 @@ fold_kernel(const Kahan *__restrict__ part, int nblocks, int Q, double *__restri
     if (threadIdx.x == 0) out[q] = sa[0].s + sa[0].c;
 }
 
+// The index of the first site whose phi1 or phi2 is NaN (the reference's
+// debug scan, s_run.cpp:4-12): each work-item's first hit in its strided
+// (ascending) set of sites, the wave's minimum, and one atomic minimum per
+// wave into *first -- the minimum of a set does not depend on the order, so
+// the result is the lowest such index whatever the schedule.
+template <class V>
+__global__ void __launch_bounds__(kRedBlock)
+nan_first_kernel(uint32_t L, const V *__restrict__ phi, unsigned long long *__restrict__ first)
+{
+    unsigned long long best = ~0ull;
+    const uint32_t stride = gridDim.x * kRedBlock;
+    for (uint32_t m = blockIdx.x * kRedBlock + threadIdx.x; m < L; m += stride) {
+        const V w = phi[m];
+        if (isnan(w.x) || isnan(w.y)) {
+            best = m;
+            break;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long t = __shfl_down(best, o, 64);
+        best = t < best ? t : best;
+    }
+    if ((threadIdx.x & 63) == 0 && best != ~0ull) atomicMin(first, best);
+}
+
 static int red_blocks(uint32_t L)
 {
     uint32_t need = (L + kRedBlock - 1) / kRedBlock;
@@ -556,6 +581,31 @@ static void energy_scale(const Context *c, const double raw[3], double out[3])
     out[0] = raw[0] * vol;
     out[1] = raw[1] / (dx * dx) * vol;
     out[2] = raw[2] * vol;
+}
+
+int run_nan_check(Context *c, Member &m, int64_t *first)
+{
+    if (!m.has_state) {
+        set_error("member has no state: call axs_set_member_state first");
+        return AXS_ERR_STATE;
+    }
+    AXS_TRY(ensure_scratch(c, 256));
+    unsigned long long *d = (unsigned long long *)c->scratch;
+    AXS_HIP_TRY(hipMemsetAsync(d, 0xff, sizeof *d, c->stream));
+    const uint32_t L = (uint32_t)m.L;
+    const int nb = red_blocks(L);
+    if (c->prec == AXS_FP64)
+        hipLaunchKernelGGL(nan_first_kernel<dvec2>, dim3(nb), dim3(kRedBlock), 0, c->stream, L,
+                           (const dvec2 *)m.state[c->cur][PHI], d);
+    else
+        hipLaunchKernelGGL(nan_first_kernel<fvec4>, dim3(nb), dim3(kRedBlock), 0, c->stream, L,
+                           (const fvec4 *)m.state[c->cur][PHI], d);
+    AXS_HIP_TRY(hipGetLastError());
+    unsigned long long h = 0;
+    AXS_HIP_TRY(hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    AXS_HIP_TRY(hipStreamSynchronize(c->stream));
+    *first = (h == ~0ull) ? -1 : (int64_t)h;
+    return AXS_OK;
 }
 
 int run_background(Context *c, Member &m, double out[6])

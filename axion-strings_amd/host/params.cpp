@@ -67,6 +67,7 @@ std::vector<Tag> tag_table(parameters *p)
         {"restart_path", Type::STRING, p->restart_path, AXS_MAX_LEN, false, false},
         {"eom", Type::STRING, p->eom, sizeof p->eom, false, false},
         {"laplacian", Type::STRING, p->laplacian, sizeof p->laplacian, false, false},
+        {"debug_nan", Type::INT, &p->debug_nan, 0, false, false},
     };
 }
 
@@ -87,6 +88,7 @@ void parameters_set_defaults(parameters *p)
     p->restart_path[0] = 0;
     snprintf(p->eom, sizeof p->eom, "field_rescaled");
     snprintf(p->laplacian, sizeof p->laplacian, "stencil");
+    p->debug_nan = 0;
 }
 
 int read_parameter_file(const char *fname, parameters *p)

@@ -61,6 +61,7 @@ struct parameters {
     char restart_path[AXS_MAX_LEN];     // "" = none; start from this checkpoint
     char eom[24];           // "field_rescaled" (s_run.cpp:215) or "hamiltonian" (:216)
     char laplacian[16];     // "stencil" (s_integrate.cpp:322) or "spectral" (:326-328)
+    int debug_nan;          // 1: the NaN scan before every step (s_run.cpp:4-12, 213)
 };
 
 // Gadget2-style reader with the reference's rules (read_parameters.cpp:

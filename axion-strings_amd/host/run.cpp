@@ -358,6 +358,22 @@ int run_standard(const parameters &p, Out &o)
     double step_ms_total = 0.0, sample_s = 0.0;
     int64_t steps_done = 0;
 
+    // the reference's debug scan before a step (s_run.cpp:4-12, at :213):
+    // the message, then the run stops (the reference asserts)
+    auto nan_scan = [&](int step) {
+        if (!p.debug_nan) return true;
+        int64_t first = -1;
+        if (axs_member_nan_check(ctx, 0, &first) != AXS_OK) {
+            fprintf(stderr, "axs: axs_member_nan_check failed: %s\n", axs_last_error());
+            return false;
+        }
+        if (first < 0) return true;
+        fprintf(o.main, "Error: NaN encountered in solution vector.\n");
+        fprintf(o.main, " tstep = %d\n", step);
+        fflush(o.main);
+        return false;
+    };
+
     int tstep = first_step;
     while (tstep < final_step) {
         // next step at which something is sampled (sampling happens before
@@ -366,7 +382,17 @@ int run_standard(const parameters &p, Out &o)
         while (next < final_step && !on_cadence(p.sample_time_series, next, p.n_samples, final_step) &&
                !on_cadence(p.save_snapshots, next, p.n_snapshots, final_step))
             ++next;
-        if (next > tstep) {
+        if (next > tstep && p.debug_nan) {
+            for (; tstep < next; ++tstep) {          // one step at a time, scanned before each
+                if (!nan_scan(tstep)) return 1;
+                CHECK(axs_step(ctx, 1));
+                double ms = 0.0;
+                axs_last_step_ms(ctx, &ms);
+                step_ms_total += ms;
+                steps_done += 1;
+            }
+            CHECK(axs_get_tau(ctx, &ph.tau));
+        } else if (next > tstep) {
             CHECK(axs_step(ctx, next - tstep));
             double ms = 0.0;
             axs_last_step_ms(ctx, &ms);
@@ -447,6 +473,7 @@ int run_standard(const parameters &p, Out &o)
         }
         sample_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 
+        if (!nan_scan(tstep)) return 1;                                   // s_run.cpp:213
         CHECK(axs_step(ctx, 1));                                          // s_run.cpp:215
         double ms = 0.0;
         axs_last_step_ms(ctx, &ms);
@@ -550,6 +577,7 @@ extern "C" int axs_host_params_json(const char *path, char *buf, int cap)
     json_str(s, "restart_path", p.restart_path);
     json_str(s, "eom", p.eom);
     json_str(s, "laplacian", p.laplacian);
+    json_num(s, "debug_nan", p.debug_nan);
     s.resize(s.size() - 2);
     s += "}";
     if ((int)s.size() + 1 > cap) return 2;

@@ -176,6 +176,14 @@ int axs_string_counts(axs_ctx *ctx, int64_t *n_plaquettes, int64_t *num_cores);
 int axs_member_background(axs_ctx *ctx, int member, double sums[6]);
 int axs_background(axs_ctx *ctx, double sums[6]);
 
+/* The reference's debug scan of the field (s_run.cpp:4-12; its call at
+ * s_run.cpp:213, before each step, is commented out there): *first_nan =
+ * the index (reference layout) of the first site whose phi1 or phi2 is NaN,
+ * -1 if there is none.  The reference then prints "Error: NaN encountered
+ * in solution vector." with the step and asserts; axs_main does the same
+ * with the optional tag debug_nan 1.                                      */
+int axs_member_nan_check(axs_ctx *ctx, int member, int64_t *first_nan);
+
 /* Energy components (kinetic, gradient, potential) times the cell volume;
  * no reference counterpart.  Compensated FP64.
  * FP32 form: one pass computes the background sums and the energy of a
