@@ -94,6 +94,9 @@ StepScalarsF to_float(const StepScalars &s)
     f.T03 = (float)s.T03;
     f.pot0 = (float)(s.T03 - s.tau2);
     f.ac0 = (float)(s.alpha * s.c[0]);
+    f.dta = (float)(s.dt * s.alpha);
+    f.dtcl = (float)(s.dt * s.cl);
+    f.dtac0 = (float)(s.dt * s.alpha * s.c[0]);
     for (int i = 0; i < 5; ++i) f.c[i] = (float)s.c[i];
     f.ncoef = s.ncoef;
     return f;

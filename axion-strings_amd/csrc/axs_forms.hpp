@@ -46,6 +46,20 @@ struct F32Form {
     {
         return Own{__builtin_nondeterministic_value(0.0f), __builtin_nondeterministic_value(0.0f)};
     }
+    // values a level reads from vector registers, set once per work-group:
+    // a multiply-add reads one scalar register at most, and dtac0 is the
+    // same at every level of a launch (dt, alpha and c0 do not change with
+    // tau); the empty asm keeps it in a vector register instead of a copy
+    // from the scalar one at every use
+    struct Vc {
+        float dtac0;
+    };
+    static __device__ __forceinline__ Vc vconst(const Sc &s)
+    {
+        Vc v{s.dtac0};
+        __asm__ volatile("" : "+v"(v.dtac0));
+        return v;
+    }
     static __device__ __forceinline__ void fetch(const In &a, uint32_t q, Raw &r) { r = a.st[q]; }
     // Laplacian accumulation: start (the centre joins the potential), a + c v
     static __device__ __forceinline__ T centre(const Sc &, T) { return -0.0f; }
@@ -59,13 +73,12 @@ struct F32Form {
     }
     // one step at a site; nx, ny: the next step's drift (sn: its scalars)
     static __device__ __forceinline__ void level(T x, T y, T a, T c, const Own &o, const Sc &s,
-                                                 const Sc &sn, Own &on, T &nx, T &ny)
+                                                 const Sc &sn, const Vc &vc, Own &on, T &nx,
+                                                 T &ny)
     {
-        const T t = F32Math::own(s.cl, F32Math::pot(x, y, s.pot0), s.ac0);
-        const T k1 = F32Math::force(s.alpha, a, x, t);
-        const T k2 = F32Math::force(s.alpha, c, y, t);
-        on.u1 = F32Math::kick(o.u1, s.dt, k1);
-        on.u2 = F32Math::kick(o.u2, s.dt, k2);
+        const T tdt = F32Math::own(s.dtcl, F32Math::pot(x, y, s.pot0), vc.dtac0);
+        on.u1 = F32Math::kick_dt(o.u1, s.dta, a, x, tdt);
+        on.u2 = F32Math::kick_dt(o.u2, s.dta, c, y, tdt);
         nx = F32Math::drift(x, on.u1, sn.dt);
         ny = F32Math::drift(y, on.u2, sn.dt);
     }
@@ -96,6 +109,9 @@ struct F64FormT {
     struct Own {
         dvec2 v, k;
     };
+    struct Vc {
+    };
+    static __device__ __forceinline__ Vc vconst(const Sc &) { return Vc{}; }
     // own data of a site whose value is never used: any bits (no moves)
     static __device__ __forceinline__ Own unset()
     {
@@ -131,7 +147,8 @@ struct F64FormT {
         o.k = r.k;
     }
     static __device__ __forceinline__ void level(T x, T y, T a, T c, const Own &o, const Sc &s,
-                                                 const Sc &sn, Own &on, T &nx, T &ny)
+                                                 const Sc &sn, const Vc &, Own &on, T &nx,
+                                                 T &ny)
     {
         const T lap1 = s.alpha * a, lap2 = s.alpha * c;
         const T S = x * x + y * y - s.tau2 + s.T03;

@@ -59,6 +59,9 @@ struct StepScalarsF {
     float dt, alpha, cl, tau2, T03, half_dt;
     float pot0;     // T03 - tau2, rounded once from the double difference
     float ac0;      // alpha * c[0]
+    // the step's kick scaled by dt (rounded once from the double products):
+    // dta = dt alpha, dtcl = dt cl, dtac0 = dt alpha c[0]
+    float dta, dtcl, dtac0;
     float c[5];
     int ncoef;
 };

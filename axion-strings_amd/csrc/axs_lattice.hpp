@@ -105,9 +105,13 @@ __device__ __forceinline__ double lane_prev(double v)
 //         = alpha * a + x * (alpha c0 + cl S),    a = sum_m c_m n_m
 // (the neighbour sum a starts from -0.0, the additive identity), and every
 // product-sum is a fused multiply-add: one rounding each and a third fewer
-// instructions per site than the reference's expression tree.  The FP64
-// form keeps the reference's tree and separate roundings, which its
-// bit-for-bit parity needs.
+// instructions per site than the reference's expression tree.  The step
+// folds dt into the scalars, u' = u + dt ker = (u + x tdt) + dta a with
+// tdt = dtcl S + dtac0 (dta = dt alpha, dtcl = dt cl, dtac0 = dt alpha c0),
+// two multiply-adds per component where force then u + dt ker take three; the
+// read-back of ker keeps the unscaled force.  The FP64 form keeps the
+// reference's tree and separate roundings, which its bit-for-bit parity
+// needs.
 struct F32Math {
     // drifted field x = phi + dt u
     static __device__ __forceinline__ float drift(float phi, float u, float dt)
@@ -135,12 +139,25 @@ struct F32Math {
     {
         return __builtin_fmaf(alpha, a, x * t);
     }
-    // u' = u + dt ker
-    static __device__ __forceinline__ float kick(float u, float dt, float k)
+    // the step's kick with dt folded in: u' = (u + x tdt) + dta a, where
+    // tdt = own(dtcl, S, dtac0) is dt t for both components
+    static __device__ __forceinline__ float kick_dt(float u, float dta, float a, float x,
+                                                    float tdt)
     {
-        return __builtin_fmaf(dt, k, u);
+        return __builtin_fmaf(dta, a, __builtin_fmaf(x, tdt, u));
     }
 };
+
+// The dispatcher deals the work-groups of a launch to the eight XCDs in
+// turn (work-group b runs on XCD b % 8).  xcd_order(b, nb) gives XCD x the
+// contiguous logical range [x q + min(x, r), ...) of q = nb / 8 indices, one
+// more for the first r = nb % 8 XCDs: a bijection of [0, nb) for any nb, so
+// tiles that are neighbours in the logical order share one XCD's L2.
+__device__ __forceinline__ uint32_t xcd_order(uint32_t b, uint32_t nb)
+{
+    const uint32_t x = b & 7u, q = nb >> 3, r = nb & 7u;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
 
 __device__ __forceinline__ int wrap_coord(int x, int N)
 {

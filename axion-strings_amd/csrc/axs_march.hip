@@ -135,14 +135,12 @@ struct HalfF32 {
     static __device__ __forceinline__ void update(const Out &o, uint32_t q, const S &s, float x1,
                                                   float x2, float sum1, float sum2, const Own &w)
     {
-        float t = F32Math::own(s.cl, F32Math::pot(x1, x2, s.pot0), s.ac0);
-        float k1 = F32Math::force(s.alpha, sum1, x1, t);
-        float k2 = F32Math::force(s.alpha, sum2, x2, t);
+        float tdt = F32Math::own(s.dtcl, F32Math::pot(x1, x2, s.pot0), s.dtac0);
         fvec4 r;
         r.x = x1;
         r.y = x2;
-        r.z = F32Math::kick(w.u1, s.dt, k1);
-        r.w = F32Math::kick(w.u2, s.dt, k2);
+        r.z = F32Math::kick_dt(w.u1, s.dta, sum1, x1, tdt);
+        r.w = F32Math::kick_dt(w.u2, s.dta, sum2, x2, tdt);
         if (NT)
             store_nt(o.st + q, r);
         else
@@ -154,7 +152,7 @@ struct MarchGeom {
     int N;        // lattice side
     int CI;       // planes per chunk
     int nTj, nTk; // tiles along j and k
-    int remap;    // XCD-aware work-group order
+    int remap;    // XCD-aware work-group order (xcd_order)
     int kfast;    // 1: tiles ordered k fastest (k-neighbours co-resident)
 };
 
@@ -323,7 +321,7 @@ march3d_kernel(MarchGeom g, typename Tr::In in, typename Tr::Out out, typename T
 {
     __shared__ MarchLds<Tr, RJ> sh;
     uint32_t b = blockIdx.x;
-    if (g.remap) b = (b & 7u) * (gridDim.x >> 3) + (b >> 3);
+    if (g.remap) b = xcd_order(b, gridDim.x);
     int tj, tk;
     if (g.kfast) {
         tk = (int)(b % (uint32_t)g.nTk);

@@ -25,12 +25,13 @@
 // so a pass is bit-identical to D one-step launches.  Tiles at the lattice
 // edge overhang it: loads wrap periodically and stores are masked.
 //
-// Issue cost (the kernel is bound by vector instructions, not HBM): a row
-// outside a level's region is skipped with a scalar branch (the wave index
-// is uniform), every lane of the other rows computes (lanes outside the
-// region cost no extra issue cycles and their values never reach a stored
-// site), the LDS planes carry RAD padding rows so that those lanes' reads
-// stay inside them, and the march loop runs in unrolled blocks of planes.
+// Issue cost: a row outside a level's region is skipped with a scalar
+// branch (the wave index is uniform) except in the FP32 7-point form, where
+// every row computes (kSkipRows); every lane of a computed row computes
+// (lanes outside the region cost no extra issue cycles and their values
+// never reach a stored site), the LDS planes carry RAD padding rows so that
+// those lanes' reads stay inside them, and the march loop runs in unrolled
+// blocks of planes.
 #pragma once
 
 #include <type_traits>
@@ -42,6 +43,21 @@
 namespace axs {
 
 constexpr int kTBW = 64;   // columns loaded per row (one wave)
+
+// FP32 7-point stencil: the k neighbours of a site are x of the lanes next
+// to it, which a DPP whole-wave shift hands to the add as its operand
+// (folded into the v_add: no LDS read, no extra instruction).  The end
+// lanes read zero, and lie outside every level's region.
+template <class T, int RAD>
+constexpr bool kLaneK = std::is_same<T, float>::value && RAD == 1;
+// Rows outside a level's region: skipped with a scalar branch, except in
+// that form, where every row computes -- the branch saves a sixth of the
+// arithmetic at depth 4, but merging the values of its two paths costs more
+// register moves than that, and the stores after all rows (below) keep the
+// DPP operands folded.  The wider stencils keep the branch: computing every
+// row there needs more registers than a 16-wave work-group has.
+template <class T, int RAD>
+constexpr bool kSkipRows = !kLaneK<T, RAD>;
 // planes per unrolled block of the march loop: the register rings then
 // rotate partly by renaming (measured: three planes 8% faster than one at
 // 1024^3, FP32 depth 4; two and six no better).  Shapes whose rings and
@@ -56,8 +72,38 @@ constexpr int plane_unroll()
 }
 
 struct TBGeom {
-    int N, CI, nTj, nTk, remap;
+    int N, CI, nTj, nTk;
+    int band;     // tile order: bands of `band` tile rows, j fastest in a band
+    int xcd;      // 1: each XCD takes a contiguous range of the order (xcd_order)
 };
+
+// The tile order of a launch, chosen by timing at 1024^3 (whose launches
+// hold 13 work-groups per CU in the FP32 form, 116 in the FP64 form): FP32
+// walks whole columns of tiles, j fastest, in dispatch order (work-groups
+// dealt to the XCDs in turn); FP64 gives each XCD a contiguous range
+// (xcd_order) of bands of 8 tile rows, j fastest in a band.  Timings of one
+// order differ by up to 10% between processes (the state's placement in
+// memory), so the choice rests on comparisons within one process.
+template <class Tr>
+inline void tile_order(TBGeom &g)
+{
+    const bool f32 = std::is_same<typename Tr::T, float>::value;
+    g.band = f32 ? g.nTj : 8;
+    g.xcd = f32 ? 0 : 1;
+}
+
+// tile (tj, tk) of logical index t of a layer of nTj x nTk tiles
+__device__ __forceinline__ void tile_of(uint32_t t, const TBGeom &g, int &tj, int &tk)
+{
+    const uint32_t nTk = (uint32_t)g.nTk, nTj = (uint32_t)g.nTj;
+    // band of `band` tile rows (the last one shorter), walked j fastest:
+    // consecutive work-groups form a band x (count / band) block of tiles
+    const uint32_t bw = (uint32_t)g.band * nTk;
+    const uint32_t j0 = (t / bw) * (uint32_t)g.band, loc = t % bw;
+    const uint32_t h = (nTj - j0 < (uint32_t)g.band) ? nTj - j0 : (uint32_t)g.band;
+    tj = (int)(j0 + loc % h);
+    tk = (int)(loc / h);
+}
 
 // x_l of the plane level l works on: the ER loaded rows plus RAD padding
 // rows above and below, so that the neighbour reads of the rim rows and
@@ -136,6 +182,7 @@ __device__ __forceinline__ void marchd_tile(const int N, const int CI, const int
             for (int p = 0; p < RAD; ++p) fifo[l][p][r] = Own{};
         }
     Raw raw[R], pf[R];
+    const typename Tr::Vc vc = Tr::vconst(sc.s[0]);
 
     if (PF) load(i0 - H, pf);
     auto plane_step = [&](const int it) __attribute__((always_inline)) {
@@ -167,11 +214,13 @@ __device__ __forceinline__ void marchd_tile(const int N, const int CI, const int
                 const uint32_t po = plane(i + (D - l) * RAD);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    // rows outside the region: any value (never used)
-                    nx[r][0] = __builtin_nondeterministic_value((T)0);
-                    nx[r][1] = __builtin_nondeterministic_value((T)0);
-                    no[r] = Tr::unset();
-                    if (!row_in(erow[r], l)) continue;
+                    if constexpr (kSkipRows<T, RAD>) {
+                        // rows outside the region: any value (never used)
+                        nx[r][0] = __builtin_nondeterministic_value((T)0);
+                        nx[r][1] = __builtin_nondeterministic_value((T)0);
+                        no[r] = Tr::unset();
+                        if (!row_in(erow[r], l)) continue;
+                    }
                     const int o = (erow[r] + RAD) * kTBW + lane;
                     const T x = ring[l - 1][RAD][r][0], y = ring[l - 1][RAD][r][1];
                     T a = Tr::centre(s, x), c = Tr::centre(s, y);
@@ -194,22 +243,47 @@ __device__ __forceinline__ void marchd_tile(const int N, const int CI, const int
                         c = add(c, L2[o + m * kTBW]);                       // j+m
                         a = add(a, L1[o - m * kTBW]);
                         c = add(c, L2[o - m * kTBW]);                       // j-m
-                        a = add(a, L1[o + m]);
-                        c = add(c, L2[o + m]);                              // k+m
-                        a = add(a, L1[o - m]);
-                        c = add(c, L2[o - m]);                              // k-m
+                        if constexpr (kLaneK<T, RAD>) {
+                            // k+-1 are the neighbour lanes' x: DPP operands
+                            a = add(a, lane_next(x));
+                            c = add(c, lane_next(y));                       // k+1
+                            a = add(a, lane_prev(x));
+                            c = add(c, lane_prev(y));                       // k-1
+                        } else {
+                            a = add(a, L1[o + m]);
+                            c = add(c, L2[o + m]);                          // k+m
+                            a = add(a, L1[o - m]);
+                            c = add(c, L2[o - m]);                          // k-m
+                        }
                     }
                     Own on;
                     T ex, ey;
-                    Tr::level(x, y, a, c, fifo[l - 1][0][r], s, sn, on, ex, ey);
-                    if (l == D) {
+                    Tr::level(x, y, a, c, fifo[l - 1][0][r], s, sn, vc, on, ex, ey);
+                    if (l == D && !kLaneK<T, RAD>) {
                         if (it >= 0 && rowout[r] && colout)
                             Tr::store(out, po + rowoff[r], x, y, on);
-                    } else {
-                        nx[r][0] = ex;
-                        nx[r][1] = ey;
-                        no[r] = on;
+                        continue;
                     }
+                    // kLaneK: level D's stored site too, stored after every
+                    // row's arithmetic, so that the stores' exec-masked
+                    // branches split no row's instructions
+                    nx[r][0] = (l == D) ? x : ex;
+                    nx[r][1] = (l == D) ? y : ey;
+                    no[r] = on;
+                }
+                if (l == D && kLaneK<T, RAD>) {
+                    if constexpr (kLaneK<T, RAD>) {
+                        // the stored values are computed here, for every
+                        // lane: sunk into the stores' masked branches, the
+                        // adds would lose their DPP operands to separate moves
+#pragma unroll
+                        for (int r = 0; r < R; ++r)
+                            __asm__ volatile("" : "+v"(no[r].u1), "+v"(no[r].u2));
+                    }
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        if (it >= 0 && rowout[r] && colout)
+                            Tr::store(out, po + rowoff[r], nx[r][0], nx[r][1], no[r]);
                 }
             }
             // shift level l by one plane: the value handed up enters at the top
@@ -265,12 +339,11 @@ marchd_kernel(TBGeom g, typename Tr::In in, typename Tr::Out out,
               LevelScalars<typename Tr::Sc, D> sc)
 {
     __shared__ TBLds<typename Tr::T, D, W, R, RAD> X;
-    uint32_t b = blockIdx.x;
-    if (g.remap) b = (b & 7u) * (gridDim.x >> 3) + (b >> 3);   // XCD-aware order
-    const int tk = (int)(b % (uint32_t)g.nTk);
-    b /= (uint32_t)g.nTk;
-    const int tj = (int)(b % (uint32_t)g.nTj);
-    const int ch = (int)(b / (uint32_t)g.nTj);
+    const uint32_t b = g.xcd ? xcd_order(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t layer = (uint32_t)g.nTj * (uint32_t)g.nTk;
+    int tj, tk;
+    tile_of(b % layer, g, tj, tk);
+    const int ch = (int)(b / layer);
     marchd_tile<Tr, D, W, R, RAD, PF>(g.N, g.CI, tj, tk, ch, in, out, sc, X);
 }
 
@@ -360,7 +433,7 @@ struct SingleLaunch {
         g.nTk = (N + TK - 1) / TK;
         g.CI = multi_chunk(c, N, TJ, TK);
         const uint32_t nb = (uint32_t)g.nTj * (uint32_t)g.nTk * (uint32_t)(N / g.CI);
-        g.remap = (nb % 8 == 0) ? 1 : 0;
+        tile_order<Tr>(g);
         if (pf)
             hipLaunchKernelGGL((marchd_kernel<Tr, D, W, R, RAD, true>), dim3(nb), dim3(64 * W), 0,
                                c->stream, g, in, out, sc);

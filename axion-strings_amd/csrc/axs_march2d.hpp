@@ -85,6 +85,7 @@ __device__ __forceinline__ void march2d_strip(const int N, const int CJ, const i
         for (int p = 0; p < RAD; ++p) fifo[l][p] = Own{};
     }
     Raw raw, pf;
+    const typename Tr::Vc vc = Tr::vconst(sc.s[0]);
     if (PF) Tr::fetch(in, row(j0 - H), pf);
 
     auto row_step = [&](const int it) __attribute__((always_inline)) {
@@ -144,7 +145,7 @@ __device__ __forceinline__ void march2d_strip(const int N, const int CJ, const i
                     }
                     Own on;
                     T ex, ey;
-                    Tr::level(x, y, a, c, fifo[l - 1][0], s, sn, on, ex, ey);
+                    Tr::level(x, y, a, c, fifo[l - 1][0], s, sn, vc, on, ex, ey);
                     if (l == D) {
                         if (it >= 0 && colout) Tr::store(out, row(j), x, y, on);
                     } else {

@@ -124,14 +124,12 @@ __device__ __forceinline__ void site_fp32(int N, uint32_t m, const fvec4 *__rest
             s2 = F32Math::acc(s2, cl, F32Math::drift(q.y, q.w, s.dt));
         }
     }
-    float t = F32Math::own(s.cl, F32Math::pot(x1, x2, s.pot0), s.ac0);
-    float k1 = F32Math::force(s.alpha, s1, x1, t);
-    float k2 = F32Math::force(s.alpha, s2, x2, t);
+    float tdt = F32Math::own(s.dtcl, F32Math::pot(x1, x2, s.pot0), s.dtac0);
     fvec4 o;
     o.x = x1;
     o.y = x2;
-    o.z = F32Math::kick(c.z, s.dt, k1);
-    o.w = F32Math::kick(c.w, s.dt, k2);
+    o.z = F32Math::kick_dt(c.z, s.dta, s1, x1, tdt);
+    o.w = F32Math::kick_dt(c.w, s.dta, s2, x2, tdt);
     out[m] = o;
 }
 
