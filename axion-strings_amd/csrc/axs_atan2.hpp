@@ -237,6 +237,19 @@ AXS_HD double abs_atan2_core(double num, double den, bool sw, bool nx, const dou
     return (bh + ((sw != nx) ? -at : at)) + bl;
 }
 
+// |atan2(y, x)| where x or y is NaN, den = max(|x|, |y|) is infinite or
+// zero (num = min(|x|, |y|)): the C99 Annex F values
+AXS_HD double abs_atan2_special(double y, double x, double num, double den, bool sw, bool nx)
+{
+    using namespace ddm;
+    if (isnan(x) || isnan(y)) return x + y;
+    if (isinf(den)) {
+        if (isinf(num)) return nx ? 0x1.2d97c7f3321d2p+1 : 0x1.921fb54442d18p-1;   // 3pi/4, pi/4
+        return sw ? kPio2.hi : (nx ? kPi.hi : 0.0);
+    }
+    return nx ? kPi.hi : 0.0;     // both zero
+}
+
 // |atan2(y, x)| (see abs_atan2_core).  Zeros, infinities and NaN give the
 // C99 Annex F values; finite magnitudes outside [2^-1000, 2^1000] are
 // scaled by a common power of two first (the angle does not change).
@@ -247,17 +260,32 @@ AXS_HD double abs_atan2_fast(double y, double x, const double (*tab)[2] = kAtanT
     const bool sw = ay > ax, nx = signbit(x);
     double num = sw ? ax : ay, den = sw ? ay : ax;
     if (!(ax + ay <= 0x1p+1000) || !(den >= 0x1p-1000)) {
-        if (isnan(x) || isnan(y)) return x + y;
-        if (isinf(den)) {
-            if (isinf(num)) return nx ? 0x1.2d97c7f3321d2p+1 : 0x1.921fb54442d18p-1;   // 3pi/4, pi/4
-            return sw ? kPio2.hi : (nx ? kPi.hi : 0.0);
-        }
-        if (den == 0.0) return nx ? kPi.hi : 0.0;
+        if (isnan(x) || isnan(y) || isinf(den) || den == 0.0)
+            return abs_atan2_special(y, x, num, den, sw, nx);
         int e;
         (void)frexp(den, &e);
         num = ldexp(num, -e);
         den = ldexp(den, -e);
     }
+    return abs_atan2_core(num, den, sw, nx, tab);
+}
+
+// abs_atan2_fast for arguments that are float values: their nonzero finite
+// magnitudes lie in [2^-149, 2^128], inside the core's range, so only
+// zeros, infinities and NaN (one class test and one comparison) leave the
+// core, for the special values.
+AXS_HD double abs_atan2_f32in(double y, double x, const double (*tab)[2] = kAtanTable)
+{
+    const double ax = fabs(x), ay = fabs(y);
+    const bool sw = ay > ax, nx = signbit(x);
+    const double num = sw ? ax : ay, den = sw ? ay : ax;
+#if defined(__HIP_DEVICE_COMPILE__)
+    // den +-0 or +inf (class bits 0x20, 0x40, 0x200), or NaN in either
+    const bool special = __builtin_amdgcn_class(den, 0x260) || !(num <= den);
+#else
+    const bool special = !(den > 0.0) || isinf(den) || !(num <= den);
+#endif
+    if (special) return abs_atan2_special(y, x, num, den, sw, nx);
     return abs_atan2_core(num, den, sw, nx, tab);
 }
 

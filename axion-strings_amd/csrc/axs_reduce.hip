@@ -148,6 +148,23 @@ __device__ __forceinline__ void stage_atan_table(double (*tab)[2])
     __syncthreads();
 }
 
+// sqrt(a^2 + b^2) for float values a, b: the sum of squares (a^2 exact in
+// double, one rounding) lies in {0} and [2^-298, 2^257], so the hardware
+// reciprocal square root needs no range scaling; one Goldschmidt step from
+// it and one residual correction (y + h (s - y^2)) give the root within
+// an ulp, and s = 0 gives 0.
+__device__ __forceinline__ double norm_f32in(double a, double b)
+{
+    const double s = fma(a, a, b * b);
+    const double r = __builtin_amdgcn_rsq(s);
+    double y = s * r, h = 0.5 * r;
+    const double e = fma(-h, y, 0.5);
+    y = fma(y, e, y);
+    h = fma(h, e, h);
+    y = fma(fma(-y, y, s), h, y);
+    return (s > 0.0) ? y : 0.0;
+}
+
 // FP32 form: the six background sums and the three energy components in one
 // pass over the stored (phi, u) state, nothing written but the block sums.
 // phidot = u - dt/2 ker is rebuilt per site with canonical_fp32_kernel's
@@ -303,8 +320,8 @@ sample_fp32_kernel(int N, int CI, uint32_t ncols, const fvec4 *__restrict__ in,
             acc[1] += b;
             acc[2] += va;
             acc[3] += vb;
-            acc[4] += abs_atan2_fast(a, b, atab);
-            acc[5] += sqrt(a * a + b * b);
+            acc[4] += abs_atan2_f32in(a, b, atab);
+            acc[5] += norm_f32in(a, b);
             // forward differences in energy_kernel's axis order
             const fvec2 fs = ring[R + 1], fa = cur.a[0];
             double g = 0.0, d1, d2;

@@ -17,3 +17,8 @@ extern "C" void axs_host_abs_atan2_fast(int64_t n, const double *y, const double
 {
     for (int64_t i = 0; i < n; ++i) out[i] = axs::abs_atan2_fast(y[i], x[i]);
 }
+
+extern "C" void axs_host_abs_atan2_f32in(int64_t n, const float *y, const float *x, double *out)
+{
+    for (int64_t i = 0; i < n; ++i) out[i] = axs::abs_atan2_f32in(y[i], x[i]);
+}

@@ -47,6 +47,10 @@ void axs_host_atan2_dd(int64_t n, const double *y, const double *x, double *out)
    device (within 6e-16 of the exact angle). */
 void axs_host_abs_atan2_fast(int64_t n, const double *y, const double *x, double *out);
 
+/* The same for float arguments (the FP32 form's phi), by the shortcut the
+   FP32 background sums take: the same bits as axs_host_abs_atan2_fast. */
+void axs_host_abs_atan2_f32in(int64_t n, const float *y, const float *x, double *out);
+
 /* run_standard (s_run.cpp:26-232) with the parameters of `path`; returns 0
  * on success.  This is what axs_main runs.                                */
 int axs_host_run(const char *path);

@@ -165,6 +165,35 @@ def test_abs_atan2_fast_bound(seed):
     assert (err == 0).mean() > 0.6
 
 
+def test_abs_atan2_f32in_matches_fast():
+    """The float-argument shortcut of the FP32 background sums gives
+    abs_atan2_fast's bits: random, wide-range and near-table-midpoint float
+    pairs, and every pair of float special values (zeros, infinities, NaN,
+    the smallest subnormal, the largest finite)."""
+    h = axs.load_host_library()
+    f = h.axs_host_abs_atan2_f32in
+    f.argtypes = [ctypes.c_int64, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                  ctypes.POINTER(ctypes.c_double)]
+    f.restype = None
+    rng = np.random.default_rng(8)
+    n = 200000
+    y = (rng.standard_normal(n) * np.exp(rng.uniform(-80, 80, n))).astype(np.float32)
+    x = (rng.standard_normal(n) * np.exp(rng.uniform(-80, 80, n))).astype(np.float32)
+    xm = rng.standard_normal(n).astype(np.float32)
+    mid = ((np.arange(n) % 256 + 0.5) / 256).astype(np.float32)
+    sp = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1e-45, -1e-45, 3.4028235e38, 1.0, -1.0],
+                  np.float32)
+    ys, xs = np.meshgrid(sp, sp, indexing="ij")
+    y = np.ascontiguousarray(np.concatenate([y, xm * mid, ys.ravel()]), np.float32)
+    x = np.ascontiguousarray(np.concatenate([x, xm, xs.ravel()]), np.float32)
+    out = np.empty(y.size)
+    f(y.size, y.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+      x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+      out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    want = _fast(y.astype(np.float64), x.astype(np.float64))
+    np.testing.assert_array_equal(out, want)
+
+
 def test_abs_atan2_fast_special_values():
     """C99 Annex F values exactly where a component is infinite or NaN or
     both are zero (NaN for NaN); the other pairs of the grid (one zero,

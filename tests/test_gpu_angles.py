@@ -265,3 +265,35 @@ def test_background_angle_each_value(prec):
     err = np.abs(got - want)
     assert err.max() <= 4.5e-16, (err.max(), a[err.argmax()], b[err.argmax()])
     assert np.all(err <= 2 * np.spacing(want))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_background_norm_each_value(prec):
+    """The |phi| term of the background sums (s_run.cpp:146), value by
+    value: each member of an ensemble has one site at (a, b) and every other
+    site at (0, 0), so the member's |phi| sum is the device's
+    sqrt(a^2 + b^2) itself.  Each must lie within one ulp of the root of the
+    exact sum of squares (long double, then rounded); zeros, tiny and huge
+    magnitudes and both slice layouts (sides 64 and 16) included."""
+    rng = np.random.default_rng(2025)
+    M = 384
+    Ns = [64] * (M // 2) + [16] * (M // 2)
+    a = rng.standard_normal(M) * np.exp(rng.uniform(-30, 30, M))
+    b = rng.standard_normal(M) * np.exp(rng.uniform(-30, 30, M))
+    a[:8], b[:8] = 0.0, 0.0
+    b[8:16] = 0.0
+    a[16:24] = b[16:24] * (1 + rng.uniform(-1e-6, 1e-6, 8))
+    if prec == "fp32":
+        a, b = a.astype(np.float32).astype(np.float64), b.astype(np.float32).astype(np.float64)
+    lat = axs.Lattice(2, members=Ns, precision=prec)
+    for r, N in enumerate(Ns):
+        p1, p2 = np.zeros(N * N), np.zeros(N * N)
+        p1[0], p2[0] = a[r], b[r]
+        z = np.zeros(N * N)
+        lat.set_state(p1, p2, z, z, member=r)
+    got = np.array([lat.background(member=r)[5] for r in range(M)])
+    ld = np.longdouble
+    want = np.sqrt(ld(a) * ld(a) + ld(b) * ld(b)).astype(np.float64)
+    assert np.all(got[:8] == 0.0)
+    err = np.abs(got - want)
+    assert np.all(err <= np.spacing(want)), (err / np.spacing(want)).max()
