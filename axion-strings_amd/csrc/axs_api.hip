@@ -1017,11 +1017,11 @@ int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
                                                      : ((W == 8 && c->prec == AXS_FP32) ? 3 : 2)));
         rows_ok = wg != (AXS_MARCH_WG512 | AXS_MARCH_WG1024) &&
                   multi_shape_built(depth, W, R, c->prec, RAD);
-        bool fits = cfg->kernel != AXS_KERNEL_SIMPLE && depth <= (RAD > 1 ? 2 : 4);
+        bool fits = cfg->kernel != AXS_KERNEL_SIMPLE && depth <= (RAD > 1 ? 2 : 5);
         for (const Member &m : c->members) fits = fits && march2_supported(c, m.N);
         if (!fits) {
             set_error("depth %d needs 3D, the stencil Laplacian, sides >= 64, a non-simple "
-                      "kernel and depth <= 4 (stencil 0) or <= 2 (higher orders)", depth);
+                      "kernel and depth <= 5 (stencil 0) or <= 2 (higher orders)", depth);
             return AXS_ERR_UNSUPPORTED;
         }
     } else if (RAD > 1) {
@@ -1038,7 +1038,7 @@ int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
                   (cfg->rows == 2 && c->prec == AXS_FP64);
     }
     if (cfg->kernel < AXS_KERNEL_AUTO || cfg->kernel > AXS_KERNEL_MARCH || !rows_ok ||
-        cfg->chunk < 0 || depth < 1 || depth > (c->nd == 2 ? 8 : 4) || (cfg->flags & ~63) != 0) {
+        cfg->chunk < 0 || depth < 1 || depth > (c->nd == 2 ? 8 : 5) || (cfg->flags & ~63) != 0) {
         set_error("unsupported kernel configuration (kernel %d rows %d chunk %d depth %d "
                   "flags %d)", cfg->kernel, cfg->rows, cfg->chunk, cfg->depth, cfg->flags);
         return AXS_ERR_UNSUPPORTED;

@@ -70,7 +70,7 @@ int stencil_coefficients(int stencil, int ndims, double *c);
 StepScalars make_scalars(const axs_params &p, double tau_after);
 StepScalarsF to_float(const StepScalars &s);
 
-// most time steps one multi-step launch takes (2D strips: 8, 3D: 4)
+// most time steps one multi-step launch takes (2D strips: 8, 3D: 5)
 constexpr int kMaxDepth = 8;
 
 // ---------------------------------------------------------------------------

@@ -12,7 +12,8 @@ bool march2_supported(const Context *c, int N)
 }
 
 // shapes built per form and stencil radius (none spills registers).
-// Radius 1: FP32 depth 2-4 with 4, 8 or 16 waves; FP64 (double pairs) 8
+// Radius 1: FP32 depth 2-4 with 4, 8 or 16 waves, depth 5 with 16 waves of
+// 2 rows (115 registers; deeper or taller shapes spill); FP64 (double pairs) 8
 // waves, 2 or 3 rows, depth 2 or 3 (16 waves would spill: 128 registers per
 // lane).  Radii 2-4: FP32 16 waves of 2 rows, depth 1 or 2 (radius 4: 1);
 // FP64 8 waves of 2 rows (radius 2: or 3), depth 1.
@@ -27,7 +28,7 @@ bool multi_shape_built(int D, int W, int R, int prec, int RAD)
     switch (W) {
     case 4: return (D == 2 && (R == 4 || R == 6 || R == 8)) || (D == 3 && (R == 4 || R == 6));
     case 8: return D >= 2 && D <= 4 && R >= 2 && R <= 4;
-    case 16: return D >= 2 && D <= 4 && (R == 2 || R == 3);
+    case 16: return (D >= 2 && D <= 4 && (R == 2 || R == 3)) || (D == 5 && R == 2);
     default: return false;
     }
 }
@@ -109,7 +110,9 @@ struct Rad1Shapes {
     template <class F>
     int operator()(int W, int R, F &f) const
     {
-        if constexpr (std::is_same<Tr, F32Form>::value) {
+        if constexpr (std::is_same<Tr, F32Form>::value && D == 5) {
+            if (W == 16 && R == 2) return f.template run<16, 2>();
+        } else if constexpr (std::is_same<Tr, F32Form>::value) {
             if (W == 16) {
                 if (R == 3) return f.template run<16, 3>();
                 return f.template run<16, 2>();
@@ -149,11 +152,12 @@ static int launch_rad1(Context *c, int src, int dst, const StepScalars *s)
 template <class Tr>
 static int launch_form(Context *c, int src, int dst, const StepScalars *s, int depth)
 {
-    if (depth == 4) {
+    if (depth == 4 || depth == 5) {
         if constexpr (std::is_same<Tr, F32Form>::value) {
-            return launch_rad1<Tr, 4>(c, src, dst, s);
+            return (depth == 5) ? launch_rad1<Tr, 5>(c, src, dst, s)
+                                : launch_rad1<Tr, 4>(c, src, dst, s);
         } else {
-            set_error("FP64 depth 4 is not built");
+            set_error("FP64 depth %d is not built", depth);
             return AXS_ERR_UNSUPPORTED;
         }
     }

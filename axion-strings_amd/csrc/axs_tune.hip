@@ -242,7 +242,7 @@ int run_autotune(Context *c, int trials, double *best_ms)
         // several steps per pass: (depth, waves per work-group, rows per wave)
         const int shapes32[][3] = {{2, 4, 4}, {2, 8, 2}, {2, 8, 3}, {3, 4, 4}, {3, 8, 2},
                                    {3, 8, 3}, {3, 8, 4}, {3, 16, 2}, {4, 8, 3}, {4, 8, 4},
-                                   {4, 16, 2}, {4, 16, 3}};
+                                   {4, 16, 2}, {4, 16, 3}, {5, 16, 2}};
         const int shapes64[][3] = {{2, 8, 2}, {2, 8, 3}, {3, 8, 2}, {3, 8, 3}};
         std::vector<std::array<int, 3>> shapes;
         if (c->prec == AXS_FP32)

@@ -268,7 +268,8 @@ typedef struct axs_kernel_config {
     int chunk;    /* planes marched per work-group (divides N)              */
     int depth;    /* time steps per pass over the lattice: 1 (0 = 1), or,
                      with stencil 0, the stencil Laplacian and every side
-                     >= 64: 3D FP32 2 to 4 (a new context starts at 4
+                     >= 64: 3D FP32 2 to 4, or 5 with 16 waves of 2 rows
+                     (a new context starts at 4
                      when its lattices fill the chip in 32-plane chunks --
                      192^3 and up, ensembles -- else 3, with
                      AXS_MARCH_WG1024), 3D FP64 2 or 3 with AXS_MARCH_WG512

@@ -128,7 +128,8 @@ def test_autotune_keeps_state_and_results(nd, Ns, precision):
             np.testing.assert_array_equal(a[k], b[k])
 
 
-@pytest.mark.parametrize("depth,flags,rows", [(2, 3, 0), (3, 35, 0), (4, 19, 3), (3, 2, 4)])
+@pytest.mark.parametrize("depth,flags,rows", [(2, 3, 0), (3, 35, 0), (4, 19, 3), (3, 2, 4),
+                                               (5, 35, 0)])
 def test_multi_step_kernel_on_ensemble_matches_single(depth, flags, rows):
     # several steps per launch on an uneven FP32 ensemble (one persistent
     # launch for all members): every member equals its lone one-step run

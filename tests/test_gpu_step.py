@@ -174,9 +174,10 @@ def test_march_kernel_fp64_oracle_192():
     (64, 3, 4, 0, 3, 9), (100, 3, 6, 0, 3, 10), (128, 3, 4, 32, 2, 8), (128, 3, 4, 0, 19, 7),
     (192, 3, 3, 0, 27, 6), (128, 3, 6, 16, 3, 3), (80, 3, 2, 0, 18, 11), (128, 3, 2, 0, 34, 7),
     (128, 4, 3, 0, 19, 9), (100, 4, 2, 0, 35, 8), (192, 4, 4, 32, 18, 7), (64, 2, 3, 16, 35, 6),
-    (64, 4, 3, 0, 34, 12)])
+    (64, 4, 3, 0, 34, 12), (128, 5, 2, 0, 35, 11), (100, 5, 2, 0, 35, 13),
+    (192, 5, 2, 32, 34, 10), (64, 5, 2, 16, 35, 7)])
 def test_multi_step_kernel_matches_one_step(N, depth, rows, chunk, flags, nsteps):
-    # two to four time steps per pass (temporal blocking) evaluate the same
+    # two to five time steps per pass (temporal blocking) evaluate the same
     # expressions as one-step launches: identical bits on every side (also
     # sides the tiles do not divide), launch shape (4, 8 or 16 waves per
     # work-group: AXS_MARCH_WG512 / AXS_MARCH_WG1024) and step count (a
@@ -217,7 +218,8 @@ def test_two_step_kernel_rejects_other_forms(nd, N, precision, kernel):
 
 
 @pytest.mark.parametrize("depth,rows,flags", [(4, 0, 3), (5, 0, 19), (3, 8, 3), (2, 6, 19),
-                                              (2, 5, 3), (3, 4, 35), (2, 2, 51)])
+                                              (2, 5, 3), (3, 4, 35), (2, 2, 51), (5, 3, 35),
+                                              (6, 0, 35)])
 def test_multi_step_kernel_rejects_unbuilt_shapes(depth, rows, flags):
     lat = axs.Lattice(3, 64, precision="fp32")
     before = lat.kernel_config()
