@@ -523,7 +523,7 @@ static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ct
     else if (c->nd == 2)   // FP32 from 2^25 sites on: eight (13% at 8192^2)
         c->tune.depth = (c->prec == AXS_FP32 && total_sites(c) >= (1LL << 25)) ? 8 : 6;
     else   // four steps, 16 waves of 2 rows: tiles of 24 x 56 sites
-        c->tune.depth = (c->prec == AXS_FP32) ? (multi_chunk_cap(c, 24, 56) >= 32 ? 4 : 3) : 3;
+        c->tune.depth = (c->prec == AXS_FP32) ? (multi_chunk_cap(c, 24, 56, 4) >= 32 ? 4 : 3) : 3;
     if (c->nd == 3) c->tune.flags |= (c->prec == AXS_FP32) ? kTuneWG1024 : kTuneWG512;
     if (multi_depth(c) == 0) {
         c->tune.depth = 1;

@@ -269,7 +269,7 @@ int multi_waves(int flags);
 int multi_waves_of(const Context *c);
 int multi_rows(const Context *c, int N, int D, int W);
 // chunk length cap of the 3D multi-step kernels for tiles TJ x TK
-int multi_chunk_cap(const Context *c, int TJ, int TK);
+int multi_chunk_cap(const Context *c, int TJ, int TK, int H);
 // largest divisor of N not above cap
 inline int divisor_below(int N, int cap)
 {

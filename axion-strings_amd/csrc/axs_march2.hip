@@ -51,10 +51,9 @@ int stencil_radius_of(const Context *c) { return c->p.stencil_setting + 1; }
 // ensemble), where the former rule (fill the chip with at least 3/4
 // (FP32) or 4 (FP64) items per CU) took 8 at FP64 192^3 (1.6x slower)
 // and 128 at FP32 384^3 (1.2x slower).
-int multi_chunk_cap(const Context *c, int TJ, int TK)
+int multi_chunk_cap(const Context *c, int TJ, int TK, int H)
 {
     const bool f32 = c->prec == AXS_FP32;
-    const int H = (kTBW - TK) / 2;
     const uint64_t cus = (uint64_t)(c->num_cu > 0 ? c->num_cu : 1);
     int min_side = 1 << 30;
     for (const Member &m : c->members) min_side = m.N < min_side ? m.N : min_side;
@@ -216,7 +215,7 @@ int prepare_multi_batch(Context *c, int D)
     const int R = multi_rows(c, c->members[0].N, D, W);
     const int key = (((D * 10 + RAD) * 100 + W) * 100 + R) * 10000 + c->tune.ci32;
     if (key == mt.key && mt.tab[0]) return AXS_OK;
-    const int TJ = W * R - 2 * D * RAD, TK = kTBW - 2 * D * RAD;
+    const int TJ = W * R - 2 * D * RAD, TK = tile_cols(D * RAD);
     std::vector<BatchMember> tab[2];
     uint32_t items = 0;
     for (const Member &m : c->members) {
@@ -226,7 +225,7 @@ int prepare_multi_batch(Context *c, int D)
         b.item_begin = items;
         b.nTj = (m.N + TJ - 1) / TJ;
         b.nTk = (m.N + TK - 1) / TK;
-        b.CI = multi_chunk(c, m.N, TJ, TK);
+        b.CI = multi_chunk(c, m.N, TJ, TK, D * RAD);
         items += (uint32_t)b.nTj * b.nTk * (m.N / b.CI);
         for (int h = 0; h < 2; ++h) {
             for (int a = 0; a < 3; ++a) {

@@ -44,6 +44,17 @@ namespace axs {
 
 constexpr int kTBW = 64;   // columns loaded per row (one wave)
 
+// The columns of a tile: each row loads the 64 columns [k0 - HK, k0 - HK +
+// 64) around the TK columns [k0, k0 + TK) it writes.  With a halo of at most
+// four sites the rows keep four halo columns on each side and write 56, and
+// the tiles start at k0 = 56 tk + 4: every loaded row segment then begins
+// on a 128-byte line (56 sites of 16 bytes are 7 lines) and takes 8 lines,
+// not 9.  Deeper halos keep HK = H, TK = 64 - 2H, k0 = TK tk.  The tiles
+// cover the columns [origin, origin + N), wrapping at N.
+__host__ __device__ constexpr int tile_halo_k(int H) { return H <= 4 ? 4 : H; }
+__host__ __device__ constexpr int tile_cols(int H) { return kTBW - 2 * tile_halo_k(H); }
+__host__ __device__ constexpr int tile_origin(int H) { return H <= 4 ? 4 : 0; }
+
 // FP32 7-point stencil: the k neighbours of a site are x of the lanes next
 // to it, which a DPP whole-wave shift hands to the add as its operand
 // (folded into the v_add: no LDS read, no extra instruction).  The end
@@ -126,7 +137,9 @@ __device__ __forceinline__ void marchd_tile(const int N, const int CI, const int
     constexpr int ER = W * R;            // rows loaded
     constexpr int H = D * RAD;           // halo width
     constexpr int TJ = ER - 2 * H;       // rows written
-    constexpr int TK = kTBW - 2 * H;     // columns written
+    constexpr int HK = tile_halo_k(H);   // halo columns on each side
+    constexpr int TK = tile_cols(H);     // columns written
+    constexpr int K0 = tile_origin(H);
     static_assert(TJ > 0 && TK > 0, "tile shape");
 
     // the wave index is uniform: held in a scalar register, so the row
@@ -134,12 +147,14 @@ __device__ __forceinline__ void marchd_tile(const int N, const int CI, const int
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t NN = (uint32_t)N * (uint32_t)N;
-    const int j0 = tj * TJ, k0 = tk * TK, i0 = ch * CI;
+    const int j0 = tj * TJ, k0 = tk * TK + K0, i0 = ch * CI;
 
-    // column of this lane, wrapped: -H <= k0 - H + lane < 2N for N >= 64
-    int kk = k0 - H + lane;
+    // column of this lane, wrapped: -HK <= k0 - HK + lane < 2N for N >= 64;
+    // it is written if it lies in the lanes' output range and in the first
+    // pass of the columns from the tiles' origin
+    int kk = k0 - HK + lane;
     kk = (kk < 0) ? kk + N : ((kk >= N) ? kk - N : kk);
-    const bool colout = lane >= H && lane < kTBW - H && (k0 + lane - H) < N;
+    const bool colout = lane >= HK && lane < kTBW - HK && (k0 + lane - HK) < N + K0;
 
     uint32_t rowoff[R];
     int erow[R];
@@ -397,11 +412,11 @@ marchd_batch_kernel(const BatchMember *__restrict__ mem, int n_members, uint32_t
 
 // planes per chunk of a member of side N, tiles TJ x TK: the configured
 // count if it divides N, else the ensemble-wide choice of multi_chunk_cap
-inline int multi_chunk(const Context *c, int N, int TJ, int TK)
+inline int multi_chunk(const Context *c, int N, int TJ, int TK, int H)
 {
     const int ci = c->tune.ci32;
     if (ci > 0 && N % ci == 0) return ci;
-    return divisor_below(N, multi_chunk_cap(c, TJ, TK));
+    return divisor_below(N, multi_chunk_cap(c, TJ, TK, H));
 }
 
 template <class Sc>
@@ -425,13 +440,13 @@ struct SingleLaunch {
     int run()
     {
         constexpr int TJ = W * R - 2 * D * RAD;
-        constexpr int TK = kTBW - 2 * D * RAD;
+        constexpr int TK = tile_cols(D * RAD);
         static_assert(TJ > 0 && W * R <= 64, "tile shape");
         TBGeom g;
         g.N = N;
         g.nTj = (N + TJ - 1) / TJ;
         g.nTk = (N + TK - 1) / TK;
-        g.CI = multi_chunk(c, N, TJ, TK);
+        g.CI = multi_chunk(c, N, TJ, TK, D * RAD);
         const uint32_t nb = (uint32_t)g.nTj * (uint32_t)g.nTk * (uint32_t)(N / g.CI);
         tile_order<Tr>(g);
         if (pf)
