@@ -103,7 +103,8 @@ inline void tile_order(TBGeom &g)
     g.xcd = f32 ? 0 : 1;
 }
 
-// tile (tj, tk) of logical index t of a layer of nTj x nTk tiles
+// tile (tj, tk) of logical index t of a layer of nTj x nTk tiles (band >= 1;
+// band 1 is k fastest)
 __device__ __forceinline__ void tile_of(uint32_t t, const TBGeom &g, int &tj, int &tk)
 {
     const uint32_t nTk = (uint32_t)g.nTk, nTj = (uint32_t)g.nTj;
@@ -286,19 +287,19 @@ __device__ __forceinline__ void marchd_tile(const int N, const int CI, const int
                     nx[r][1] = (l == D) ? y : ey;
                     no[r] = on;
                 }
-                if (l == D && kLaneK<T, RAD>) {
-                    if constexpr (kLaneK<T, RAD>) {
+                if constexpr (kLaneK<T, RAD>) {
+                    if (l == D) {
                         // the stored values are computed here, for every
                         // lane: sunk into the stores' masked branches, the
                         // adds would lose their DPP operands to separate moves
 #pragma unroll
                         for (int r = 0; r < R; ++r)
                             __asm__ volatile("" : "+v"(no[r].u1), "+v"(no[r].u2));
-                    }
 #pragma unroll
-                    for (int r = 0; r < R; ++r)
-                        if (it >= 0 && rowout[r] && colout)
-                            Tr::store(out, po + rowoff[r], nx[r][0], nx[r][1], no[r]);
+                        for (int r = 0; r < R; ++r)
+                            if (it >= 0 && rowout[r] && colout)
+                                Tr::store(out, po + rowoff[r], nx[r][0], nx[r][1], no[r]);
+                    }
                 }
             }
             // shift level l by one plane: the value handed up enters at the top
