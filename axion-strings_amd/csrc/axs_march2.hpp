@@ -152,18 +152,28 @@ __device__ __forceinline__ void marchd_tile(const int N, const int CI, const int
 
     // column of this lane, wrapped: -HK <= k0 - HK + lane < 2N for N >= 64;
     // it is written if it lies in the lanes' output range and in the first
-    // pass of the columns from the tiles' origin
-    int kk = k0 - HK + lane;
+    // pass of the columns from the tiles' origin.  In the last tile of a row
+    // of tiles, the lanes past the halo of its last written column load the
+    // column of the last lane still needed: the same lines, so no fetch
+    // (their values, like every lane's outside the levels' regions, reach
+    // no stored site: a wrong value spreads one stencil radius per level)
+    const int wcols = (N + K0 - k0 < TK) ? N + K0 - k0 : TK;       // columns written
+    const int lane_ld = (lane < wcols + 2 * HK) ? lane : wcols + 2 * HK - 1;
+    int kk = k0 - HK + lane_ld;
     kk = (kk < 0) ? kk + N : ((kk >= N) ? kk - N : kk);
     const bool colout = lane >= HK && lane < kTBW - HK && (k0 + lane - HK) < N + K0;
 
     uint32_t rowoff[R];
     int erow[R];
     bool rowout[R];
+    // rows likewise: past the halo of the last written row, a copy of the
+    // last row still needed (the same lines, fetched once)
+    const int wrows = (N - j0 < TJ) ? N - j0 : TJ;                   // rows written
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int e = wave * R + r;                 // row within the loaded region
-        int j = j0 - H + e;                         // -H <= j < 2N (ER <= 64 <= N)
+        const int e_ld = (e < wrows + 2 * H) ? e : wrows + 2 * H - 1;
+        int j = j0 - H + e_ld;                      // -H <= j < 2N (ER <= 64 <= N)
         j = (j < 0) ? j + N : ((j >= N) ? j - N : j);
         rowoff[r] = (uint32_t)j * (uint32_t)N + (uint32_t)kk;
         erow[r] = e;
