@@ -175,7 +175,8 @@ def test_march_kernel_fp64_oracle_192():
     (192, 3, 3, 0, 27, 6), (128, 3, 6, 16, 3, 3), (80, 3, 2, 0, 18, 11), (128, 3, 2, 0, 34, 7),
     (128, 4, 3, 0, 19, 9), (100, 4, 2, 0, 35, 8), (192, 4, 4, 32, 18, 7), (64, 2, 3, 16, 35, 6),
     (64, 4, 3, 0, 34, 12), (128, 5, 2, 0, 35, 11), (100, 5, 2, 0, 35, 13),
-    (192, 5, 2, 32, 34, 10), (64, 5, 2, 16, 35, 7)])
+    (192, 5, 2, 32, 34, 10), (64, 5, 2, 16, 35, 7), (66, 4, 2, 0, 35, 9), (70, 5, 2, 0, 35, 7),
+    (130, 4, 2, 0, 35, 6), (122, 3, 3, 0, 19, 5)])
 def test_multi_step_kernel_matches_one_step(N, depth, rows, chunk, flags, nsteps):
     # two to five time steps per pass (temporal blocking) evaluate the same
     # expressions as one-step launches: identical bits on every side (also
@@ -262,7 +263,7 @@ def test_2d_multi_step_rejects_unbuilt_shapes(depth, rows, flags):
 
 @pytest.mark.parametrize("N,depth,rows,chunk,flags,nsteps", [
     (64, 2, 2, 0, 19, 8), (64, 3, 3, 16, 18, 7), (100, 2, 3, 0, 19, 9), (128, 3, 2, 32, 27, 10),
-    (192, 2, 2, 0, 18, 5)])
+    (192, 2, 2, 0, 18, 5), (66, 3, 2, 0, 19, 6), (70, 2, 3, 0, 19, 5)])
 def test_fp64_multi_step_bit_exact_vs_oracle(N, depth, rows, chunk, flags, nsteps):
     # the plain FP64 form, several steps per pass (8 waves per work-group):
     # the reference's velocity-Verlet bit for bit, ker included
