@@ -203,7 +203,9 @@ __global__ void __launch_bounds__(64 * kWaves2D)
 march2d_kernel(const BatchMember *__restrict__ mem, int n_members, uint32_t n_items,
                LevelScalars<typename Tr::Sc, D> sc)
 {
-    const uint32_t wave = blockIdx.x * kWaves2D + (threadIdx.x >> 6);
+    // the wave index is uniform: held in a scalar register, so the item's
+    // member lookup, sides and row offsets are scalar work
+    const uint32_t wave = blockIdx.x * kWaves2D + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t stride = gridDim.x * kWaves2D;
     for (uint32_t item = wave; item < n_items; item += stride) {
         int lo = 0, hi = n_members - 1;        // last member starting at or before item

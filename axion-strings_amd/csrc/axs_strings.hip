@@ -302,7 +302,8 @@ __global__ void __launch_bounds__(64 * kSfW)
 strings_march_kernel(SfGeom g, P phi, double accept, unsigned long long *__restrict__ words)
 {
     __shared__ float A[2][kSfTJ + 1][65];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // the wave index is uniform (a scalar register): row arithmetic is scalar
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int N = g.N;
     // XCD-aware order (work-groups go round-robin to the 8 XCDs): each XCD
     // takes a contiguous run of tiles, k fastest, so the 65th column and the
