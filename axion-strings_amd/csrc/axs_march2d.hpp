@@ -21,8 +21,9 @@
 // computes every level (a lane mask would save no issue cycles; values
 // outside a level's region never reach a stored site).  The Laplacian
 // sums in the one-step kernels' order (centre, then for each distance m:
-// +i, -i, +j, -j, each times the stencil coefficient) with the same
-// expressions, so a pass is bit-identical to D one-step launches.  Work
+// FP64 +i, -i, +j, -j, the FP32 form +j, -j, +i, -i, each times the
+// stencil coefficient) with the same expressions, so a pass is
+// bit-identical to D one-step launches.  Work
 // items (member, strip, chunk) are dealt to waves from a persistent list;
 // strips at the lattice edge overhang it (columns wrap periodically, stores
 // are masked).
@@ -134,14 +135,29 @@ __device__ __forceinline__ void march2d_strip(const int N, const int CJ, const i
                         const int up = (RAD + m < 2 * RAD) ? RAD + m : 2 * RAD - 1;
                         const T ux = (m == RAD) ? fx[0] : ring[l - 1][up][0];
                         const T uy = (m == RAD) ? fx[1] : ring[l - 1][up][1];
-                        a = add(a, xr[m - 1]);
-                        c = add(c, yr[m - 1]);                   // i+m
-                        a = add(a, xl[m - 1]);
-                        c = add(c, yl[m - 1]);                   // i-m
-                        a = add(a, ux);
-                        c = add(c, uy);                          // j+m
-                        a = add(a, ring[l - 1][RAD - m][0]);
-                        c = add(c, ring[l - 1][RAD - m][1]);     // j-m
+                        if constexpr (std::is_same<T, float>::value) {
+                            // the FP32 form's order: j, then i -- the sum
+                            // starts from a register and both lane shifts
+                            // fold into their adds as DPP operands
+                            a = add(a, ux);
+                            c = add(c, uy);                      // j+m
+                            a = add(a, ring[l - 1][RAD - m][0]);
+                            c = add(c, ring[l - 1][RAD - m][1]); // j-m
+                            a = add(a, xr[m - 1]);
+                            c = add(c, yr[m - 1]);               // i+m
+                            a = add(a, xl[m - 1]);
+                            c = add(c, yl[m - 1]);               // i-m
+                        } else {
+                            // FP64: the reference's COO order, i then j
+                            a = add(a, xr[m - 1]);
+                            c = add(c, yr[m - 1]);               // i+m
+                            a = add(a, xl[m - 1]);
+                            c = add(c, yl[m - 1]);               // i-m
+                            a = add(a, ux);
+                            c = add(c, uy);                      // j+m
+                            a = add(a, ring[l - 1][RAD - m][0]);
+                            c = add(c, ring[l - 1][RAD - m][1]); // j-m
+                        }
                     }
                     Own on;
                     T ex, ey;

@@ -297,14 +297,16 @@ sample_fp32_kernel(int N, int CI, uint32_t ncols, const fvec4 *__restrict__ in,
                         s1x = F32Math::acc(s1x, cl, q2.x);
                         s2x = F32Math::acc(s2x, cl, q2.y);
                     } else {
-                        s1x = F32Math::acc(s1x, cl, p1.x);
-                        s2x = F32Math::acc(s2x, cl, p1.y);
-                        s1x = F32Math::acc(s1x, cl, q1.x);
-                        s2x = F32Math::acc(s2x, cl, q1.y);
+                        // the FP32 form's 2D order: j (the slow axis, from
+                        // the ring), then i
                         s1x = F32Math::acc(s1x, cl, ring[R + l].x);
                         s2x = F32Math::acc(s2x, cl, ring[R + l].y);
                         s1x = F32Math::acc(s1x, cl, ring[R - l].x);
                         s2x = F32Math::acc(s2x, cl, ring[R - l].y);
+                        s1x = F32Math::acc(s1x, cl, p1.x);
+                        s2x = F32Math::acc(s2x, cl, p1.y);
+                        s1x = F32Math::acc(s1x, cl, q1.x);
+                        s2x = F32Math::acc(s2x, cl, q1.y);
                     }
                 }
                 const float t = F32Math::own(s.cl, F32Math::pot(c.x, c.y, s.pot0), s.ac0);

@@ -115,7 +115,11 @@ __device__ __forceinline__ void site_fp32(int N, uint32_t m, const fvec4 *__rest
     for (int l = 1; l < s.ncoef; ++l) {
         float cl = s.c[l];
 #pragma unroll
-        for (int a = 0; a < ND; ++a) {
+        for (int ax = 0; ax < ND; ++ax) {
+            // the FP32 form's axis order: 3D 0, 1, 2; 2D 1 (j), then 0 (i),
+            // so that the strip kernels start each sum from a register
+            // operand and take both lane-shifted i neighbours as DPP operands
+            const int a = (ND == 2) ? 1 - ax : ax;
             fvec4 p = in[shifted<ND>(site, a, l, N)];
             fvec4 q = in[shifted<ND>(site, a, -l, N)];
             s1 = F32Math::acc(s1, cl, F32Math::drift(p.x, p.z, s.dt));
@@ -344,15 +348,16 @@ canonical_fp32_kernel(int N, int CI, uint32_t ncols, const fvec4 *__restrict__ i
                 s1x = F32Math::acc(s1x, cl, q2.x);
                 s2x = F32Math::acc(s2x, cl, q2.y);
             } else {
-                // axis 0: in-plane i; axis 1: the slow axis j, from the ring
-                s1x = F32Math::acc(s1x, cl, p1.x);
-                s2x = F32Math::acc(s2x, cl, p1.y);
-                s1x = F32Math::acc(s1x, cl, q1.x);
-                s2x = F32Math::acc(s2x, cl, q1.y);
+                // the FP32 form's 2D order: axis 1 (the slow axis j, from
+                // the ring), then axis 0 (in-plane i)
                 s1x = F32Math::acc(s1x, cl, ring[R + l].x);
                 s2x = F32Math::acc(s2x, cl, ring[R + l].y);
                 s1x = F32Math::acc(s1x, cl, ring[R - l].x);
                 s2x = F32Math::acc(s2x, cl, ring[R - l].y);
+                s1x = F32Math::acc(s1x, cl, p1.x);
+                s2x = F32Math::acc(s2x, cl, p1.y);
+                s1x = F32Math::acc(s1x, cl, q1.x);
+                s2x = F32Math::acc(s2x, cl, q1.y);
             }
         }
         const float t = F32Math::own(s.cl, F32Math::pot(c.x, c.y, s.pot0), s.ac0);

@@ -42,8 +42,9 @@ def _ker_emulated(x, y, nd, stencil, tau, dx=0.5, lam=1.0, T0=4.0):
     ac0 = f32(alpha * c[0])
     alpha = f32(alpha)
     # numpy axes of the reference layout: 3D [i, j, k] (axis 0 = i);
-    # 2D [j, i] (axis 0 of the lattice = i is the last numpy axis)
-    axes = [0, 1, 2] if nd == 3 else [1, 0]
+    # 2D [j, i].  The FP32 form sums 3D i, j, k and 2D j, then i (numpy
+    # axis 0, then 1)
+    axes = [0, 1, 2] if nd == 3 else [0, 1]
     s1 = np.full(x.shape, -0.0, np.float32)
     s2 = np.full(x.shape, -0.0, np.float32)
     for l in range(1, len(c)):
