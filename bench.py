@@ -304,6 +304,14 @@ def ensemble_leg(np, axs, name, T0, steps_override=0, autotune=True):
     rel = np.abs(c32 - c64) / np.maximum(c64, 1.0)
     out["string_count_rel_diff_max"] = float(rel.max())
     out["string_count_rel_diff_mean"] = float(rel.mean())
+    # where the largest relative difference is (late in the run a small
+    # lattice holds a handful of strings, so one string is a large fraction)
+    # and the largest difference in strings
+    w = int(np.argmax(rel))
+    out["string_count_rel_diff_max_at"] = {"member": w, "side": Ns[w], "fp32": int(c32[w]),
+                                           "fp64": int(c64[w])}
+    out["string_count_abs_diff_max"] = int(np.abs(c32 - c64).max())
+    out["string_count_members_equal"] = int(np.sum(c32 == c64))
     out["string_count_total_fp32"] = int(c32.sum())
     out["string_count_total_fp64"] = int(c64.sum())
     pk_err = pk_pow = 0.0
