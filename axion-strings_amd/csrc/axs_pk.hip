@@ -714,34 +714,84 @@ fft_z_kernel(const PkSrc *__restrict__ src, double2 *__restrict__ y,
 }
 
 // Y pass (3D): rows Y[b][i][k_z][*] along j, R = kFftPtsY / N rows of
-// consecutive i per work-group (eight at N = 1024: whole 128-byte lines in
-// the transposed write); writes Y'[b][k_z][k_j][i]
+// consecutive i per group (eight at N = 1024: whole 128-byte lines in the
+// transposed write); writes Y'[b][k_z][k_j][i].  A group's rows fill most
+// of a CU's LDS, so one work-group runs per CU: each takes a contiguous run
+// of groups and loads the next group's points into registers while the
+// current one is transformed and stored, and reads the twiddles from a copy
+// in LDS (a global twiddle load would wait for those earlier loads).
 constexpr int kFftPtsY = 8192;
 
 template <int N>
 __global__ void __launch_bounds__(kFftPtsY / 16)
 fft_y_kernel(const double2 *__restrict__ ya, double2 *__restrict__ yb,
-             const double2 *__restrict__ tw)
+             const double2 *__restrict__ tw, uint32_t ngroups)
 {
     constexpr int M = N / 2, R = kFftPtsY / N, LS = fft_ls<N>(), GI = N / R;
-    constexpr int THR = kFftPtsY / 16;
+    constexpr int THR = kFftPtsY / 16, PER = kFftPtsY / THR;
+    static_assert(N <= 1024, "rows and twiddles must fit the LDS");
     __shared__ double2 buf[R * LS];
-    const uint32_t gq = blockIdx.x;
-    const uint32_t ig = gq % GI, t = gq / GI, kz = t % M, b = t / M;
-    const uint32_t i0 = ig * R;
-#pragma unroll 4
-    for (int q = 0; q < kFftPtsY / THR; ++q) {
-        const int e = threadIdx.x + q * THR;
-        const int r = e / N, j = e - r * N;
-        buf[r * LS + fpad(j)] = ya[(((size_t)b * N + i0 + r) * M + kz) * N + j];
+    __shared__ double2 twl[N];
+    for (int m = threadIdx.x; m < N; m += THR) twl[m] = tw[m];
+    const uint32_t g0 = (uint32_t)(((uint64_t)ngroups * blockIdx.x) / gridDim.x);
+    const uint32_t g1 = (uint32_t)(((uint64_t)ngroups * (blockIdx.x + 1)) / gridDim.x);
+    // point q of a thread is e = tid + q THR of the group: row e / N, column
+    // e mod N.  N and THR are powers of two, so its offsets split into a part
+    // of the thread and a constant of q: few registers stay live across the
+    // transform besides the prefetched points.
+    const uint32_t tid = threadIdx.x;
+    const uint32_t rl = (N <= THR) ? tid / N : 0, jl = (N <= THR) ? tid % N : tid;
+    const uint32_t ld_l = rl * (uint32_t)(M * N) + jl, st_l = rl * LS + fpad(jl);
+    auto ld_q = [](int q) { return (uint32_t)((q * THR / N) * M * N + (q * THR) % N); };
+    auto st_q = [](int q) { return (q * THR / N) * LS + fpad((q * THR) % N); };
+    auto src = [&](uint32_t gq) {
+        const uint32_t ig = gq % GI, t = gq / GI, kz = t % M, b = t / M;
+        return ya + (((size_t)b * N + ig * R) * M + kz) * N + ld_l;
+    };
+    // the prefetched points as plain doubles (a double2 copy goes through
+    // memory and would keep the array in scratch)
+    double nxr[PER], nxi[PER];
+    auto fetch = [&](uint32_t gq) {
+        const double2 *s = src(gq);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const double2 v = s[ld_q(q)];
+            nxr[q] = v.x;
+            nxi[q] = v.y;
+        }
+    };
+    auto put = [&]() {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) buf[st_l + st_q(q)] = double2{nxr[q], nxi[q]};
+    };
+    if (g0 < g1) {
+        fetch(g0);
+        put();
     }
-    __syncthreads();
-    fft_rows<N, kFftPtsY>(buf, tw, 1);
-#pragma unroll 4
-    for (int q = 0; q < kFftPtsY / THR; ++q) {
-        const int e = threadIdx.x + q * THR;
-        const int r = e % R, kj = e / R;
-        yb[(((size_t)b * M + kz) * N + kj) * N + i0 + r] = buf[r * LS + fpad(kj)];
+    // transposed write: point q is row tid mod R, column k_j = tid / R + q THR / R
+    const uint32_t wr = tid % R, wk = tid / R;
+    // Per group: the next group's loads are issued before the transform; the
+    // result is read out of LDS, the next group's points written in, and
+    // only then the result stored, so that no store is pending when the
+    // loads are waited for.
+    for (uint32_t gq = g0; gq < g1; ++gq) {
+        __syncthreads();
+        fetch(gq + 1 < g1 ? gq + 1 : gq);      // no branch around the loads
+        fft_rows<N, kFftPtsY>(buf, twl, 1);
+        double outr[PER], outi[PER];
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const double2 v = buf[wr * LS + fpad(wk + (uint32_t)q * (THR / R))];
+            outr[q] = v.x;
+            outi[q] = v.y;
+        }
+        __syncthreads();
+        put();
+        const uint32_t ig = gq % GI, t = gq / GI, kz = t % M, b = t / M;
+        double2 *dst = yb + ((size_t)b * M + kz) * N * N + ig * R + (size_t)wk * N + wr;
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+            dst[(size_t)q * (THR / R) * N] = double2{outr[q], outi[q]};
     }
 }
 
@@ -1145,20 +1195,25 @@ static void launch_own(Context *c, const FftPlan *p, int n, const BinLayout &B,
                        const ScanTabs &t, uint32_t nblk)
 {
     constexpr int M = N / 2;
-    if (c->nd == 3) {
+    if constexpr (N > 1024) {       // 3D sides up to 1024 (own_fft_ok)
+    } else if (c->nd == 3) {
         const uint32_t rpl = (uint32_t)N * N;
         constexpr int RZ = kFftPts / M, RY = kFftPtsY / N;
         hipLaunchKernelGGL((fft_z_kernel<3, N>), dim3(rpl / RZ, (unsigned)n), dim3(kFftThr), 0,
                            c->stream, t.src, p->ya, p->tw);
-        hipLaunchKernelGGL((fft_y_kernel<N>), dim3((uint32_t)n * M * (N / RY)),
-                           dim3(kFftPtsY / 16), 0, c->stream, p->ya, p->yb, p->tw);
+        const uint32_t ngy = (uint32_t)n * M * (N / RY);
+        hipLaunchKernelGGL((fft_y_kernel<N>),
+                           dim3(std::min<uint32_t>(ngy, (uint32_t)std::max(1, c->num_cu))),
+                           dim3(kFftPtsY / 16), 0, c->stream, p->ya, p->yb, p->tw, ngy);
         constexpr int TB = fft_bin_pts<N>() / 16;
         const size_t lds = (size_t)(TB / 64) * B.nbins * sizeof(double) +
                            (size_t)B.nbins * 4 + (size_t)M * 2;
         hipLaunchKernelGGL((fft_bin_kernel<3, N>), dim3(nblk, (unsigned)n), dim3(TB), lds,
                            c->stream, p->yb, p->tw, B.nbins, t.hi, t.shell, (uint32_t)(M * N),
                            t.part);
-    } else {
+        return;
+    }
+    if (c->nd != 3) {
         const uint32_t rpl = (uint32_t)N;
         constexpr int RZ = kFftPts / M;
         hipLaunchKernelGGL((fft_z_kernel<2, N>), dim3(rpl / RZ, (unsigned)n), dim3(kFftThr), 0,
