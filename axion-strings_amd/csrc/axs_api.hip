@@ -738,16 +738,23 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
         }
     if (nsteps == 0) return AXS_OK;
     AXS_TRY(prepare_batch(c));
-    // multi-step passes of `depth` steps; a remainder runs in shallower
-    // multi-step passes where one is built, else one step at a time (their
-    // work-item tables are built here, before any graph capture)
+    // multi-step passes of `depth` steps; a remainder runs in deeper or
+    // shallower multi-step passes where one is built (next_pass_depth), else
+    // one step at a time (the passes' work-item tables are built here,
+    // before any graph capture)
     const int depth = multi_depth(c);
-    if (depth >= 1)
-        for (int64_t r = nsteps % depth; r > 0;) {
-            const int d = remainder_depth(c, (int)r);
-            if (d >= 1) AXS_TRY(prepare_multi_batch(c, d));
-            r -= (d >= 1) ? d : 1;
+    if (depth >= 1) {
+        bool ready[kMaxDepth + 1] = {};
+        for (int64_t left = nsteps; left > 0;) {
+            const int d = next_pass_depth(c, left);
+            if (d >= 1 && !ready[d]) {
+                AXS_TRY(prepare_multi_batch(c, d));
+                ready[d] = true;
+            }
+            if (d == depth && left % depth == 0) break;     // only depth-D passes left
+            left -= (d >= 1) ? d : 1;
         }
+    }
     const double dt = (double)c->p.time_step;
     // the spectral step's rocFFT calls stay out of graph capture
     const bool graph = (c->tune.flags & kTuneGraph) && nsteps > 1 &&
@@ -767,7 +774,7 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
         double tau_next = c->tau + dt;                    // s_integrate.cpp:317
         StepScalars s = make_scalars(c->p, tau_next);
         const int64_t left = nsteps - n;
-        const int d = (depth < 1) ? 0 : (left >= depth ? depth : remainder_depth(c, (int)left));
+        const int d = next_pass_depth(c, left);
         if (!graph && c->step_fail_after >= 0 && launches >= c->step_fail_after) {
             set_error("axs_step: failure point after %d launches (AXS_STEP_FAIL_AFTER)",
                       launches);

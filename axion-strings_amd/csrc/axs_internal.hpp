@@ -258,6 +258,7 @@ int multi_depth_for(const Context *c, int D);
 // steps per pass of the multi-step pass that finishes a remainder of r < D
 // steps (the deepest usable one, at most r), or 0: one-step launches
 int remainder_depth(const Context *c, int r);
+int next_pass_depth(const Context *c, int64_t left);
 // forget the work-item tables (the launch shape changed)
 void invalidate_multi_tables(Context *c);
 // time steps per launch the configuration gives every member (>= 1)

@@ -174,6 +174,22 @@ int remainder_depth(const Context *c, int r)
     return 0;
 }
 
+// Depth of the next pass with `left` steps to go: passes of the configured
+// depth D; a remainder of one or two steps is taken up by as many passes of
+// D + 1 where that depth is built (the FP32 3D pass: a 5-step pass costs
+// about what a 4-step pass does per step, a 2-step pass ~1.7 times as much
+// and a lone step ~3 times as much), else by shallower passes at the end.
+// Every depth gives the bits of one-step launches, so only the time changes.
+int next_pass_depth(const Context *c, int64_t left)
+{
+    const int D = multi_depth(c);
+    if (D < 1) return 0;
+    const int64_t r = left % D;
+    if (r >= 1 && r <= 2 && left >= r * (D + 1) && multi_depth_for(c, D + 1) == D + 1)
+        return D + 1;
+    return left >= D ? D : remainder_depth(c, (int)left);
+}
+
 void invalidate_multi_tables(Context *c)
 {
     for (Context::MultiTables &t : c->mtab) t.key = -1;

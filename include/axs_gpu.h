@@ -280,9 +280,12 @@ typedef struct axs_kernel_config {
                      FP64 1, 2D FP32 1-4, 2D FP64 1 or 2; a new context
                      starts at 3D FP32 2 (stencil 3: 1), FP64 1, 2D FP32 4,
                      3, 2 (stencil 1, 2, 3), FP64 2.
-                     Step counts that are not a multiple end with
-                     shallower multi-step passes where built, else with
-                     one-step launches.                                     */
+                     Step counts that are not a multiple take a
+                     remainder of one or two steps in as many passes of
+                     depth + 1 where that depth is built (3D FP32, depth
+                     4), else end with shallower multi-step passes where
+                     built, else with one-step launches; every split gives
+                     the same bits.                                         */
     int flags;    /* AXS_MARCH_* bits                                       */
 } axs_kernel_config;
 
