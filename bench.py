@@ -12,8 +12,10 @@ before and after.
 
 The JSON line also carries
   roofline     : the step kernel against HBM.  One launch of the default
-                 FP32 kernel advances D time steps (D = 4 at 1024^3) and
-                 reads and writes the state once: its compulsory bytes are
+                 FP32 kernel advances D time steps (D = 4 or 5 at 1024^3:
+                 the library times both in its first long step call, here
+                 the warm-up, and keeps the faster) and reads and writes
+                 the state once: its compulsory bytes are
                  32 B per site per launch (read + write of phi1, phi2, u1,
                  u2 in float).  `achieved` = 32 B x N^3 / average launch
                  duration (HIP events on the library's stream), `frac` =
@@ -48,7 +50,8 @@ HBM_PEAK_GBS = 8000.0
 # steps per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated); each file
 # names the kernel configuration it was measured on
 TRAFFIC_FILES = {1: os.path.join(ROOT, "profiles", "traffic_march_fp32_3d_1024.json"),
-                 4: os.path.join(ROOT, "profiles", "traffic_multi_step_fp32_3d_1024.json")}
+                 4: os.path.join(ROOT, "profiles", "traffic_multi_step_fp32_3d_1024.json"),
+                 5: os.path.join(ROOT, "profiles", "traffic_multi_step_fp32_3d_1024_d5.json")}
 # compulsory bytes per site and launch: the state read once and written once
 PASS_BYTES = {"fp32": 32.0, "fp64": 96.0}
 
@@ -70,7 +73,10 @@ def gpu_leg(np, axs, N, precision, warmup, steps, T0, depth=0):
     lat.tau = 1.0
     if depth:
         try:
-            lat.set_kernel("auto", depth=depth)
+            # multi-step depths keep the context's work-group size
+            flags = lat.kernel_config()["flags"] if depth >= 2 else (
+                axs.AXS_MARCH_PREFETCH | axs.AXS_MARCH_NT_STORE)
+            lat.set_kernel("auto", depth=depth, flags=flags)
         except axs.AxsError as e:           # not available for this form
             log(f"[bench] {precision}: depth {depth} not applied ({e})")
     log(f"[bench] {precision}: {N}^3 initialised, {warmup} warm-up steps")
@@ -569,7 +575,9 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
-    ap.add_argument("--warmup", type=int, default=10)
+    # 20: room for the library's first-call comparison of 4- and 5-step
+    # passes (18 steps) before the clock starts
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--N", type=int, default=1024)
     ap.add_argument("--T-initial", type=float, default=4.0)
     ap.add_argument("--skip-fp64", action="store_true", help="time the FP32 form only")
@@ -760,7 +768,8 @@ def main() -> int:
             "traffic": traffic,
             "basis": "compulsory bytes of one launch: 32 B per site (phi1, phi2, u1, u2 "
                      "read and written once in float) x N^3, over the HIP-event average "
-                     "launch duration",
+                     "launch duration; a launch advances steps_per_launch steps, so at "
+                     "the same time per step a 5-step pass reads lower than a 4-step one",
             "steps_per_launch": spl,
             "launch_ms": launch_ms,
             "site_update_rate": update_rate,

@@ -304,3 +304,32 @@ def test_2d_default_depth(Ns, precision, depth):
     # FP32 2D contexts of 2^25 sites and more start at eight steps per pass
     lat = axs.Lattice(2, members=Ns, precision=precision)
     assert lat.kernel_config()["depth"] == depth
+
+
+def test_depth_probe_keeps_the_bits():
+    # FP32 3D from 2^27 sites: the first step call with room for two 4-step
+    # and two 5-step passes times both and keeps the faster depth per step;
+    # every depth gives the bits of one-step launches, so the state equals
+    # that of a context held at depth 4 by an explicit configuration
+    N = 512
+    F = axs.AXS_MARCH_PREFETCH | axs.AXS_MARCH_NT_STORE | axs.AXS_MARCH_WG1024
+    a = axs.Lattice(3, N, precision="fp32")
+    b = axs.Lattice(3, N, precision="fp32")
+    b.set_kernel("auto", depth=4, flags=F)
+    assert a.kernel_config() == b.kernel_config()
+    for lat in (a, b):
+        lat.init("kinv", seed=5)
+        lat.tau = 1.0
+        lat.step(9)                      # too short to probe
+    assert a.kernel_config()["depth"] == 4
+    for lat in (a, b):
+        lat.step(23)                     # probes: 4 + 5 + 4 + 5 steps, then 5 more
+    assert a.kernel_config()["depth"] in (4, 5)
+    assert b.kernel_config()["depth"] == 4
+    for lat in (a, b):
+        lat.step(11)
+    assert a.tau == b.tau
+    ga = a.get_state(fields=("phi1", "phi2"))
+    gb = b.get_state(fields=("phi1", "phi2"))
+    for k in ga:
+        np.testing.assert_array_equal(ga[k], gb[k], err_msg=k)
