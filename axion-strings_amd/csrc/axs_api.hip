@@ -308,8 +308,6 @@ Context::~Context()
     if (pinned) (void)hipHostFree(pinned);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
-    for (hipEvent_t e : probe_ev)
-        if (e) (void)hipEventDestroy(e);
     for (int h = 0; h < 2; ++h)
         if (stage_ev[h]) (void)hipEventDestroy(stage_ev[h]);
     if (stream) (void)hipStreamDestroy(stream);
@@ -531,13 +529,13 @@ static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ct
         c->tune.depth = 1;
         c->tune.flags &= ~(kTuneWG512 | kTuneWG1024);
     }
-    // FP32 3D from 2^27 sites: the 4-step pass's time per step varies
-    // between lattices and processes (1.73-1.91 ms at 1024^3 on one box),
-    // the 5-step pass's hardly (1.81): the first long enough step call
-    // times both
+    // FP32 3D from 2^29 sites (1024^3): five steps per pass.  At 1024^3 the
+    // 4-step pass ran 1.70-1.98 ms per step over sixteen lattices (slow on
+    // half of them, and turning slow after its first passes), the 5-step
+    // pass 1.78-1.82: 1.6% less on average and never slow.
     if (c->nd == 3 && c->prec == AXS_FP32 && c->p.stencil_setting == 0 &&
-        total_sites(c) >= (1LL << 27) && multi_depth(c) == 4 && multi_depth_for(c, 5) == 5)
-        c->depth_probe = 5;
+        total_sites(c) >= (1LL << 29) && multi_depth(c) == 4 && multi_depth_for(c, 5) == 5)
+        c->tune.depth = 5;
     (void)rc;
     *out = h;
     return AXS_OK;
@@ -751,23 +749,6 @@ static int prepare_passes(Context *c, int64_t nsteps)
     return AXS_OK;
 }
 
-// After probe pass k (1-4) of axs_step: its end event; after the fourth,
-// the time per step of the two depths (passes 1 and 3: D, 2 and 4: P) and
-// the faster one kept for the `left` steps still to run.
-static int probe_pass(Context *c, int k, int D, int P, int64_t left)
-{
-    AXS_HIP_TRY(hipEventRecord(c->probe_ev[k], c->stream));
-    if (k < 4) return AXS_OK;
-    AXS_HIP_TRY(hipEventSynchronize(c->probe_ev[4]));
-    float t[4];
-    for (int i = 0; i < 4; ++i)
-        AXS_HIP_TRY(hipEventElapsedTime(&t[i], c->probe_ev[i], c->probe_ev[i + 1]));
-    const double per_d = (t[0] + t[2]) / (2.0 * D), per_p = (t[1] + t[3]) / (2.0 * P);
-    if (per_p < per_d) c->tune.depth = P;
-    c->depth_probe = 0;
-    return prepare_passes(c, left);
-}
-
 int axs_step(axs_ctx *ctx, int64_t nsteps)
 {
     AXS_TRY(check_member(ctx, 0));
@@ -783,23 +764,11 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
         }
     if (nsteps == 0) return AXS_OK;
     AXS_TRY(prepare_batch(c));
-    const int depth = multi_depth(c);
     const double dt = (double)c->p.time_step;
     // the spectral step's rocFFT calls stay out of graph capture
     const bool graph = (c->tune.flags & kTuneGraph) && nsteps > 1 &&
                        c->p.laplacian == AXS_LAPLACIAN_STENCIL;
-    // the depth probe (Context::depth_probe): passes depth, P, depth, P
-    // first, each between two events; after them the faster depth per step
-    // is kept and the call goes on with it
-    const int P = c->depth_probe;
-    const bool probe = P > 0 && depth >= 1 && !graph && c->step_fail_after < 0 &&
-                       nsteps >= 2 * (depth + P) && multi_depth_for(c, P) == P;
-    if (probe) {
-        for (hipEvent_t &e : c->probe_ev)
-            if (!e) AXS_HIP_TRY(hipEventCreate(&e));
-        AXS_TRY(prepare_multi_batch(c, P));
-    }
-    AXS_TRY(prepare_passes(c, probe ? nsteps - 2 * (depth + P) : nsteps));
+    AXS_TRY(prepare_passes(c, nsteps));
     const double tau0 = c->tau;
     const int cur0 = c->cur;
     hipGraph_t g = nullptr;
@@ -811,20 +780,11 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
         AXS_HIP_TRY(hipEventRecord(c->ev0, c->stream));
     int rc = AXS_OK;
     int launches = 0;      // launches that returned AXS_OK (direct loop)
-    int probes = probe ? 0 : 4;
     for (int64_t n = 0; n < nsteps && rc == AXS_OK;) {
         double tau_next = c->tau + dt;                    // s_integrate.cpp:317
         StepScalars s = make_scalars(c->p, tau_next);
         const int64_t left = nsteps - n;
-        const int d = (probes < 4) ? ((probes & 1) ? P : depth) : next_pass_depth(c, left);
-        if (probes == 0) {
-            const hipError_t e = hipEventRecord(c->probe_ev[0], c->stream);
-            if (e != hipSuccess) {
-                set_error("axs_step: hipEventRecord failed: %s", hipGetErrorString(e));
-                rc = AXS_ERR_HIP;
-                break;
-            }
-        }
+        const int d = next_pass_depth(c, left);
         if (!graph && c->step_fail_after >= 0 && launches >= c->step_fail_after) {
             set_error("axs_step: failure point after %d launches (AXS_STEP_FAIL_AFTER)",
                       launches);
@@ -844,7 +804,6 @@ int axs_step(axs_ctx *ctx, int64_t nsteps)
                 ++launches;
             }
             n += d;
-            if (rc == AXS_OK && probes < 4) rc = probe_pass(c, ++probes, depth, P, nsteps - n);
         } else {
             rc = launch_step(c, s);
             if (rc == AXS_OK) {
@@ -1116,7 +1075,6 @@ int axs_set_kernel_config(axs_ctx *ctx, const axs_kernel_config *cfg)
     c->tune.ci32 = cfg->chunk;
     c->tune.ci64 = cfg->chunk;
     c->tune.depth = depth;
-    c->depth_probe = 0;
     c->tune.flags = cfg->flags;
     return AXS_OK;
 }

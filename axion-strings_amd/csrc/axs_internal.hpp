@@ -151,14 +151,6 @@ struct Context {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t stage_ev[2] = {nullptr, nullptr};   // copy_to_host chunk completions
     double last_step_ms = 0.0;
-    // A depth to time against tune.depth on the first direct step call with
-    // room for two passes of each (0: none pending): the faster per step is
-    // kept.  Set at creation where both are built and their order varies
-    // (FP32 3D, from 2^27 sites); cleared by an explicit or tuned
-    // configuration.  The passes are real steps and every depth gives the
-    // same bits, so only the time depends on the choice.
-    int depth_probe = 0;
-    hipEvent_t probe_ev[5] = {};
     int device = 0;
     int num_cu = 256;
     // failure point for tests of the error paths: axs_step reports an error

@@ -153,7 +153,6 @@ int run_autotune(Context *c, int trials, double *best_ms)
         auto it = g_tune_cache.find(key);
         if (it != g_tune_cache.end()) {
             c->tune = it->second.first;
-            c->depth_probe = 0;
             if (best_ms) *best_ms = it->second.second;
             return AXS_OK;
         }
@@ -294,7 +293,6 @@ int run_autotune(Context *c, int trials, double *best_ms)
         }
     }
     c->tune = best;
-    c->depth_probe = 0;
     c->batch_key = -1;          // tables follow the chosen shape
     invalidate_multi_tables(c);
     {

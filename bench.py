@@ -12,10 +12,8 @@ before and after.
 
 The JSON line also carries
   roofline     : the step kernel against HBM.  One launch of the default
-                 FP32 kernel advances D time steps (D = 4 or 5 at 1024^3:
-                 the library times both in its first long step call, here
-                 the warm-up, and keeps the faster) and reads and writes
-                 the state once: its compulsory bytes are
+                 FP32 kernel advances D time steps (D = 5 at 1024^3) and
+                 reads and writes the state once: its compulsory bytes are
                  32 B per site per launch (read + write of phi1, phi2, u1,
                  u2 in float).  `achieved` = 32 B x N^3 / average launch
                  duration (HIP events on the library's stream), `frac` =
@@ -575,8 +573,6 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
-    # 20: room for the library's first-call comparison of 4- and 5-step
-    # passes (18 steps) before the clock starts
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--N", type=int, default=1024)
     ap.add_argument("--T-initial", type=float, default=4.0)

@@ -290,10 +290,12 @@ def test_2d_fp64_multi_step_bit_exact_vs_oracle(N, depth, chunk, flags, nsteps):
     _assert_same(lat, st)
 
 
-@pytest.mark.parametrize("Ns,depth", [([128], 3), ([192], 4), ([256], 4), ([128] * 4 + [192] * 4, 4)])
+@pytest.mark.parametrize("Ns,depth", [([128], 3), ([192], 4), ([256], 4), ([128] * 4 + [192] * 4, 4),
+                                      ([1024], 5)])
 def test_fp32_default_depth_follows_the_work(Ns, depth):
     # four steps per pass when the four-step tiles still fill the chip in
-    # chunks of 32 planes or more (192^3 and up, ensembles), else three
+    # chunks of 32 planes or more (192^3 and up, ensembles), else three;
+    # five from 2^29 sites (1024^3)
     lat = axs.Lattice(3, members=Ns, precision="fp32")
     assert lat.kernel_config()["depth"] == depth
 
@@ -305,31 +307,3 @@ def test_2d_default_depth(Ns, precision, depth):
     lat = axs.Lattice(2, members=Ns, precision=precision)
     assert lat.kernel_config()["depth"] == depth
 
-
-def test_depth_probe_keeps_the_bits():
-    # FP32 3D from 2^27 sites: the first step call with room for two 4-step
-    # and two 5-step passes times both and keeps the faster depth per step;
-    # every depth gives the bits of one-step launches, so the state equals
-    # that of a context held at depth 4 by an explicit configuration
-    N = 512
-    F = axs.AXS_MARCH_PREFETCH | axs.AXS_MARCH_NT_STORE | axs.AXS_MARCH_WG1024
-    a = axs.Lattice(3, N, precision="fp32")
-    b = axs.Lattice(3, N, precision="fp32")
-    b.set_kernel("auto", depth=4, flags=F)
-    assert a.kernel_config() == b.kernel_config()
-    for lat in (a, b):
-        lat.init("kinv", seed=5)
-        lat.tau = 1.0
-        lat.step(9)                      # too short to probe
-    assert a.kernel_config()["depth"] == 4
-    for lat in (a, b):
-        lat.step(23)                     # probes: 4 + 5 + 4 + 5 steps, then 5 more
-    assert a.kernel_config()["depth"] in (4, 5)
-    assert b.kernel_config()["depth"] == 4
-    for lat in (a, b):
-        lat.step(11)
-    assert a.tau == b.tau
-    ga = a.get_state(fields=("phi1", "phi2"))
-    gb = b.get_state(fields=("phi1", "phi2"))
-    for k in ga:
-        np.testing.assert_array_equal(ga[k], gb[k], err_msg=k)
