@@ -290,9 +290,13 @@ Context::~Context()
     destroy_inverse_plans(this);
     destroy_spec_plans(this);
     for (Member &m : members) {
-        for (int h = 0; h < 2; ++h)
-            for (int a = 0; a < 3; ++a)
-                if (m.state[h][a]) (void)hipFree(m.state[h][a]);
+        if (m.state_block) {
+            (void)hipFree(m.state[0][0]);
+        } else {
+            for (int h = 0; h < 2; ++h)
+                for (int a = 0; a < 3; ++a)
+                    if (m.state[h][a]) (void)hipFree(m.state[h][a]);
+        }
         if (m.aux_ker) (void)hipFree(m.aux_ker);
         if (m.aux_vel) (void)hipFree(m.aux_vel);
     }
@@ -486,18 +490,36 @@ static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ct
         // FP64: (phi, phidot, ker) pairs of doubles; FP32: one float
         // quadruple per site plus float-pair read-back buffers
         size_t bytes = (size_t)m.L * 16;
-        int narr = (c->prec == AXS_FP64) ? 3 : 1;
-        for (int hh = 0; hh < 2; ++hh)
-            for (int a = 0; a < narr; ++a) {
-                if (hipMalloc(&m.state[hh][a], bytes) != hipSuccess) {
-                    set_error("device allocation of %zu bytes failed (member %d)", bytes, r);
-                    return fail(AXS_ERR_OUT_OF_MEMORY);
-                }
-                if (hipMemsetAsync(m.state[hh][a], 0, bytes, c->stream) != hipSuccess) {
-                    set_error("hipMemsetAsync failed");
-                    return fail(AXS_ERR_HIP);
-                }
+        const int narr = (c->prec == AXS_FP64) ? 3 : 1;
+        if (c->prec == AXS_FP32) {
+            // both halves in one allocation (Member::state_block), the
+            // second on a 256-byte boundary
+            const size_t stride = (bytes + 255) & ~(size_t)255;
+            char *blk = nullptr;
+            if (hipMalloc(reinterpret_cast<void **>(&blk), 2 * stride) != hipSuccess) {
+                set_error("device allocation of %zu bytes failed (member %d)", 2 * stride, r);
+                return fail(AXS_ERR_OUT_OF_MEMORY);
             }
+            m.state_block = true;
+            m.state[0][0] = blk;
+            m.state[1][0] = blk + stride;
+            if (hipMemsetAsync(blk, 0, 2 * stride, c->stream) != hipSuccess) {
+                set_error("hipMemsetAsync failed");
+                return fail(AXS_ERR_HIP);
+            }
+        } else {
+            for (int hh = 0; hh < 2; ++hh)
+                for (int a = 0; a < narr; ++a) {
+                    if (hipMalloc(&m.state[hh][a], bytes) != hipSuccess) {
+                        set_error("device allocation of %zu bytes failed (member %d)", bytes, r);
+                        return fail(AXS_ERR_OUT_OF_MEMORY);
+                    }
+                    if (hipMemsetAsync(m.state[hh][a], 0, bytes, c->stream) != hipSuccess) {
+                        set_error("hipMemsetAsync failed");
+                        return fail(AXS_ERR_HIP);
+                    }
+                }
+        }
         if (c->prec == AXS_FP32) {
             size_t pb = (size_t)m.L * 8;
             if (hipMalloc(&m.aux_ker, pb) != hipSuccess || hipMalloc(&m.aux_vel, pb) != hipSuccess) {
@@ -529,13 +551,6 @@ static int create_impl(const axs_params *p, int n_members, const int *Ns, axs_ct
         c->tune.depth = 1;
         c->tune.flags &= ~(kTuneWG512 | kTuneWG1024);
     }
-    // FP32 3D from 2^29 sites (1024^3): five steps per pass.  At 1024^3 the
-    // 4-step pass ran 1.70-1.98 ms per step over sixteen lattices (slow on
-    // half of them, and turning slow after its first passes), the 5-step
-    // pass 1.78-1.82: 1.6% less on average and never slow.
-    if (c->nd == 3 && c->prec == AXS_FP32 && c->p.stencil_setting == 0 &&
-        total_sites(c) >= (1LL << 29) && multi_depth(c) == 4 && multi_depth_for(c, 5) == 5)
-        c->tune.depth = 5;
     (void)rc;
     *out = h;
     return AXS_OK;

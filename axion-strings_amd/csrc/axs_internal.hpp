@@ -85,6 +85,12 @@ struct Member {
     int N = 0;
     int64_t L = 0;
     void *state[2][3] = {{nullptr}};   // ping-pong halves
+    // FP32: both halves in one allocation at state[0][0] (the second at L 16
+    // bytes rounded up to 256): with separate allocations the halves'
+    // relative placement in memory changed the 1024^3 four-step pass
+    // between 1.70 and 1.98 ms per step from lattice to lattice; in one
+    // allocation it ran 1.72-1.75 on 13 lattices of 14
+    bool state_block = false;
     // FP32 only: (ker1, ker2) of the current state while ker_valid, and the
     // (phidot1, phidot2) read-back buffer, float pairs
     void *aux_ker = nullptr;

@@ -12,7 +12,7 @@ before and after.
 
 The JSON line also carries
   roofline     : the step kernel against HBM.  One launch of the default
-                 FP32 kernel advances D time steps (D = 5 at 1024^3) and
+                 FP32 kernel advances D time steps (D = 4 at 1024^3) and
                  reads and writes the state once: its compulsory bytes are
                  32 B per site per launch (read + write of phi1, phi2, u1,
                  u2 in float).  `achieved` = 32 B x N^3 / average launch

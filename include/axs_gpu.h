@@ -272,7 +272,7 @@ typedef struct axs_kernel_config {
                      (a new context starts at 4
                      when its lattices fill the chip in 32-plane chunks --
                      192^3 and up, ensembles -- else 3, with
-                     AXS_MARCH_WG1024; 5 from 2^29 sites, 1024^3), 3D FP64 2 or 3 with AXS_MARCH_WG512
+                     AXS_MARCH_WG1024), 3D FP64 2 or 3 with AXS_MARCH_WG512
                      (starts at 3), 2D 2, 3, 4, 6 or 8 (starts at 6, FP32
                      from 2^25 sites at 8; rows 0, no work-group flags).  stencil_setting 1-3 (sides
                      >= 64, kernel not SIMPLE): the multi-step kernels

@@ -291,11 +291,10 @@ def test_2d_fp64_multi_step_bit_exact_vs_oracle(N, depth, chunk, flags, nsteps):
 
 
 @pytest.mark.parametrize("Ns,depth", [([128], 3), ([192], 4), ([256], 4), ([128] * 4 + [192] * 4, 4),
-                                      ([1024], 5)])
+                                      ([1024], 4)])
 def test_fp32_default_depth_follows_the_work(Ns, depth):
     # four steps per pass when the four-step tiles still fill the chip in
-    # chunks of 32 planes or more (192^3 and up, ensembles), else three;
-    # five from 2^29 sites (1024^3)
+    # chunks of 32 planes or more (192^3 and up, ensembles), else three
     lat = axs.Lattice(3, members=Ns, precision="fp32")
     assert lat.kernel_config()["depth"] == depth
 
