@@ -758,7 +758,12 @@ static int prepare_passes(Context *c, int64_t nsteps)
             AXS_TRY(prepare_multi_batch(c, d));
             ready[d] = true;
         }
-        if (d == depth && left % depth == 0) break;     // only depth-D passes left
+        // once a depth-D pass comes, depth-D passes follow down to the
+        // remainder (next_pass_depth's deeper passes need more steps left)
+        if (d == depth) {
+            left %= depth;
+            continue;
+        }
         left -= (d >= 1) ? d : 1;
     }
     return AXS_OK;
